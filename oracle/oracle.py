@@ -1,0 +1,457 @@
+"""ctypes view of the FP64 CPU oracle (oracle/libvmc_oracle.so) and of the
+reference-as-checker library (oracle/_ref/libvmc_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker and by bench.py's cpu_baseline leg.  The product path
+(variational-monte-carlo_amd) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libvmc_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libvmc_ref.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_u8p = C.POINTER(C.c_ubyte)
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class System(C.Structure):
+    """Mirror of orc_system (vmc_oracle.h)."""
+
+    _fields_ = [
+        ("n_up", C.c_int),
+        ("n_dn", C.c_int),
+        ("L", C.c_double),
+        ("a_same", C.c_double),
+        ("a_anti", C.c_double),
+        ("b", C.c_double),
+        ("alpha", C.c_double),
+        ("recip_nmax2", C.c_int),
+        ("rcut", C.c_double),
+        ("erfc_kind", C.c_int),
+    ]
+
+
+def reference_system(n, L, a=0.25, b=1.0):
+    """Polarised single determinant with the reference's Ewald rules."""
+    return System(n, 0, L, a, a, b, 0.0, 0, 0.0, 0)
+
+
+def baseline_system(n_total, rs, b, a_same=0.25, a_anti=0.5, nmax2=25, erfc_exact=True):
+    """Unpolarised two-determinant system with alpha = 5/L, rcut = L/2."""
+    L = box_length_from_rs(rs, n_total)
+    return System(n_total // 2, n_total - n_total // 2, L, a_same, a_anti, b, 5.0 / L, nmax2,
+                  0.5 * L, 1 if erfc_exact else 0)
+
+
+def box_length_from_rs(rs, n):
+    """L = (4 pi N / 3)^(1/3) r_s (tests/test_utilities.hpp:127-129)."""
+    return float(np.cbrt(4.0 * np.pi * n / 3.0) * rs)
+
+
+class RefConfig(C.Structure):
+    _fields_ = [
+        ("num_particles", C.c_size_t),
+        ("box_length", C.c_double),
+        ("warmup_steps", C.c_size_t),
+        ("measure_steps", C.c_size_t),
+        ("step_size", C.c_double),
+        ("seed", C.c_uint64),
+        ("block_size", C.c_size_t),
+        ("jastrow_a", C.c_double),
+        ("jastrow_b", C.c_double),
+    ]
+
+
+class RefSummary(C.Structure):
+    _fields_ = [
+        ("mean_energy", C.c_double),
+        ("standard_error", C.c_double),
+        ("has_standard_error", C.c_int),
+        ("acceptance_rate", C.c_double),
+        ("accepted", C.c_size_t),
+        ("proposed", C.c_size_t),
+        ("final_step_size", C.c_double),
+    ]
+
+
+class SweepConfig(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("move_kind", C.c_int), ("step0", C.c_double)]
+
+
+class MT64(C.Structure):
+    _fields_ = [("mt", C.c_uint64 * 312), ("idx", C.c_int)]
+
+
+class Blocking(C.Structure):
+    _fields_ = [
+        ("num_blocks", C.c_size_t),
+        ("running_mean", C.c_double),
+        ("running_m2", C.c_double),
+        ("block_size", C.c_size_t),
+        ("in_block", C.c_size_t),
+        ("block_sum", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run make -C oracle)")
+        L = C.CDLL(ORACLE_SO)
+        W = C.c_void_p
+        sp = C.POINTER(System)
+        sig = {
+            "orc_mt64_seed": (None, [C.POINTER(MT64), C.c_uint64]),
+            "orc_mt64_next": (C.c_uint64, [C.POINTER(MT64)]),
+            "orc_std_canonical": (C.c_double, [C.POINTER(MT64)]),
+            "orc_std_uniform_real": (C.c_double, [C.POINTER(MT64), C.c_double, C.c_double]),
+            "orc_std_uniform_index": (C.c_uint64, [C.POINTER(MT64), C.c_uint64]),
+            "orc_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_uint32)]),
+            "orc_shell_fill": (C.c_int, [C.c_int, _ip, _ip, _u8p]),
+            "orc_ewald_gvectors": (C.c_int, [sp, _dp, _dp, _dp, _dp, _ip, C.c_int, _dp]),
+            "orc_ewald_self": (C.c_double, [sp]),
+            "orc_ewald_background": (C.c_double, [sp]),
+            "orc_walker_new": (W, [sp]),
+            "orc_walker_free": (None, [W]),
+            "orc_walker_num_electrons": (C.c_int, [W]),
+            "orc_walker_num_g": (C.c_int, [W]),
+            "orc_walker_num_k": (C.c_int, [W, C.c_int]),
+            "orc_walker_set_positions": (C.c_int, [W, _dp]),
+            "orc_walker_get_positions": (None, [W, _dp]),
+            "orc_walker_logdet": (C.c_double, [W, C.c_int]),
+            "orc_walker_inverse": (None, [W, C.c_int, _dp]),
+            "orc_walker_dmatrix": (None, [W, C.c_int, _dp]),
+            "orc_walker_jastrow": (C.c_double, [W]),
+            "orc_walker_vreal": (C.c_double, [W]),
+            "orc_walker_vrecip": (C.c_double, [W]),
+            "orc_walker_structure_factor": (None, [W, _dp, _dp]),
+            "orc_walker_derivatives": (None, [W, _dp, _dp]),
+            "orc_walker_kinetic": (C.c_double, [W]),
+            "orc_walker_local_energy": (C.c_double, [W, _dp, _dp]),
+            "orc_walker_ratio": (C.c_double, [W, C.c_int, _dp]),
+            "orc_walker_delta_j": (C.c_double, [W, C.c_int, _dp]),
+            "orc_walker_accept": (None, [W, C.c_int, _dp]),
+            "orc_walker_derivatives_after_step": (None, [W, C.c_int, C.c_int, _dp, _dp, _dp]),
+            "orc_blocking_init": (None, [C.POINTER(Blocking), C.c_size_t]),
+            "orc_blocking_add": (None, [C.POINTER(Blocking), C.c_double]),
+            "orc_blocking_ready": (C.c_int, [C.POINTER(Blocking)]),
+            "orc_blocking_mean_se": (C.c_int, [C.POINTER(Blocking), _dp, _dp]),
+            "orc_ref_simulation_run": (C.c_int, [C.POINTER(RefConfig), C.POINTER(RefSummary),
+                                                 _dp, _dp, _dp, _dp, _dp]),
+            "orc_init_positions": (C.c_int, [sp, C.c_uint64, C.c_uint64, _dp]),
+            "orc_sweep": (C.c_int, [W, C.POINTER(SweepConfig), C.c_uint64, C.c_uint64, _dp,
+                                    C.c_int, _dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def shell_fill(n):
+    nvec = np.zeros(3 * max(n, 1), dtype=np.int32)
+    orbk = np.zeros(max(n, 1), dtype=np.int32)
+    orbt = np.zeros(max(n, 1), dtype=np.uint8)
+    nk = lib().orc_shell_fill(n, _ptr(nvec, _ip), _ptr(orbk, _ip), _ptr(orbt, _u8p))
+    return nvec[: 3 * nk].reshape(nk, 3), orbk[:n], orbt[:n]
+
+
+def gvectors(sysd):
+    l = lib()
+    n = l.orc_ewald_gvectors(C.byref(sysd), None, None, None, None, None, 0, None)
+    gx, gy, gz, w = (np.zeros(n) for _ in range(4))
+    nv = np.zeros(3 * n, dtype=np.int32)
+    alpha = C.c_double()
+    l.orc_ewald_gvectors(C.byref(sysd), _ptr(gx), _ptr(gy), _ptr(gz), _ptr(w), _ptr(nv, _ip), n,
+                         C.byref(alpha))
+    return np.stack([gx, gy, gz], 1), w, nv.reshape(n, 3), alpha.value
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+class Walker:
+    """One configuration with the reference's cached state."""
+
+    def __init__(self, sysd: System, pos=None):
+        self.sys = sysd
+        self._l = lib()
+        self._w = self._l.orc_walker_new(C.byref(sysd))
+        self.N = self._l.orc_walker_num_electrons(self._w)
+        self.ng = self._l.orc_walker_num_g(self._w)
+        self.singular = None
+        if pos is not None:
+            self.set_positions(pos)
+
+    def __del__(self):
+        if getattr(self, "_w", None):
+            self._l.orc_walker_free(self._w)
+            self._w = None
+
+    def det_sizes(self):
+        return [self.sys.n_up] + ([self.sys.n_dn] if self.sys.n_dn > 0 else [])
+
+    def set_positions(self, pos):
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.N, 3)
+        self.singular = bool(self._l.orc_walker_set_positions(self._w, _ptr(p)))
+        return self.singular
+
+    def positions(self):
+        p = np.zeros((self.N, 3))
+        self._l.orc_walker_get_positions(self._w, _ptr(p))
+        return p
+
+    def logdet(self, spin=0):
+        return self._l.orc_walker_logdet(self._w, spin)
+
+    def inverse(self, spin=0):
+        n = self.det_sizes()[spin]
+        a = np.zeros((n, n))
+        self._l.orc_walker_inverse(self._w, spin, _ptr(a))
+        return a
+
+    def dmatrix(self, spin=0):
+        n = self.det_sizes()[spin]
+        a = np.zeros((n, n))
+        self._l.orc_walker_dmatrix(self._w, spin, _ptr(a))
+        return a
+
+    def jastrow(self):
+        return self._l.orc_walker_jastrow(self._w)
+
+    def vreal(self):
+        return self._l.orc_walker_vreal(self._w)
+
+    def vrecip(self):
+        return self._l.orc_walker_vrecip(self._w)
+
+    def structure_factor(self):
+        re, im = np.zeros(self.ng), np.zeros(self.ng)
+        self._l.orc_walker_structure_factor(self._w, _ptr(re), _ptr(im))
+        return re, im
+
+    def derivatives(self):
+        g, lp = np.zeros((self.N, 3)), np.zeros(self.N)
+        self._l.orc_walker_derivatives(self._w, _ptr(g), _ptr(lp))
+        return g, lp
+
+    def kinetic(self):
+        return self._l.orc_walker_kinetic(self._w)
+
+    def local_energy(self):
+        t, v = C.c_double(), C.c_double()
+        e = self._l.orc_walker_local_energy(self._w, C.byref(t), C.byref(v))
+        return e, t.value, v.value
+
+    def ratio(self, p, newpos):
+        q = np.ascontiguousarray(newpos, dtype=np.float64)
+        return self._l.orc_walker_ratio(self._w, p, _ptr(q))
+
+    def delta_j(self, p, newpos):
+        q = np.ascontiguousarray(newpos, dtype=np.float64)
+        return self._l.orc_walker_delta_j(self._w, p, _ptr(q))
+
+    def accept(self, p, newpos):
+        q = np.ascontiguousarray(newpos, dtype=np.float64)
+        self._l.orc_walker_accept(self._w, p, _ptr(q))
+
+    def derivatives_after_step(self, accepted, p, oldpos):
+        o = np.ascontiguousarray(oldpos, dtype=np.float64)
+        g, lp = np.zeros((self.N, 3)), np.zeros(self.N)
+        self._l.orc_walker_derivatives_after_step(self._w, int(accepted), p, _ptr(o), _ptr(g),
+                                                  _ptr(lp))
+        return g, lp
+
+    def sweep(self, seed, walker, sweep_index, step, move_kind=1, adapt=False, measure=True):
+        cfg = SweepConfig(seed, move_kind, step)
+        st = C.c_double(step)
+        e = C.c_double(0.0)
+        acc = self._l.orc_sweep(self._w, C.byref(cfg), walker, sweep_index, C.byref(st),
+                                int(adapt), C.byref(e) if measure else None)
+        return acc, st.value, (e.value if measure else None)
+
+
+def init_positions(sysd, seed, walker):
+    n = sysd.n_up + sysd.n_dn
+    pos = np.zeros((n, 3))
+    att = lib().orc_init_positions(C.byref(sysd), seed, walker, _ptr(pos))
+    return pos, att
+
+
+def oracle_simulation_run(n, L, warmup_steps, measure_steps, step, seed, block, a=0.25, b=1.0,
+                          want_pos=False):
+    cfg = RefConfig(n, L, warmup_steps, measure_steps, step, seed, block, a, b)
+    out = RefSummary()
+    fe, fm, fs, fa = (np.zeros(max(measure_steps, 1)) for _ in range(4))
+    fp = np.zeros(max(measure_steps, 1) * n * 3) if want_pos else None
+    rc = lib().orc_ref_simulation_run(C.byref(cfg), C.byref(out), _ptr(fe), _ptr(fm), _ptr(fs),
+                                      _ptr(fa), _ptr(fp))
+    if rc:
+        raise RuntimeError("oracle simulation failed to initialise")
+    return out, fe[:measure_steps], fm[:measure_steps], fs[:measure_steps], fa[:measure_steps], fp
+
+
+class MT19937_64:
+    def __init__(self, seed):
+        self.s = MT64()
+        lib().orc_mt64_seed(C.byref(self.s), seed)
+
+    def next(self):
+        return lib().orc_mt64_next(C.byref(self.s))
+
+    def canonical(self):
+        return lib().orc_std_canonical(C.byref(self.s))
+
+    def uniform(self, a, b):
+        return lib().orc_std_uniform_real(C.byref(self.s), a, b)
+
+    def index(self, n):
+        return lib().orc_std_uniform_index(C.byref(self.s), n)
+
+
+def blocking(block_size, xs):
+    b = Blocking()
+    lib().orc_blocking_init(C.byref(b), block_size)
+    for x in xs:
+        lib().orc_blocking_add(C.byref(b), float(x))
+    m, s = C.c_double(), C.c_double()
+    rc = lib().orc_blocking_mean_se(C.byref(b), C.byref(m), C.byref(s))
+    return (None, None) if rc else (m.value, s.value)
+
+
+# ----------------------------------------------------------- reference build
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        R = C.CDLL(REF_SO)
+        R.ref_slater_eval.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _u8p]
+        R.ref_slater_move.argtypes = [C.c_int, C.c_double, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp,
+                                      _dp]
+        R.ref_jastrow.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, C.c_int, _dp,
+                                  _dp, _dp, _dp, _dp, _dp, _dp]
+        R.ref_energy.argtypes = [C.c_int, C.c_double, _dp, C.c_int, _dp, _ip, _dp, _dp, _dp, _dp,
+                                 _dp, _dp, _dp, _dp]
+        R.ref_local_energy.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp,
+                                       _dp, _dp]
+        R.ref_simulation_run.argtypes = [C.c_size_t, C.c_double, C.c_size_t, C.c_size_t,
+                                         C.c_double, C.c_uint64, C.c_size_t, C.c_double,
+                                         C.c_double, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _dp]
+        R.ref_blocking.argtypes = [C.c_size_t, _dp, C.c_size_t, _dp, _dp]
+        _ref = R
+    return _ref
+
+
+@dataclass
+class RefSlater:
+    logdet: float
+    D: np.ndarray
+    inv: np.ndarray
+    nvec: np.ndarray
+    orbk: np.ndarray
+    orbt: np.ndarray
+
+
+def ref_slater_eval(pos, L):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n = pos.shape[0]
+    ld = C.c_double()
+    D, inv = np.zeros((n, n)), np.zeros((n, n))
+    nk = C.c_int()
+    nv = np.zeros(3 * n, dtype=np.int32)
+    ok, ot = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.uint8)
+    ref().ref_slater_eval(n, L, _ptr(pos), C.byref(ld), _ptr(D), _ptr(inv), C.byref(nk),
+                          _ptr(nv, _ip), _ptr(ok, _ip), _ptr(ot, _u8p))
+    return RefSlater(ld.value, D, inv, nv[: 3 * nk.value].reshape(-1, 3), ok, ot)
+
+
+def ref_slater_move(pos, L, p, newpos):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n = pos.shape[0]
+    q = np.ascontiguousarray(newpos, dtype=np.float64)
+    r = C.c_double()
+    inv, D = np.zeros((n, n)), np.zeros((n, n))
+    g, lp = np.zeros((n, 3)), np.zeros(n)
+    ref().ref_slater_move(n, L, _ptr(pos), p, _ptr(q), C.byref(r), _ptr(inv), _ptr(D), _ptr(g),
+                          _ptr(lp))
+    return r.value, inv, D, g, lp
+
+
+def ref_jastrow(pos, L, a, b, p=-1, newpos=None):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n = pos.shape[0]
+    q = np.ascontiguousarray(newpos if newpos is not None else np.zeros(3), dtype=np.float64)
+    v, d = C.c_double(), C.c_double()
+    g, lp, g2, lp2 = np.zeros((n, 3)), np.zeros(n), np.zeros((n, 3)), np.zeros(n)
+    ref().ref_jastrow(n, L, a, b, _ptr(pos), p, _ptr(q), C.byref(v), _ptr(g), _ptr(lp),
+                      C.byref(d), _ptr(g2), _ptr(lp2))
+    return v.value, g, lp, d.value, g2, lp2
+
+
+def ref_energy(pos, L, p=-1, newpos=None):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n = pos.shape[0]
+    q = np.ascontiguousarray(newpos if newpos is not None else np.zeros(3), dtype=np.float64)
+    ng = C.c_int()
+    vals = [C.c_double() for _ in range(6)]
+    sre, sim = np.zeros(4096), np.zeros(4096)
+    ref().ref_energy(n, L, _ptr(pos), p, _ptr(q), C.byref(ng), C.byref(vals[0]), C.byref(vals[1]),
+                     C.byref(vals[2]), C.byref(vals[3]), _ptr(sre), _ptr(sim), C.byref(vals[4]),
+                     C.byref(vals[5]))
+    k = ng.value
+    return dict(ng=k, vreal=vals[0].value, vrecip=vals[1].value, self=vals[2].value,
+                bg=vals[3].value, s_re=sre[:k].copy(), s_im=sim[:k].copy(),
+                vreal_after=vals[4].value, vrecip_after=vals[5].value)
+
+
+def ref_local_energy(pos, L, a=0.25, b=1.0):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n = pos.shape[0]
+    lp_, e = C.c_double(), C.c_double()
+    g, lp = np.zeros((n, 3)), np.zeros(n)
+    ref().ref_local_energy(n, L, a, b, _ptr(pos), C.byref(lp_), _ptr(g), _ptr(lp), C.byref(e))
+    return lp_.value, g, lp, e.value
+
+
+def ref_simulation_run(n, L, warmup_steps, measure_steps, step, seed, block, a=0.25, b=1.0,
+                       want_pos=False):
+    m, s, a_ = C.c_double(), C.c_double(), C.c_double()
+    hs = C.c_int()
+    fe, fm, fs, fa = (np.zeros(max(measure_steps, 1)) for _ in range(4))
+    fp = np.zeros(max(measure_steps, 1) * n * 3) if want_pos else None
+    ref().ref_simulation_run(n, L, warmup_steps, measure_steps, step, seed, block, a, b,
+                             C.byref(m), C.byref(s), C.byref(hs), C.byref(a_), _ptr(fe), _ptr(fm),
+                             _ptr(fs), _ptr(fa), _ptr(fp))
+    return dict(mean=m.value, se=s.value, has_se=bool(hs.value), acc=a_.value,
+                frames_e=fe[:measure_steps], frames_mean=fm[:measure_steps],
+                frames_se=fs[:measure_steps], frames_acc=fa[:measure_steps], frames_pos=fp)
