@@ -1,0 +1,186 @@
+/*
+ * vmc_hip.h -- C ABI of the MI355X-native VMC sweep for the 3D homogeneous
+ * electron gas (libvmc_hip.so, gfx950).
+ *
+ * The reference has no FFI: its hot path is in-process C++ classes
+ * (SlaterPlaneWave, JastrowPade, EnergyTracker, WaveFunction, Simulation in
+ * src/).  This header is the drop-in boundary that replaces them.  Each entry
+ * point names the reference interface it stands in for (file:line, paths
+ * relative to the reference repository root).  The C++ facade in
+ * variational-monte-carlo_amd/host/ restores the reference's class
+ * signatures above it.
+ *
+ * Conventions
+ *  - Every function returns VMC_OK (0) or a VMC_ERR_* code; the message of
+ *    the last failure on the calling thread is vmc_last_error().  No C++
+ *    exception crosses this boundary.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *    Functions that take or return host buffers synchronise that stream
+ *    before returning; the others are stream-ordered.
+ *  - Positions are AoS [walker][electron][3] doubles in [0, L).
+ *  - An ensemble is not thread-safe: one host thread per handle, or
+ *    external synchronisation.
+ */
+#ifndef VMC_HIP_H
+#define VMC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VMC_OK = 0,
+    VMC_ERR_INVALID = 1,  /* bad argument (reference: std::invalid_argument, config.hpp:64-75) */
+    VMC_ERR_DEVICE = 2,   /* HIP runtime failure */
+    VMC_ERR_SINGULAR = 3, /* no non-singular start found (reference: std::runtime_error,
+                             simulation.cpp:68-70) */
+    VMC_ERR_MEMORY = 4,   /* device allocation failed */
+    VMC_ERR_STATE = 5     /* call out of order */
+};
+
+enum { VMC_PREC_FP64 = 0, VMC_PREC_MIXED = 1 };
+enum { VMC_MOVE_UNIFORM = 0, VMC_MOVE_GAUSSIAN = 1 };
+enum { VMC_ERFC_AS7126 = 0, VMC_ERFC_EXACT = 1 };
+
+/* Sweep flags. */
+enum { VMC_SWEEP_MEASURE = 1, VMC_SWEEP_ADAPT = 2 };
+
+/* One HEG system.  Zero/negative optional fields select the reference's own
+ * rules, so {n_up=N, n_dn=0, L, a, a, b, 0, 0, 0, AS7126, UNIFORM, FP64, 0}
+ * is exactly the reference's physics (slater_plane_wave.cpp:30-38,
+ * jastrow_pade.hpp:10-18, energy_tracking.cpp:6-70). */
+typedef struct vmc_system {
+    int32_t n_up;              /* electrons of determinant 0 (all, when n_dn == 0) */
+    int32_t n_dn;              /* electrons of determinant 1; 0 = polarised, one determinant */
+    double box_length;         /* L (reference Config::box_length, config.hpp:20) */
+    double jastrow_a_same;     /* Pade a for parallel spins (reference Jastrow_A) */
+    double jastrow_a_anti;     /* Pade a for antiparallel spins */
+    double jastrow_b;          /* Pade b (reference Jastrow_B) */
+    double ewald_alpha;        /* <= 0: reference 6/L (energy_tracking.cpp:7) */
+    int32_t recip_nmax2;       /* > 0: G = 2 pi n / L with 0 < |n|^2 <= recip_nmax2, half space;
+                                  0: reference rule G^2 <= 4 alpha^2 ln(1e6) (energy_tracking.cpp:13-21) */
+    double real_cutoff;        /* > 0: spherical real-space cutoff; <= 0: every minimum-image pair */
+    int32_t erfc_kind;         /* VMC_ERFC_*; AS7126 is energy_tracking.cpp:101-139 */
+    int32_t move_kind;         /* VMC_MOVE_*; UNIFORM is simulation.cpp:101-103 */
+    int32_t precision;         /* VMC_PREC_* */
+    int32_t reinvert_interval; /* accepted moves between FP64 re-inversions; 0 = never
+                                  (the reference never re-inverts) */
+} vmc_system;
+
+/* Per-walker FP64 statistics, vmc_ensemble_read_stats() layout (doubles). */
+enum {
+    VMC_STAT_NMEAS = 0,    /* measured sweeps */
+    VMC_STAT_SUM_E = 1,    /* sum of local energies */
+    VMC_STAT_SUM_E2 = 2,   /* sum of squared local energies */
+    VMC_STAT_SUM_KIN = 3,  /* sum of kinetic parts */
+    VMC_STAT_SUM_POT = 4,  /* sum of potential parts */
+    VMC_STAT_LAST_E = 5,   /* last local energy */
+    VMC_STAT_ACCEPTED = 6, /* accepted moves since the last reset */
+    VMC_STAT_PROPOSED = 7, /* proposed moves since the last reset */
+    VMC_STAT_BLK_SUM = 8,  /* blocking (blocking_analysis.cpp:17-39): open block sum */
+    VMC_STAT_BLK_IN = 9,   /* samples in the open block */
+    VMC_STAT_BLK_N = 10,   /* completed blocks */
+    VMC_STAT_BLK_MEAN = 11,/* Welford mean of block means */
+    VMC_STAT_BLK_M2 = 12,  /* Welford M2 of block means */
+    VMC_STAT_LAST_KIN = 13,
+    VMC_STAT_LAST_POT = 14,
+    VMC_STAT_REINV = 15,   /* FP64 re-inversions performed */
+    VMC_STAT_COUNT = 16
+};
+
+/* Scalars returned per configuration by vmc_frozen_eval(). */
+enum {
+    VMC_FZ_LOGDET0 = 0,     /* log|det D0|   (slater_plane_wave.cpp:202-318) */
+    VMC_FZ_LOGDET1 = 1,     /* log|det D1|   (0 when polarised) */
+    VMC_FZ_JASTROW = 2,     /* J             (jastrow_pade.cpp:7-55) */
+    VMC_FZ_KINETIC = 3,     /* -1/2 sum(lap + |grad|^2) (energy_tracking.cpp:253-278) */
+    VMC_FZ_VREAL = 4,       /* real-space Ewald  (energy_tracking.cpp:93-144) */
+    VMC_FZ_VRECIP = 5,      /* reciprocal Ewald  (energy_tracking.cpp:72-91,146-191) */
+    VMC_FZ_ELOCAL = 6,      /* kinetic + potential (energy_tracking.cpp:367-382) */
+    VMC_FZ_RATIO = 7,       /* determinant ratio of the move (slater_plane_wave.cpp:351-365) */
+    VMC_FZ_DELTA_J = 8,     /* Jastrow change of the move (jastrow_pade.cpp:140-200) */
+    VMC_FZ_VREAL_INC = 9,   /* V_real after the move, incremental (energy_tracking.cpp:280-365) */
+    VMC_FZ_VRECIP_INC = 10, /* V_recip after the move, incremental (energy_tracking.cpp:193-251) */
+    VMC_FZ_ELOCAL_AFTER = 11, /* local energy after the move, full re-evaluation */
+    VMC_FZ_KINETIC_AFTER = 12,
+    VMC_FZ_JASTROW_AFTER = 13, /* J + delta_J tracked incrementally */
+    VMC_FZ_SINGULAR = 14,   /* 1 if a determinant was singular */
+    VMC_FZ_COUNT = 15
+};
+
+typedef struct vmc_ensemble vmc_ensemble;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *vmc_last_error(void);
+
+/* Library and device sanity: returns VMC_OK when a gfx950 device is usable. */
+int vmc_device_check(int32_t *device_count);
+
+/* Derived sizes of a system: unique k per determinant (n_k[2]), half-space G
+ * count, inverse elements per walker (n_up^2 + n_dn^2). */
+int vmc_system_sizes(const vmc_system *sys, int32_t *n_k, int32_t *n_g, int32_t *n_inv);
+
+/* Ensemble of independent walkers resident on the current device.  Replaces
+ * one Simulation per std::async thread (simulation.hpp:77-85, main.cpp:50-68)
+ * and its Particles/SlaterPlaneWave/JastrowPade/EnergyTracker state
+ * (particles.hpp:10-75, slater_plane_wave.hpp:147-169, jastrow_pade.hpp:5-44,
+ * energy_tracking.hpp:10-93).  `step0` is the initial proposal width (uniform
+ * half-width or Gaussian sigma; reference default L/10, config.hpp:78). */
+int vmc_ensemble_create(const vmc_system *sys, int64_t n_walkers, uint64_t seed, double step0,
+                        vmc_ensemble **out);
+int vmc_ensemble_destroy(vmc_ensemble *e);
+
+/* Uniform random start with the singular-retry loop of
+ * Simulation::initialize_positions (simulation.cpp:43-76): up to 100 Philox
+ * draws per walker; VMC_ERR_SINGULAR if a walker found none. */
+int vmc_ensemble_init_positions(vmc_ensemble *e, void *stream);
+
+/* Host positions in/out ([W][N][3]).  set_positions rebuilds every inverse in
+ * FP64 (the log_abs_det path, slater_plane_wave.cpp:202-318). */
+int vmc_ensemble_set_positions(vmc_ensemble *e, const double *pos, void *stream);
+int vmc_ensemble_get_positions(vmc_ensemble *e, double *pos, void *stream);
+
+/* n_sweeps sequential sweeps (every electron proposed once per sweep, in
+ * index order) of every walker in one launch.  Replaces the warmup/measure
+ * loops over Simulation::metropolis_step (simulation.cpp:80-263).
+ * VMC_SWEEP_MEASURE evaluates the local energy after each sweep and feeds the
+ * per-walker accumulators and blocking (block_size measured sweeps per
+ * block); VMC_SWEEP_ADAPT applies the warmup step rule
+ * s *= exp(0.25 (acc - 0.5)), s <= L/2 after each sweep
+ * (simulation.cpp:178-194). */
+int vmc_ensemble_sweep(vmc_ensemble *e, int32_t n_sweeps, int32_t flags, int32_t block_size,
+                       void *stream);
+
+/* Zero the per-walker statistics (start of a measurement phase). */
+int vmc_ensemble_reset_stats(vmc_ensemble *e, void *stream);
+/* Copy per-walker statistics to host: [W][VMC_STAT_COUNT]. */
+int vmc_ensemble_read_stats(vmc_ensemble *e, double *stats, void *stream);
+/* Copy per-walker step sizes to host: [W]. */
+int vmc_ensemble_read_steps(vmc_ensemble *e, double *steps, void *stream);
+/* Global sweep counter (the Philox stream position) and walker count. */
+int vmc_ensemble_info(const vmc_ensemble *e, int64_t *n_walkers, uint64_t *sweeps_done);
+
+/* Frozen-configuration evaluation for parity: for each configuration c,
+ * builds D and its FP64 inverse, evaluates every energy term, proposes the
+ * move of electron moved[c] to newpos[c] (ratio, delta J), applies it
+ * unconditionally (Sherman-Morrison, incremental S_G and V_real) and
+ * re-evaluates.  Uses the same device routines as the sweep, in the
+ * system's precision.  Host buffers:
+ *   pos [C][N][3], moved [C], newpos [C][3]        (inputs)
+ *   scalars [C][VMC_FZ_COUNT]                       (required)
+ *   grad_lap [C][N][4]   grad x,y,z and lap of ln psi before the move (optional)
+ *   inv_before, inv_after [C][n_inv] transposed inverses
+ *                        (invT[k][j] = (D^-1)[j][k], slater_plane_wave.cpp:312-314) (optional)
+ *   s_k [C][n_g][2]      structure factor before the move (optional)        */
+int vmc_frozen_eval(const vmc_system *sys, int32_t n_configs, const double *pos,
+                    const int32_t *moved, const double *newpos, double *scalars, double *grad_lap,
+                    double *inv_before, double *inv_after, double *s_k, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,159 @@
+// vmc_device.hpp -- wavefront-level building blocks of the gfx950 VMC sweep.
+//
+// One wavefront (64 lanes) owns one walker.  Lanes stride over electrons,
+// orbitals or G vectors; sums are butterfly-reduced so every lane holds the
+// identical value, which keeps accept/reject decisions wave-uniform.
+//
+// `Real` is the working precision of orbitals, ratios, Sherman-Morrison,
+// Jastrow and real-space Ewald (double in the FP64 path, float in the mixed
+// path).  Sums of physical terms always accumulate in double.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace vmc {
+
+constexpr int kWave = 64;
+constexpr int kStatCount = 16;
+
+// ------------------------------------------------------------------ system
+struct DevSys {
+    int N, n_up, n_dn, ndet;
+    int n[2];        // electrons (= orbitals) per determinant
+    int first[2];    // index of the determinant's first electron
+    int inv_off[2];  // offset of the determinant's inverse in a walker's block
+    int ninv;        // n[0]^2 + n[1]^2
+    int nmax;
+    int ng;
+    int erfc_kind;   // 0 = Abramowitz-Stegun 7.1.26, 1 = erfc
+    int move_kind;   // 0 = uniform cube, 1 = Gaussian
+    int use_cut;
+    int reinv_k;     // accepted moves between FP64 re-inversions (0 = never)
+    double L, inv_L, half_L;
+    double a_same, a_anti, b;
+    double alpha, rcut;
+    double e_const;  // Ewald self + background
+    double pref;     // 1 / (2 pi L^3)
+    double kunit;    // 2 pi / L
+    uint32_t key0, key1;
+    const int4 *orb;     // [ndet][nmax] = {nx, ny, nz, type (0 cos, 1 sin)}
+    const int4 *gvec;    // [ng] = {mx, my, mz, 0}
+    const double *gw;    // [ng] weights 8 pi^2 / G^2 exp(-G^2 / 4 alpha^2)
+};
+
+// -------------------------------------------------------------- reductions
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ inline int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ inline double wave_uniform(double v) {
+    union { double d; int i[2]; } u;
+    u.d = v;
+    u.i[0] = __builtin_amdgcn_readfirstlane(u.i[0]);
+    u.i[1] = __builtin_amdgcn_readfirstlane(u.i[1]);
+    return u.d;
+}
+
+// Wave-wide barrier for LDS traffic between lanes of one wavefront.  The
+// kernels run one wavefront per workgroup, so the workgroup barrier is the
+// wavefront barrier.
+__device__ inline void wave_sync() { __syncthreads(); }
+
+// ------------------------------------------------------------------ Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) with the published
+// multipliers 0xD2511F53, 0xCD9E8D57 and Weyl keys 0x9E3779B9, 0xBB67AE85.
+__device__ inline uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
+        const uint32_t lo1 = 0xCD9E8D57u * c.z;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// Stream tags (counter word 3): which draw of a (electron, walker, sweep).
+constexpr uint32_t kTagMove = 0x4D4F5645u;  // "MOVE"
+constexpr uint32_t kTagAccept = 0x41434350u; // "ACCP"
+constexpr uint32_t kTagInit = 0x494E4954u;   // "INIT"
+
+__device__ inline double u01(uint32_t x) { return ((double)x + 0.5) * 2.3283064365386963e-10; }
+
+// ----------------------------------------------------------------- geometry
+// Minimum image by the reference's mask form (jastrow_pade.cpp:37-39):
+// d + L [d <= -L/2] - L [d > L/2], valid for |d| < L.
+__device__ inline double min_image(double d, const DevSys &s) {
+    return d + s.L * (double)(d <= -s.half_L) - s.L * (double)(d > s.half_L);
+}
+
+// Fractional phase n . r / L reduced to [-1/2, 1/2]; the orbital and
+// structure-factor phases are 2 pi times it.
+__device__ inline double frac_phase(int nx, int ny, int nz, double ux, double uy, double uz) {
+    double t = (double)nx * ux + (double)ny * uy + (double)nz * uz;
+    return t - rint(t);
+}
+
+template <typename Real>
+__device__ inline void sincos_2pi(double t, Real *s, Real *c);
+
+template <>
+__device__ inline void sincos_2pi<double>(double t, double *s, double *c) {
+    sincospi(2.0 * t, s, c);
+}
+
+template <>
+__device__ inline void sincos_2pi<float>(double t, float *s, float *c) {
+    sincospif((float)(2.0 * t), s, c);
+}
+
+template <typename Real>
+__device__ inline Real rsqrt_r(Real x);
+template <>
+__device__ inline double rsqrt_r<double>(double x) { return 1.0 / sqrt(x); }
+template <>
+__device__ inline float rsqrt_r<float>(float x) { return 1.0f / sqrtf(x); }
+
+// Abramowitz & Stegun 7.1.26 with the reference's constants
+// (energy_tracking.cpp:101-107,133-137).
+template <typename Real>
+__device__ inline Real erfc_as7126(Real x) {
+    const Real p = (Real)0.3275911;
+    const Real a1 = (Real)0.254829592, a2 = (Real)-0.284496736, a3 = (Real)1.421413741,
+               a4 = (Real)-1.453152027, a5 = (Real)1.061405429;
+    const Real t = (Real)1 / ((Real)1 + p * x);
+    const Real tau = t * (a1 + t * (a2 + t * (a3 + t * (a4 + t * a5))));
+    return tau * exp(-x * x);
+}
+
+template <typename Real>
+__device__ inline Real erfc_r(Real x, int kind) {
+    if (kind == 0) return erfc_as7126<Real>(x);
+    return erfc(x);
+}
+
+// Pade pair value u(r) = a r / (1 + b r)   (jastrow_pade.hpp:15-16).
+template <typename Real>
+__device__ inline Real pade_u(Real r, Real a, Real b) {
+    return a * r / ((Real)1 + b * r);
+}
+
+// Real-space Ewald pair term erfc(alpha r) / r with the reference's r < 1e-12
+// guard (energy_tracking.cpp:132,139) and an optional spherical cutoff.
+template <typename Real>
+__device__ inline Real ewald_pair(Real r, const DevSys &s) {
+    if (s.use_cut && !(r < (Real)s.rcut)) return (Real)0;
+    const Real inv_r = (r < (Real)1e-12) ? (Real)1 : (Real)1 / r;
+    return erfc_r<Real>((Real)s.alpha * r, s.erfc_kind) * inv_r;
+}
+
+}  // namespace vmc

@@ -1,0 +1,774 @@
+// vmc_hip.hip -- gfx950 kernels and the C ABI (include/vmc_hip.h) of the VMC
+// sweep for the 3D homogeneous electron gas.
+//
+// Layout in HBM, per ensemble (walker-major SoA):
+//   pos     double [W][3][N]   x block, y block, z block
+//   inv     Real   [W][ninv]   transposed inverses of the determinants
+//   sigma   double [W]         proposal width
+//   stats   double [W][16]     FP64 accumulators (VMC_STAT_*)
+//   logdet  double [W][2]
+//   since   int    [W]         accepted moves since the last FP64 re-inversion
+//   flags   int    [W]         bit0: singular determinant met
+//   scratch double [W][nmax^2] FP64 Gauss-Jordan workspace
+// One wavefront (one 64-lane workgroup) runs one walker; its positions,
+// Jastrow row sums and draws stay in LDS for a whole launch, and so do the
+// inverses when they fit.
+#include "../../include/vmc_hip.h"
+#include "vmc_device.hpp"
+#include "vmc_tables.hpp"
+#include "vmc_walker.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numbers>
+#include <string>
+#include <vector>
+
+namespace vmc {
+
+struct DevState {
+    int W;
+    double *pos;
+    void *inv;
+    double *sigma;
+    double *stats;
+    double *logdet;
+    int *since;
+    int *flags;
+    double *scratch;
+};
+
+constexpr size_t kMaxLdsBytes = 64 * 1024;
+
+// ----------------------------------------------------------------- kernels
+
+// Start configurations: Philox draws (tag INIT, counter {i, walker, attempt})
+// retried while a determinant is singular, mirroring
+// Simulation::initialize_positions (simulation.cpp:43-76).  With draw == 0
+// the current positions are kept and only the FP64 inverses are rebuilt.
+template <typename Real>
+__global__ __launch_bounds__(kWave) void k_init(DevSys s, DevState st, int draw) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int w = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int N = s.N;
+    Real *inv_hbm = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
+    double *A = st.scratch + (size_t)w * s.nmax * s.nmax;
+    WalkerView<Real> v = carve<Real>(smem, s, false, inv_hbm, A);
+    double *gp = st.pos + (size_t)w * 3 * N;
+    const int attempts = draw ? 100 : 1;
+    int ok = 0;
+    double ld[2] = {0.0, 0.0};
+    for (int att = 0; att < attempts && !ok; ++att) {
+        for (int i = lane; i < N; i += kWave) {
+            if (draw) {
+                const uint4 r = philox4x32_10(make_uint4((uint32_t)i, (uint32_t)w, (uint32_t)att,
+                                                         kTagInit),
+                                              s.key0, s.key1);
+                v.px[i] = u01(r.x) * s.L;
+                v.py[i] = u01(r.y) * s.L;
+                v.pz[i] = u01(r.z) * s.L;
+            } else {
+                v.px[i] = gp[i];
+                v.py[i] = gp[N + i];
+                v.pz[i] = gp[2 * N + i];
+            }
+        }
+        wave_sync();
+        ok = 1;
+        for (int d = 0; d < s.ndet; ++d) {
+            ld[d] = reinvert<Real>(s, d, v, lane);
+            if (!(ld[d] > -INFINITY)) ok = 0;
+        }
+    }
+    for (int i = lane; i < N; i += kWave) {
+        gp[i] = v.px[i];
+        gp[N + i] = v.py[i];
+        gp[2 * N + i] = v.pz[i];
+    }
+    if (lane == 0) {
+        st.logdet[2 * w + 0] = ld[0];
+        st.logdet[2 * w + 1] = ld[1];
+        st.flags[w] = ok ? 0 : 1;
+        st.since[w] = 0;
+    }
+}
+
+// n_sweeps sequential sweeps of walker blockIdx.x (simulation.cpp:80-263 as
+// an ensemble): per electron p a Philox proposal (tag MOVE, counter
+// {p, walker, sweep}), ratio and Jastrow change, Metropolis test with a
+// Philox uniform (tag ACCP), Sherman-Morrison on accept; after each sweep the
+// optional step adaptation and local-energy measurement.
+template <typename Real, bool kInvLds>
+__global__ __launch_bounds__(kWave) void k_sweep(DevSys s, DevState st, int n_sweeps,
+                                                 unsigned long long sweep0, int flags,
+                                                 int block_size) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int w = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int N = s.N;
+    Real *inv_hbm = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
+    double *A = st.scratch + (size_t)w * s.nmax * s.nmax;
+    WalkerView<Real> v = carve<Real>(smem, s, kInvLds, inv_hbm, A);
+    if (kInvLds)
+        for (int e = lane; e < s.ninv; e += kWave) v.inv[e] = inv_hbm[e];
+    double *gp = st.pos + (size_t)w * 3 * N;
+    for (int i = lane; i < N; i += kWave) {
+        v.px[i] = gp[i];
+        v.py[i] = gp[N + i];
+        v.pz[i] = gp[2 * N + i];
+    }
+    wave_sync();
+    jastrow_rows<Real>(s, v, lane);
+
+    double *ws = st.stats + (size_t)w * kStatCount;
+    double stat[kStatCount];
+#pragma unroll
+    for (int k = 0; k < kStatCount; ++k) stat[k] = ws[k];
+    double sigma = st.sigma[w];
+    int since = st.since[w];
+    int wflags = st.flags[w];
+    double ld0 = st.logdet[2 * w + 0], ld1 = st.logdet[2 * w + 1];
+
+    for (int sw = 0; sw < n_sweeps; ++sw) {
+        const uint32_t sweep = (uint32_t)(sweep0 + (unsigned long long)sw);
+        for (int q = lane; q < N; q += kWave) {
+            const uint4 r = philox4x32_10(make_uint4((uint32_t)q, (uint32_t)w, sweep, kTagMove),
+                                          s.key0, s.key1);
+            const uint4 ra = philox4x32_10(make_uint4((uint32_t)q, (uint32_t)w, sweep, kTagAccept),
+                                           s.key0, s.key1);
+            double dx, dy, dz;
+            if (s.move_kind == 1) {
+                const double rho0 = sqrt(-2.0 * log(u01(r.x)));
+                const double rho1 = sqrt(-2.0 * log(u01(r.z)));
+                double s0, c0, s1, c1;
+                sincospi(2.0 * u01(r.y), &s0, &c0);
+                sincospi(2.0 * u01(r.w), &s1, &c1);
+                dx = rho0 * c0;
+                dy = rho0 * s0;
+                dz = rho1 * c1;
+            } else {
+                dx = 2.0 * u01(r.x) - 1.0;
+                dy = 2.0 * u01(r.y) - 1.0;
+                dz = 2.0 * u01(r.z) - 1.0;
+            }
+            v.draw[q] = dx;
+            v.draw[N + q] = dy;
+            v.draw[2 * N + q] = dz;
+            v.draw[3 * N + q] = u01(ra.x);
+        }
+        wave_sync();
+        int acc = 0;
+        for (int p = 0; p < N; ++p) {
+            double x = v.px[p] + sigma * v.draw[p];
+            double y = v.py[p] + sigma * v.draw[N + p];
+            double z = v.pz[p] + sigma * v.draw[2 * N + p];
+            x -= s.L * floor(x * s.inv_L);
+            y -= s.L * floor(y * s.inv_L);
+            z -= s.L * floor(z * s.inv_L);
+            const Proposal pr = propose<Real>(s, v, p, x, y, z, lane);
+            const double prob = pr.ratio * pr.ratio * exp(2.0 * pr.dj);
+            const int accept = wave_uniform((int)(v.draw[3 * N + p] < prob));
+            if (accept) {
+                accept_move<Real>(s, v, p, x, y, z, pr, lane);
+                ++acc;
+                const int d = spin_of(s, p);
+                if (d == 0) ld0 += log(fabs(pr.ratio)); else ld1 += log(fabs(pr.ratio));
+                if (s.reinv_k > 0 && ++since >= s.reinv_k) {
+                    since = 0;
+                    for (int dd = 0; dd < s.ndet; ++dd) {
+                        const double l = reinvert<Real>(s, dd, v, lane);
+                        if (!(l > -INFINITY)) wflags |= 1;
+                        else if (dd == 0) ld0 = l;
+                        else ld1 = l;
+                    }
+                    stat[15] += 1.0;
+                }
+            }
+        }
+        stat[6] += (double)acc;
+        stat[7] += (double)N;
+        if (flags & VMC_SWEEP_ADAPT) {
+            sigma *= exp(0.25 * ((double)acc / (double)N - 0.5));
+            if (sigma > 0.5 * s.L) sigma = 0.5 * s.L;
+        }
+        if (flags & VMC_SWEEP_MEASURE) {
+            jastrow_rows<Real>(s, v, lane);
+            const Measured m = measure<Real>(s, v, lane, nullptr, nullptr);
+            const double e = m.e_local;
+            stat[0] += 1.0;
+            stat[1] += e;
+            stat[2] += e * e;
+            stat[3] += m.kinetic;
+            stat[4] += m.vreal + m.vrecip + s.e_const;
+            stat[5] = e;
+            stat[13] = m.kinetic;
+            stat[14] = m.vreal + m.vrecip + s.e_const;
+            // Blocking over measured sweeps (blocking_analysis.cpp:17-39).
+            stat[8] += e;
+            stat[9] += 1.0;
+            if (stat[9] >= (double)block_size) {
+                const double bm = stat[8] / (double)block_size;
+                stat[10] += 1.0;
+                const double delta = bm - stat[11];
+                stat[11] += delta / stat[10];
+                stat[12] += delta * (bm - stat[11]);
+                stat[8] = 0.0;
+                stat[9] = 0.0;
+            }
+        }
+    }
+    if (kInvLds)
+        for (int e = lane; e < s.ninv; e += kWave) inv_hbm[e] = v.inv[e];
+    for (int i = lane; i < N; i += kWave) {
+        gp[i] = v.px[i];
+        gp[N + i] = v.py[i];
+        gp[2 * N + i] = v.pz[i];
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < kStatCount; ++k) ws[k] = stat[k];
+        st.sigma[w] = sigma;
+        st.since[w] = since;
+        st.flags[w] = wflags;
+        st.logdet[2 * w + 0] = ld0;
+        st.logdet[2 * w + 1] = ld1;
+    }
+}
+
+// Frozen-configuration evaluation (one configuration per wavefront).
+template <typename Real, bool kInvLds>
+__global__ __launch_bounds__(kWave) void k_frozen(DevSys s, int C, const double *pos,
+                                                  const int *moved, const double *newpos,
+                                                  double *scalars, double *gl, double *inv_before,
+                                                  double *inv_after, double *sk, Real *inv_g,
+                                                  double *A_g, double *sk_scratch) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int c = blockIdx.x;
+    if (c >= C) return;
+    const int lane = threadIdx.x;
+    const int N = s.N;
+    Real *inv_hbm = inv_g + (size_t)c * s.ninv;
+    double *A = A_g + (size_t)c * s.nmax * s.nmax;
+    WalkerView<Real> v = carve<Real>(smem, s, kInvLds, inv_hbm, A);
+    const double *cp = pos + (size_t)c * 3 * N;
+    for (int i = lane; i < N; i += kWave) {
+        v.px[i] = cp[3 * i + 0];
+        v.py[i] = cp[3 * i + 1];
+        v.pz[i] = cp[3 * i + 2];
+    }
+    wave_sync();
+    double *out = scalars + (size_t)c * VMC_FZ_COUNT;
+    double ld[2] = {0.0, 0.0};
+    int singular = 0;
+    for (int d = 0; d < s.ndet; ++d) {
+        ld[d] = reinvert<Real>(s, d, v, lane);
+        if (!(ld[d] > -INFINITY)) singular = 1;
+    }
+    if (inv_before)
+        for (int e = lane; e < s.ninv; e += kWave) inv_before[(size_t)c * s.ninv + e] = (double)v.inv[e];
+    jastrow_rows<Real>(s, v, lane);
+    double jp = 0.0;
+    for (int i = lane; i < N; i += kWave) jp += v.U[i];
+    const double J = 0.5 * wave_sum(jp);
+    double *skc = sk_scratch + (size_t)c * 2 * s.ng;
+    const Measured m0 = measure<Real>(s, v, lane, gl ? gl + (size_t)c * 4 * N : nullptr, skc);
+    wave_sync();
+    if (sk)
+        for (int g = lane; g < 2 * s.ng; g += kWave) sk[(size_t)c * 2 * s.ng + g] = skc[g];
+
+    const int p = moved[c];
+    const double x = newpos[3 * c + 0], y = newpos[3 * c + 1], z = newpos[3 * c + 2];
+    const double ox = v.px[p], oy = v.py[p], oz = v.pz[p];
+    const Proposal pr = propose<Real>(s, v, p, x, y, z, lane);
+
+    // Incremental Ewald updates of the move (energy_tracking.cpp:193-251, 280-365).
+    double dq = 0.0;
+    for (int g = lane; g < s.ng; g += kWave) {
+        const int4 G = s.gvec[g];
+        Real sn, cs, so, co;
+        sincos_2pi<Real>(frac_phase(G.x, G.y, G.z, x * s.inv_L, y * s.inv_L, z * s.inv_L), &sn, &cs);
+        sincos_2pi<Real>(frac_phase(G.x, G.y, G.z, ox * s.inv_L, oy * s.inv_L, oz * s.inv_L), &so,
+                         &co);
+        const double dr = (double)cs - (double)co, di = (double)sn - (double)so;
+        dq += s.gw[g] * (2.0 * (skc[2 * g] * dr + skc[2 * g + 1] * di) + dr * dr + di * di);
+    }
+    const double vrecip_inc = m0.vrecip + s.pref * wave_sum(dq);
+    double dv = 0.0;
+    for (int j = lane; j < N; j += kWave) {
+        if (j == p) continue;
+        const Real rn = pair_dist<Real>(s, x, y, z, v, j, nullptr, nullptr, nullptr);
+        const Real ro = pair_dist<Real>(s, ox, oy, oz, v, j, nullptr, nullptr, nullptr);
+        dv += (double)ewald_pair<Real>(rn, s) - (double)ewald_pair<Real>(ro, s);
+    }
+    const double vreal_inc = m0.vreal + wave_sum(dv);
+
+    accept_move<Real>(s, v, p, x, y, z, pr, lane);
+    if (inv_after)
+        for (int e = lane; e < s.ninv; e += kWave) inv_after[(size_t)c * s.ninv + e] = (double)v.inv[e];
+    const Measured m1 = measure<Real>(s, v, lane, nullptr, nullptr);
+    if (lane == 0) {
+        out[VMC_FZ_LOGDET0] = ld[0];
+        out[VMC_FZ_LOGDET1] = ld[1];
+        out[VMC_FZ_JASTROW] = J;
+        out[VMC_FZ_KINETIC] = m0.kinetic;
+        out[VMC_FZ_VREAL] = m0.vreal;
+        out[VMC_FZ_VRECIP] = m0.vrecip;
+        out[VMC_FZ_ELOCAL] = m0.e_local;
+        out[VMC_FZ_RATIO] = pr.ratio;
+        out[VMC_FZ_DELTA_J] = pr.dj;
+        out[VMC_FZ_VREAL_INC] = vreal_inc;
+        out[VMC_FZ_VRECIP_INC] = vrecip_inc;
+        out[VMC_FZ_ELOCAL_AFTER] = m1.e_local;
+        out[VMC_FZ_KINETIC_AFTER] = m1.kinetic;
+        out[VMC_FZ_JASTROW_AFTER] = J + pr.dj;
+        out[VMC_FZ_SINGULAR] = (double)singular;
+    }
+}
+
+}  // namespace vmc
+
+// ================================================================== host side
+
+using namespace vmc;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define VMC_HIP_TRY(expr)                                                                    \
+    do {                                                                                     \
+        hipError_t err__ = (expr);                                                           \
+        if (err__ != hipSuccess)                                                             \
+            return fail(VMC_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(err__));    \
+    } while (0)
+
+// Host copy of the constant tables and the device-side DevSys.
+struct SystemTables {
+    vmc_system desc{};
+    DevSys dev{};
+    std::vector<int4> orb;
+    std::vector<int4> gvec;
+    std::vector<double> gw;
+    int4 *d_orb = nullptr;
+    int4 *d_gvec = nullptr;
+    double *d_gw = nullptr;
+    int real_bytes = 8;
+
+    ~SystemTables() {
+        if (d_orb) (void)hipFree(d_orb);
+        if (d_gvec) (void)hipFree(d_gvec);
+        if (d_gw) (void)hipFree(d_gw);
+    }
+};
+
+int validate(const vmc_system *s) {
+    if (!s) return fail(VMC_ERR_INVALID, "system descriptor is NULL");
+    if (s->n_up < 1) return fail(VMC_ERR_INVALID, "n_up must be >= 1");
+    if (s->n_dn < 0) return fail(VMC_ERR_INVALID, "n_dn must be >= 0");
+    if (!(s->box_length > 0.0) || !std::isfinite(s->box_length))
+        return fail(VMC_ERR_INVALID, "box_length must be finite and > 0");
+    if (s->precision != VMC_PREC_FP64 && s->precision != VMC_PREC_MIXED)
+        return fail(VMC_ERR_INVALID, "unknown precision %d", s->precision);
+    if (s->move_kind != VMC_MOVE_UNIFORM && s->move_kind != VMC_MOVE_GAUSSIAN)
+        return fail(VMC_ERR_INVALID, "unknown move kind %d", s->move_kind);
+    if (s->erfc_kind != VMC_ERFC_AS7126 && s->erfc_kind != VMC_ERFC_EXACT)
+        return fail(VMC_ERR_INVALID, "unknown erfc kind %d", s->erfc_kind);
+    if (s->reinvert_interval < 0) return fail(VMC_ERR_INVALID, "reinvert_interval must be >= 0");
+    if (s->recip_nmax2 < 0) return fail(VMC_ERR_INVALID, "recip_nmax2 must be >= 0");
+    if (s->n_up + s->n_dn > 4096) return fail(VMC_ERR_INVALID, "at most 4096 electrons");
+    return VMC_OK;
+}
+
+int build_tables(const vmc_system *sys, uint64_t seed, SystemTables &t) {
+    int rc = validate(sys);
+    if (rc) return rc;
+    t.desc = *sys;
+    DevSys &d = t.dev;
+    d.n_up = sys->n_up;
+    d.n_dn = sys->n_dn;
+    d.N = sys->n_up + sys->n_dn;
+    d.ndet = sys->n_dn > 0 ? 2 : 1;
+    d.n[0] = sys->n_up;
+    d.n[1] = sys->n_dn;
+    d.first[0] = 0;
+    d.first[1] = sys->n_up;
+    d.inv_off[0] = 0;
+    d.inv_off[1] = sys->n_up * sys->n_up;
+    d.ninv = sys->n_up * sys->n_up + sys->n_dn * sys->n_dn;
+    d.nmax = std::max(sys->n_up, sys->n_dn);
+    d.erfc_kind = sys->erfc_kind;
+    d.move_kind = sys->move_kind;
+    d.use_cut = sys->real_cutoff > 0.0 ? 1 : 0;
+    d.reinv_k = sys->reinvert_interval;
+    d.L = sys->box_length;
+    d.inv_L = 1.0 / sys->box_length;
+    d.half_L = 0.5 * sys->box_length;
+    d.a_same = sys->jastrow_a_same;
+    d.a_anti = sys->jastrow_a_anti;
+    d.b = sys->jastrow_b;
+    d.rcut = sys->real_cutoff;
+    constexpr double pi = std::numbers::pi;
+    const GTable g = ewald_gvectors(sys->box_length, sys->ewald_alpha, sys->recip_nmax2);
+    d.alpha = g.alpha;
+    const double N = static_cast<double>(d.N), L = sys->box_length;
+    // Self and background terms (energy_tracking.cpp:8-9 for alpha = 6/L).
+    double self_term, background;
+    if (sys->ewald_alpha > 0.0) {
+        self_term = -g.alpha * N / std::sqrt(pi);
+        background = -pi * N * N / (2.0 * g.alpha * g.alpha * L * L * L);
+    } else {
+        self_term = -6.0 * N / (std::sqrt(pi) * L);
+        background = -pi * N * N / (72.0 * L);
+    }
+    d.e_const = self_term + background;
+    d.pref = 1.0 / (2.0 * pi * L * L * L);
+    d.kunit = 2.0 * pi / L;
+    d.key0 = static_cast<uint32_t>(seed);
+    d.key1 = static_cast<uint32_t>(seed >> 32);
+    d.ng = static_cast<int>(g.m.size());
+    t.orb.assign(static_cast<size_t>(d.ndet) * d.nmax, make_int4(0, 0, 0, 0));
+    for (int det = 0; det < d.ndet; ++det) {
+        const ShellTable sh = fill_shells(d.n[det]);
+        for (int j = 0; j < d.n[det]; ++j) {
+            const auto &nv = sh.nvec[static_cast<size_t>(sh.orb_k[static_cast<size_t>(j)])];
+            t.orb[static_cast<size_t>(det * d.nmax + j)] =
+                make_int4(nv[0], nv[1], nv[2], sh.orb_type[static_cast<size_t>(j)]);
+        }
+    }
+    t.gvec.resize(g.m.size());
+    for (size_t i = 0; i < g.m.size(); ++i) t.gvec[i] = make_int4(g.m[i][0], g.m[i][1], g.m[i][2], 0);
+    t.gw = g.w;
+    t.real_bytes = sys->precision == VMC_PREC_FP64 ? 8 : 4;
+    VMC_HIP_TRY(hipMalloc(&t.d_orb, sizeof(int4) * std::max<size_t>(t.orb.size(), 1)));
+    VMC_HIP_TRY(hipMemcpy(t.d_orb, t.orb.data(), sizeof(int4) * t.orb.size(), hipMemcpyHostToDevice));
+    VMC_HIP_TRY(hipMalloc(&t.d_gvec, sizeof(int4) * std::max<size_t>(t.gvec.size(), 1)));
+    VMC_HIP_TRY(hipMalloc(&t.d_gw, sizeof(double) * std::max<size_t>(t.gw.size(), 1)));
+    if (!t.gvec.empty()) {
+        VMC_HIP_TRY(hipMemcpy(t.d_gvec, t.gvec.data(), sizeof(int4) * t.gvec.size(), hipMemcpyHostToDevice));
+        VMC_HIP_TRY(hipMemcpy(t.d_gw, t.gw.data(), sizeof(double) * t.gw.size(), hipMemcpyHostToDevice));
+    }
+    d.orb = t.d_orb;
+    d.gvec = t.d_gvec;
+    d.gw = t.d_gw;
+    return VMC_OK;
+}
+
+template <typename T>
+int dev_alloc(T **p, size_t count) {
+    if (hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) {
+        *p = nullptr;
+        return fail(VMC_ERR_MEMORY, "hipMalloc of %zu bytes failed", sizeof(T) * count);
+    }
+    return VMC_OK;
+}
+
+size_t lds_for(const DevSys &d, int real_bytes, bool inv_lds) {
+    return walker_lds_bytes(d.N, d.nmax, d.ninv, real_bytes, inv_lds);
+}
+
+}  // namespace
+
+struct vmc_ensemble {
+    SystemTables tables;
+    DevState st{};
+    int64_t W = 0;
+    uint64_t seed = 0;
+    uint64_t sweeps_done = 0;
+    double step0 = 0.0;
+    bool positions_ready = false;
+
+    ~vmc_ensemble() {
+        if (st.pos) (void)hipFree(st.pos);
+        if (st.inv) (void)hipFree(st.inv);
+        if (st.sigma) (void)hipFree(st.sigma);
+        if (st.stats) (void)hipFree(st.stats);
+        if (st.logdet) (void)hipFree(st.logdet);
+        if (st.since) (void)hipFree(st.since);
+        if (st.flags) (void)hipFree(st.flags);
+        if (st.scratch) (void)hipFree(st.scratch);
+    }
+};
+
+namespace {
+
+template <typename Real>
+int launch_init(vmc_ensemble *e, int draw, hipStream_t stream) {
+    const DevSys &d = e->tables.dev;
+    const size_t lds = lds_for(d, sizeof(Real), false);
+    hipLaunchKernelGGL((k_init<Real>), dim3(static_cast<unsigned>(e->W)), dim3(kWave), lds, stream,
+                       d, e->st, draw);
+    VMC_HIP_TRY(hipGetLastError());
+    return VMC_OK;
+}
+
+template <typename Real>
+int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipStream_t stream) {
+    const DevSys &d = e->tables.dev;
+    const size_t lds_in = lds_for(d, sizeof(Real), true);
+    const unsigned grid = static_cast<unsigned>(e->W);
+    if (lds_in <= kMaxLdsBytes) {
+        hipLaunchKernelGGL((k_sweep<Real, true>), dim3(grid), dim3(kWave), lds_in, stream, d, e->st,
+                           n_sweeps, (unsigned long long)e->sweeps_done, flags, block_size);
+    } else {
+        const size_t lds_out = lds_for(d, sizeof(Real), false);
+        if (lds_out > kMaxLdsBytes) return fail(VMC_ERR_INVALID, "system too large for one walker per wavefront");
+        hipLaunchKernelGGL((k_sweep<Real, false>), dim3(grid), dim3(kWave), lds_out, stream, d,
+                           e->st, n_sweeps, (unsigned long long)e->sweeps_done, flags, block_size);
+    }
+    VMC_HIP_TRY(hipGetLastError());
+    return VMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *vmc_last_error(void) { return g_last_error.c_str(); }
+
+int vmc_device_check(int32_t *device_count) {
+    int n = 0;
+    VMC_HIP_TRY(hipGetDeviceCount(&n));
+    if (device_count) *device_count = n;
+    if (n < 1) return fail(VMC_ERR_DEVICE, "no HIP device visible");
+    hipDeviceProp_t prop;
+    int dev = 0;
+    VMC_HIP_TRY(hipGetDevice(&dev));
+    VMC_HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(VMC_ERR_DEVICE, "device %d is %s, this library is built for gfx950", dev, prop.gcnArchName);
+    return VMC_OK;
+}
+
+int vmc_system_sizes(const vmc_system *sys, int32_t *n_k, int32_t *n_g, int32_t *n_inv) {
+    int rc = validate(sys);
+    if (rc) return rc;
+    if (n_k) {
+        n_k[0] = fill_shells(sys->n_up).nk;
+        n_k[1] = sys->n_dn > 0 ? fill_shells(sys->n_dn).nk : 0;
+    }
+    if (n_g) *n_g = static_cast<int32_t>(ewald_gvectors(sys->box_length, sys->ewald_alpha, sys->recip_nmax2).m.size());
+    if (n_inv) *n_inv = sys->n_up * sys->n_up + sys->n_dn * sys->n_dn;
+    return VMC_OK;
+}
+
+int vmc_ensemble_create(const vmc_system *sys, int64_t n_walkers, uint64_t seed, double step0,
+                        vmc_ensemble **out) {
+    if (!out) return fail(VMC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (n_walkers < 1 || n_walkers > (int64_t(1) << 24)) return fail(VMC_ERR_INVALID, "n_walkers out of range");
+    if (!(step0 >= 0.0) || !std::isfinite(step0)) return fail(VMC_ERR_INVALID, "step0 must be finite and >= 0");
+    auto *e = new vmc_ensemble();
+    int rc = build_tables(sys, seed, e->tables);
+    if (rc) { delete e; return rc; }
+    const DevSys &d = e->tables.dev;
+    if (lds_for(d, e->tables.real_bytes, false) > kMaxLdsBytes) {
+        delete e;
+        return fail(VMC_ERR_INVALID, "N=%d exceeds the per-walker LDS budget", d.N);
+    }
+    e->W = n_walkers;
+    e->seed = seed;
+    e->step0 = step0;
+    const size_t W = static_cast<size_t>(n_walkers);
+    DevState &st = e->st;
+    st.W = static_cast<int>(n_walkers);
+    if ((rc = dev_alloc(&st.pos, W * 3 * d.N)) || (rc = dev_alloc(reinterpret_cast<unsigned char **>(&st.inv), W * d.ninv * e->tables.real_bytes)) ||
+        (rc = dev_alloc(&st.sigma, W)) || (rc = dev_alloc(&st.stats, W * kStatCount)) ||
+        (rc = dev_alloc(&st.logdet, W * 2)) || (rc = dev_alloc(&st.since, W)) ||
+        (rc = dev_alloc(&st.flags, W)) || (rc = dev_alloc(&st.scratch, W * d.nmax * d.nmax))) {
+        delete e;
+        return rc;
+    }
+    std::vector<double> sig(W, step0);
+    if (hipMemcpy(st.sigma, sig.data(), sizeof(double) * W, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(st.stats, 0, sizeof(double) * W * kStatCount) != hipSuccess ||
+        hipMemset(st.since, 0, sizeof(int) * W) != hipSuccess ||
+        hipMemset(st.flags, 0, sizeof(int) * W) != hipSuccess ||
+        hipMemset(st.logdet, 0, sizeof(double) * W * 2) != hipSuccess) {
+        delete e;
+        return fail(VMC_ERR_DEVICE, "ensemble initialisation copy failed");
+    }
+    *out = e;
+    return VMC_OK;
+}
+
+int vmc_ensemble_destroy(vmc_ensemble *e) {
+    delete e;
+    return VMC_OK;
+}
+
+static int finish_init(vmc_ensemble *e, hipStream_t stream) {
+    VMC_HIP_TRY(hipStreamSynchronize(stream));
+    std::vector<int> fl(static_cast<size_t>(e->W));
+    VMC_HIP_TRY(hipMemcpy(fl.data(), e->st.flags, sizeof(int) * fl.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < fl.size(); ++i)
+        if (fl[i]) return fail(VMC_ERR_SINGULAR, "walker %zu: no non-singular configuration", i);
+    e->positions_ready = true;
+    return VMC_OK;
+}
+
+int vmc_ensemble_init_positions(vmc_ensemble *e, void *stream) {
+    if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = e->tables.real_bytes == 8 ? launch_init<double>(e, 1, s) : launch_init<float>(e, 1, s);
+    if (rc) return rc;
+    return finish_init(e, s);
+}
+
+int vmc_ensemble_set_positions(vmc_ensemble *e, const double *pos, void *stream) {
+    if (!e || !pos) return fail(VMC_ERR_INVALID, "NULL argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int N = e->tables.dev.N;
+    const size_t W = static_cast<size_t>(e->W);
+    std::vector<double> soa(W * 3 * N);
+    const double L = e->tables.dev.L;
+    for (size_t w = 0; w < W; ++w)
+        for (int i = 0; i < N; ++i)
+            for (int c = 0; c < 3; ++c) {
+                const double x = pos[(w * N + i) * 3 + c];
+                if (!(x >= 0.0 && x < L)) return fail(VMC_ERR_INVALID, "position outside [0, L)");
+                soa[(w * 3 + c) * N + i] = x;
+            }
+    VMC_HIP_TRY(hipMemcpyAsync(e->st.pos, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice, s));
+    int rc = e->tables.real_bytes == 8 ? launch_init<double>(e, 0, s) : launch_init<float>(e, 0, s);
+    if (rc) return rc;
+    return finish_init(e, s);
+}
+
+int vmc_ensemble_get_positions(vmc_ensemble *e, double *pos, void *stream) {
+    if (!e || !pos) return fail(VMC_ERR_INVALID, "NULL argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int N = e->tables.dev.N;
+    const size_t W = static_cast<size_t>(e->W);
+    std::vector<double> soa(W * 3 * N);
+    VMC_HIP_TRY(hipMemcpyAsync(soa.data(), e->st.pos, sizeof(double) * soa.size(), hipMemcpyDeviceToHost, s));
+    VMC_HIP_TRY(hipStreamSynchronize(s));
+    for (size_t w = 0; w < W; ++w)
+        for (int i = 0; i < N; ++i)
+            for (int c = 0; c < 3; ++c) pos[(w * N + i) * 3 + c] = soa[(w * 3 + c) * N + i];
+    return VMC_OK;
+}
+
+int vmc_ensemble_sweep(vmc_ensemble *e, int32_t n_sweeps, int32_t flags, int32_t block_size,
+                       void *stream) {
+    if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
+    if (!e->positions_ready) return fail(VMC_ERR_STATE, "positions not initialised");
+    if (n_sweeps < 0) return fail(VMC_ERR_INVALID, "n_sweeps must be >= 0");
+    if (block_size < 1) return fail(VMC_ERR_INVALID, "block_size must be >= 1");
+    if (n_sweeps == 0) return VMC_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = e->tables.real_bytes == 8 ? launch_sweep<double>(e, n_sweeps, flags, block_size, s)
+                                       : launch_sweep<float>(e, n_sweeps, flags, block_size, s);
+    if (rc) return rc;
+    e->sweeps_done += static_cast<uint64_t>(n_sweeps);
+    return VMC_OK;
+}
+
+int vmc_ensemble_reset_stats(vmc_ensemble *e, void *stream) {
+    if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
+    VMC_HIP_TRY(hipMemsetAsync(e->st.stats, 0, sizeof(double) * e->W * kStatCount, static_cast<hipStream_t>(stream)));
+    return VMC_OK;
+}
+
+int vmc_ensemble_read_stats(vmc_ensemble *e, double *stats, void *stream) {
+    if (!e || !stats) return fail(VMC_ERR_INVALID, "NULL argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    VMC_HIP_TRY(hipMemcpyAsync(stats, e->st.stats, sizeof(double) * e->W * kStatCount, hipMemcpyDeviceToHost, s));
+    VMC_HIP_TRY(hipStreamSynchronize(s));
+    return VMC_OK;
+}
+
+int vmc_ensemble_read_steps(vmc_ensemble *e, double *steps, void *stream) {
+    if (!e || !steps) return fail(VMC_ERR_INVALID, "NULL argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    VMC_HIP_TRY(hipMemcpyAsync(steps, e->st.sigma, sizeof(double) * e->W, hipMemcpyDeviceToHost, s));
+    VMC_HIP_TRY(hipStreamSynchronize(s));
+    return VMC_OK;
+}
+
+int vmc_ensemble_info(const vmc_ensemble *e, int64_t *n_walkers, uint64_t *sweeps_done) {
+    if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
+    if (n_walkers) *n_walkers = e->W;
+    if (sweeps_done) *sweeps_done = e->sweeps_done;
+    return VMC_OK;
+}
+
+int vmc_frozen_eval(const vmc_system *sys, int32_t n_configs, const double *pos,
+                    const int32_t *moved, const double *newpos, double *scalars, double *grad_lap,
+                    double *inv_before, double *inv_after, double *s_k, void *stream) {
+    if (n_configs < 1 || !pos || !moved || !newpos || !scalars) return fail(VMC_ERR_INVALID, "bad frozen-eval arguments");
+    SystemTables t;
+    int rc = build_tables(sys, 0, t);
+    if (rc) return rc;
+    const DevSys &d = t.dev;
+    const size_t C = static_cast<size_t>(n_configs);
+    for (size_t c = 0; c < C; ++c) {
+        if (moved[c] < 0 || moved[c] >= d.N) return fail(VMC_ERR_INVALID, "moved index out of range");
+        for (int k = 0; k < 3; ++k)
+            if (!(newpos[3 * c + k] >= 0.0 && newpos[3 * c + k] < d.L)) return fail(VMC_ERR_INVALID, "newpos outside [0, L)");
+        for (int i = 0; i < 3 * d.N; ++i)
+            if (!(pos[c * 3 * d.N + i] >= 0.0 && pos[c * 3 * d.N + i] < d.L)) return fail(VMC_ERR_INVALID, "position outside [0, L)");
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double *d_pos = nullptr, *d_new = nullptr, *d_sc = nullptr, *d_gl = nullptr, *d_ib = nullptr,
+           *d_ia = nullptr, *d_sk = nullptr, *d_A = nullptr, *d_skc = nullptr;
+    int *d_mv = nullptr;
+    unsigned char *d_inv = nullptr;
+    struct Guard {
+        std::vector<void *> p;
+        ~Guard() { for (void *q : p) if (q) (void)hipFree(q); }
+    } guard;
+    const size_t rb = static_cast<size_t>(t.real_bytes);
+    if ((rc = dev_alloc(&d_pos, C * 3 * d.N)) || (rc = dev_alloc(&d_new, C * 3)) ||
+        (rc = dev_alloc(&d_mv, C)) || (rc = dev_alloc(&d_sc, C * VMC_FZ_COUNT)) ||
+        (rc = dev_alloc(&d_gl, C * 4 * d.N)) || (rc = dev_alloc(&d_ib, C * d.ninv)) ||
+        (rc = dev_alloc(&d_ia, C * d.ninv)) || (rc = dev_alloc(&d_sk, C * 2 * std::max(d.ng, 1))) ||
+        (rc = dev_alloc(&d_A, C * d.nmax * d.nmax)) || (rc = dev_alloc(&d_skc, C * 2 * std::max(d.ng, 1))) ||
+        (rc = dev_alloc(&d_inv, C * d.ninv * rb))) {
+        guard.p = {d_pos, d_new, d_mv, d_sc, d_gl, d_ib, d_ia, d_sk, d_A, d_skc, d_inv};
+        return rc;
+    }
+    guard.p = {d_pos, d_new, d_mv, d_sc, d_gl, d_ib, d_ia, d_sk, d_A, d_skc, d_inv};
+    VMC_HIP_TRY(hipMemcpyAsync(d_pos, pos, sizeof(double) * C * 3 * d.N, hipMemcpyHostToDevice, s));
+    VMC_HIP_TRY(hipMemcpyAsync(d_new, newpos, sizeof(double) * C * 3, hipMemcpyHostToDevice, s));
+    VMC_HIP_TRY(hipMemcpyAsync(d_mv, moved, sizeof(int) * C, hipMemcpyHostToDevice, s));
+    const bool inv_lds = lds_for(d, t.real_bytes, true) <= kMaxLdsBytes;
+    const size_t lds = lds_for(d, t.real_bytes, inv_lds);
+    const dim3 grid(static_cast<unsigned>(C)), block(kWave);
+    if (t.real_bytes == 8) {
+        if (inv_lds)
+            hipLaunchKernelGGL((k_frozen<double, true>), grid, block, lds, s, d, n_configs, d_pos, d_mv, d_new, d_sc, d_gl, d_ib, d_ia, d_sk, reinterpret_cast<double *>(d_inv), d_A, d_skc);
+        else
+            hipLaunchKernelGGL((k_frozen<double, false>), grid, block, lds, s, d, n_configs, d_pos, d_mv, d_new, d_sc, d_gl, d_ib, d_ia, d_sk, reinterpret_cast<double *>(d_inv), d_A, d_skc);
+    } else {
+        if (inv_lds)
+            hipLaunchKernelGGL((k_frozen<float, true>), grid, block, lds, s, d, n_configs, d_pos, d_mv, d_new, d_sc, d_gl, d_ib, d_ia, d_sk, reinterpret_cast<float *>(d_inv), d_A, d_skc);
+        else
+            hipLaunchKernelGGL((k_frozen<float, false>), grid, block, lds, s, d, n_configs, d_pos, d_mv, d_new, d_sc, d_gl, d_ib, d_ia, d_sk, reinterpret_cast<float *>(d_inv), d_A, d_skc);
+    }
+    VMC_HIP_TRY(hipGetLastError());
+    VMC_HIP_TRY(hipMemcpyAsync(scalars, d_sc, sizeof(double) * C * VMC_FZ_COUNT, hipMemcpyDeviceToHost, s));
+    if (grad_lap) VMC_HIP_TRY(hipMemcpyAsync(grad_lap, d_gl, sizeof(double) * C * 4 * d.N, hipMemcpyDeviceToHost, s));
+    if (inv_before) VMC_HIP_TRY(hipMemcpyAsync(inv_before, d_ib, sizeof(double) * C * d.ninv, hipMemcpyDeviceToHost, s));
+    if (inv_after) VMC_HIP_TRY(hipMemcpyAsync(inv_after, d_ia, sizeof(double) * C * d.ninv, hipMemcpyDeviceToHost, s));
+    if (s_k) VMC_HIP_TRY(hipMemcpyAsync(s_k, d_sk, sizeof(double) * C * 2 * d.ng, hipMemcpyDeviceToHost, s));
+    VMC_HIP_TRY(hipStreamSynchronize(s));
+    return VMC_OK;
+}
+
+}  // extern "C"

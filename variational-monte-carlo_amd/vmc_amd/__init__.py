@@ -1,0 +1,301 @@
+"""Python binding of libvmc_hip.so (include/vmc_hip.h) over ctypes.
+
+This is the host-side mirror used by the tests, the benchmark and the smoke
+check.  It calls the gfx950 library only; there is no CPU fallback, and every
+entry point raises if the library or a gfx950 device is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import re
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libvmc_hip.so")
+HEADER_PATH = os.path.join(REPO_DIR, "include", "vmc_hip.h")
+
+VMC_OK = 0
+ERRORS = {1: "invalid argument", 2: "device error", 3: "singular", 4: "out of memory",
+          5: "bad state"}
+
+PREC_FP64, PREC_MIXED = 0, 1
+MOVE_UNIFORM, MOVE_GAUSSIAN = 0, 1
+ERFC_AS7126, ERFC_EXACT = 0, 1
+SWEEP_MEASURE, SWEEP_ADAPT = 1, 2
+
+STAT = dict(NMEAS=0, SUM_E=1, SUM_E2=2, SUM_KIN=3, SUM_POT=4, LAST_E=5, ACCEPTED=6,
+            PROPOSED=7, BLK_SUM=8, BLK_IN=9, BLK_N=10, BLK_MEAN=11, BLK_M2=12, LAST_KIN=13,
+            LAST_POT=14, REINV=15)
+STAT_COUNT = 16
+FZ = dict(LOGDET0=0, LOGDET1=1, JASTROW=2, KINETIC=3, VREAL=4, VRECIP=5, ELOCAL=6, RATIO=7,
+          DELTA_J=8, VREAL_INC=9, VRECIP_INC=10, ELOCAL_AFTER=11, KINETIC_AFTER=12,
+          JASTROW_AFTER=13, SINGULAR=14)
+FZ_COUNT = 15
+
+
+class VmcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vmc error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class SystemDesc(C.Structure):
+    """Mirror of vmc_system."""
+
+    _fields_ = [
+        ("n_up", C.c_int32),
+        ("n_dn", C.c_int32),
+        ("box_length", C.c_double),
+        ("jastrow_a_same", C.c_double),
+        ("jastrow_a_anti", C.c_double),
+        ("jastrow_b", C.c_double),
+        ("ewald_alpha", C.c_double),
+        ("recip_nmax2", C.c_int32),
+        ("real_cutoff", C.c_double),
+        ("erfc_kind", C.c_int32),
+        ("move_kind", C.c_int32),
+        ("precision", C.c_int32),
+        ("reinvert_interval", C.c_int32),
+    ]
+
+
+def box_length_from_rs(rs, n):
+    return float(np.cbrt(4.0 * np.pi * n / 3.0) * rs)
+
+
+@dataclass
+class System:
+    """One homogeneous-electron-gas system (vmc_system)."""
+
+    n_up: int
+    n_dn: int
+    box_length: float
+    a_same: float = 0.25
+    a_anti: float = 0.5
+    b: float = 1.0
+    alpha: float = 0.0
+    recip_nmax2: int = 0
+    real_cutoff: float = 0.0
+    erfc_kind: int = ERFC_AS7126
+    move_kind: int = MOVE_UNIFORM
+    precision: int = PREC_FP64
+    reinvert_interval: int = 0
+
+    @property
+    def N(self):
+        return self.n_up + self.n_dn
+
+    @classmethod
+    def reference(cls, n, L, a=0.25, b=1.0, precision=PREC_FP64):
+        """The reference's own physics: one polarised determinant, alpha = 6/L,
+        tolerance-rule G vectors, all minimum-image pairs, A&S erfc,
+        uniform-cube moves."""
+        return cls(n, 0, L, a, a, b, 0.0, 0, 0.0, ERFC_AS7126, MOVE_UNIFORM, precision, 0)
+
+    @classmethod
+    def heg(cls, n_total, rs, b, precision=PREC_FP64, reinvert_interval=0, a_same=0.25,
+            a_anti=0.5, nmax2=25):
+        """Unpolarised HEG: two determinants of N/2, alpha = 5/L, rcut = L/2,
+        |n|^2 <= nmax2 reciprocal shells, exact erfc, Gaussian moves."""
+        L = box_length_from_rs(rs, n_total)
+        return cls(n_total // 2, n_total - n_total // 2, L, a_same, a_anti, b, 5.0 / L, nmax2,
+                   0.5 * L, ERFC_EXACT, MOVE_GAUSSIAN, precision, reinvert_interval)
+
+    def desc(self):
+        return SystemDesc(self.n_up, self.n_dn, self.box_length, self.a_same, self.a_anti, self.b,
+                          self.alpha, self.recip_nmax2, self.real_cutoff, self.erfc_kind,
+                          self.move_kind, self.precision, self.reinvert_interval)
+
+
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_VP = C.c_void_p
+
+
+def _ptr(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def declared_functions():
+    """Names of every function include/vmc_hip.h declares."""
+    src = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(vmc_\w+)\s*\(", src, re.M)))
+
+
+def lib():
+    """Load libvmc_hip.so; raises if it is missing (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               f"`make -C {PKG_DIR}` (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        sp = C.POINTER(SystemDesc)
+        sig = {
+            "vmc_last_error": (C.c_char_p, []),
+            "vmc_device_check": (C.c_int, [_ip]),
+            "vmc_system_sizes": (C.c_int, [sp, _ip, _ip, _ip]),
+            "vmc_ensemble_create": (C.c_int, [sp, C.c_int64, C.c_uint64, C.c_double,
+                                              C.POINTER(_VP)]),
+            "vmc_ensemble_destroy": (C.c_int, [_VP]),
+            "vmc_ensemble_init_positions": (C.c_int, [_VP, _VP]),
+            "vmc_ensemble_set_positions": (C.c_int, [_VP, _dp, _VP]),
+            "vmc_ensemble_get_positions": (C.c_int, [_VP, _dp, _VP]),
+            "vmc_ensemble_sweep": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _VP]),
+            "vmc_ensemble_reset_stats": (C.c_int, [_VP, _VP]),
+            "vmc_ensemble_read_stats": (C.c_int, [_VP, _dp, _VP]),
+            "vmc_ensemble_read_steps": (C.c_int, [_VP, _dp, _VP]),
+            "vmc_ensemble_info": (C.c_int, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
+            "vmc_frozen_eval": (C.c_int, [sp, C.c_int32, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _dp,
+                                          _VP]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != VMC_OK:
+        raise VmcError(rc, lib().vmc_last_error().decode())
+
+
+def device_check():
+    n = C.c_int32(0)
+    _check(lib().vmc_device_check(C.byref(n)))
+    return n.value
+
+
+def system_sizes(system: System):
+    nk = (C.c_int32 * 2)()
+    ng, ninv = C.c_int32(), C.c_int32()
+    d = system.desc()
+    _check(lib().vmc_system_sizes(C.byref(d), nk, C.byref(ng), C.byref(ninv)))
+    return (nk[0], nk[1]), ng.value, ninv.value
+
+
+class Ensemble:
+    """Device-resident walker ensemble (vmc_ensemble)."""
+
+    def __init__(self, system: System, n_walkers: int, seed: int, step0: float | None = None,
+                 stream=None):
+        self.system = system
+        self.W = int(n_walkers)
+        self.N = system.N
+        self.seed = seed
+        self.stream = stream
+        self._h = _VP()
+        self._desc = system.desc()
+        step = system.box_length / 10.0 if step0 is None else step0
+        _check(lib().vmc_ensemble_create(C.byref(self._desc), self.W, seed, step,
+                                         C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().vmc_ensemble_destroy(self._h)
+            self._h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_positions(self):
+        _check(lib().vmc_ensemble_init_positions(self._h, self.stream))
+
+    def set_positions(self, pos):
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.W, self.N, 3)
+        _check(lib().vmc_ensemble_set_positions(self._h, _ptr(p), self.stream))
+
+    def positions(self):
+        p = np.zeros((self.W, self.N, 3))
+        _check(lib().vmc_ensemble_get_positions(self._h, _ptr(p), self.stream))
+        return p
+
+    def sweep(self, n_sweeps=1, measure=False, adapt=False, block_size=1):
+        flags = (SWEEP_MEASURE if measure else 0) | (SWEEP_ADAPT if adapt else 0)
+        _check(lib().vmc_ensemble_sweep(self._h, n_sweeps, flags, block_size, self.stream))
+
+    def reset_stats(self):
+        _check(lib().vmc_ensemble_reset_stats(self._h, self.stream))
+
+    def stats(self):
+        s = np.zeros((self.W, STAT_COUNT))
+        _check(lib().vmc_ensemble_read_stats(self._h, _ptr(s), self.stream))
+        return s
+
+    def steps(self):
+        s = np.zeros(self.W)
+        _check(lib().vmc_ensemble_read_steps(self._h, _ptr(s), self.stream))
+        return s
+
+    def sweeps_done(self):
+        w, s = C.c_int64(), C.c_uint64()
+        _check(lib().vmc_ensemble_info(self._h, C.byref(w), C.byref(s)))
+        return s.value
+
+
+@dataclass
+class FrozenResult:
+    scalars: np.ndarray
+    grad_lap: np.ndarray
+    inv_before: list
+    inv_after: list
+    s_k: np.ndarray
+
+    def __getitem__(self, key):
+        return self.scalars[:, FZ[key]]
+
+
+def frozen_eval(system: System, pos, moved, newpos, stream=None):
+    """Frozen-configuration evaluation (vmc_frozen_eval) of C configurations."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    Cn = pos.shape[0]
+    N = system.N
+    pos = pos.reshape(Cn, N, 3)
+    moved = np.ascontiguousarray(moved, dtype=np.int32).reshape(Cn)
+    newpos = np.ascontiguousarray(newpos, dtype=np.float64).reshape(Cn, 3)
+    (_, _), ng, ninv = system_sizes(system)
+    sc = np.zeros((Cn, FZ_COUNT))
+    gl = np.zeros((Cn, N, 4))
+    ib = np.zeros((Cn, ninv))
+    ia = np.zeros((Cn, ninv))
+    sk = np.zeros((Cn, max(ng, 1), 2))
+    d = system.desc()
+    _check(lib().vmc_frozen_eval(C.byref(d), Cn, _ptr(pos), _ptr(moved, _ip), _ptr(newpos),
+                                 _ptr(sc), _ptr(gl), _ptr(ib), _ptr(ia), _ptr(sk), stream))
+    n0, n1 = system.n_up, system.n_dn
+
+    def split(a):
+        out = [a[:, : n0 * n0].reshape(Cn, n0, n0)]
+        if n1:
+            out.append(a[:, n0 * n0:].reshape(Cn, n1, n1))
+        return out
+
+    return FrozenResult(sc, gl, split(ib), split(ia), sk[:, :ng])
+
+
+def blocking_summary(stats):
+    """Mean of per-walker blocked means and the pooled standard error
+    sqrt(sum SE_w^2) / W, as main.cpp:70-104 aggregates thread summaries."""
+    nb = stats[:, STAT["BLK_N"]]
+    ok = nb >= 2
+    means = np.where(ok, stats[:, STAT["BLK_MEAN"]], stats[:, STAT["SUM_E"]] /
+                     np.maximum(stats[:, STAT["NMEAS"]], 1))
+    mean = float(np.mean(means))
+    if not np.any(ok):
+        return mean, None
+    var = stats[ok, STAT["BLK_M2"]] / (nb[ok] - 1)
+    se_w = np.sqrt(var / nb[ok])
+    se = float(math.sqrt(np.sum(se_w ** 2)) / stats.shape[0])
+    return mean, se
