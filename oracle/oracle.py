@@ -158,6 +158,8 @@ def lib():
             "orc_init_positions": (C.c_int, [sp, C.c_uint64, C.c_uint64, _dp]),
             "orc_sweep": (C.c_int, [W, C.POINTER(SweepConfig), C.c_uint64, C.c_uint64, _dp,
                                     C.c_int, _dp]),
+            "orc_bench_threads": (C.c_double, [sp, C.POINTER(SweepConfig), C.c_int, C.c_int,
+                                               C.c_int, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -455,3 +457,13 @@ def ref_simulation_run(n, L, warmup_steps, measure_steps, step, seed, block, a=0
     return dict(mean=m.value, se=s.value, has_se=bool(hs.value), acc=a_.value,
                 frames_e=fe[:measure_steps], frames_mean=fm[:measure_steps],
                 frames_se=fs[:measure_steps], frames_acc=fa[:measure_steps], frames_pos=fp)
+
+
+def bench_threads(sysd, seed, step, threads, sweeps, measure=True, move_kind=1):
+    """CPU baseline: `threads` independent chains of `sweeps` sweeps each.
+    Returns (moves per second, wall seconds)."""
+    cfg = SweepConfig(seed, move_kind, step)
+    el = C.c_double()
+    rate = lib().orc_bench_threads(C.byref(sysd), C.byref(cfg), threads, sweeps, int(measure),
+                                   C.byref(el))
+    return rate, el.value

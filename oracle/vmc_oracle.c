@@ -1206,3 +1206,73 @@ int orc_sweep(orc_walker *w, const orc_sweep_config *cfg, uint64_t walker, uint6
     }
     return accepted;
 }
+
+/* ============================================== threaded CPU baseline timing */
+
+#include <pthread.h>
+#include <time.h>
+
+typedef struct {
+    const orc_system *sys;
+    orc_sweep_config cfg;
+    uint64_t walker;
+    int sweeps;
+    int measure;
+    double seconds;
+    long long moves;
+    double e_sum;
+} bench_job;
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static void *bench_thread(void *arg) {
+    bench_job *j = (bench_job *)arg;
+    double *pos = (double *)malloc(sizeof(double) * 3 * (size_t)(j->sys->n_up + j->sys->n_dn));
+    orc_init_positions(j->sys, j->cfg.seed, j->walker, pos);
+    struct orc_walker *w = orc_walker_new(j->sys);
+    orc_walker_set_positions(w, pos);
+    double step = j->cfg.step0;
+    const double t0 = now_s();
+    for (int s = 0; s < j->sweeps; ++s) {
+        double e = 0.0;
+        orc_sweep(w, &j->cfg, j->walker, (uint64_t)s, &step, 0, j->measure ? &e : NULL);
+        j->e_sum += e;
+    }
+    j->seconds = now_s() - t0;
+    j->moves = (long long)j->sweeps * w->N;
+    orc_walker_free(w);
+    free(pos);
+    return NULL;
+}
+
+/* One independent chain per thread (main.cpp:50-68), each `sweeps` sweeps of
+   orc_sweep with a measurement per sweep when measure != 0.  Returns the
+   aggregate proposed moves per second over the slowest thread's time. */
+double orc_bench_threads(const orc_system *sys, const orc_sweep_config *cfg, int threads,
+                         int sweeps, int measure, double *elapsed) {
+    pthread_t *tid = (pthread_t *)calloc((size_t)threads, sizeof(pthread_t));
+    bench_job *jobs = (bench_job *)calloc((size_t)threads, sizeof(bench_job));
+    const double t0 = now_s();
+    for (int t = 0; t < threads; ++t) {
+        jobs[t].sys = sys;
+        jobs[t].cfg = *cfg;
+        jobs[t].walker = (uint64_t)t;
+        jobs[t].sweeps = sweeps;
+        jobs[t].measure = measure;
+        pthread_create(&tid[t], NULL, bench_thread, &jobs[t]);
+    }
+    long long moves = 0;
+    for (int t = 0; t < threads; ++t) {
+        pthread_join(tid[t], NULL);
+        moves += jobs[t].moves;
+    }
+    const double wall = now_s() - t0;
+    if (elapsed) *elapsed = wall;
+    free(tid);
+    free(jobs);
+    return (double)moves / wall;
+}

@@ -173,6 +173,12 @@ int orc_init_positions(const orc_system *sys, uint64_t seed, uint64_t walker, do
 int orc_sweep(orc_walker *w, const orc_sweep_config *cfg, uint64_t walker, uint64_t sweep,
               double *step, int adapt, double *e_local);
 
+/* Threaded CPU baseline: one independent chain per thread (main.cpp:50-68),
+   `sweeps` sweeps each, measuring every sweep when measure != 0.  Returns
+   aggregate proposed moves per second; *elapsed gets the wall time. */
+double orc_bench_threads(const orc_system *sys, const orc_sweep_config *cfg, int threads,
+                         int sweeps, int measure, double *elapsed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,3 +28,12 @@ def vmc():
 
     vmc_amd.device_check()
     return vmc_amd
+
+
+@pytest.fixture(scope="session")
+def vmc_nodev():
+    """The binding without a device check (CPU-side C ABI tests)."""
+    import vmc_amd
+
+    vmc_amd.lib()
+    return vmc_amd
