@@ -45,6 +45,9 @@ enum { VMC_PREC_FP64 = 0, VMC_PREC_MIXED = 1 };
 enum { VMC_MOVE_UNIFORM = 0, VMC_MOVE_GAUSSIAN = 1 };
 enum { VMC_ERFC_AS7126 = 0, VMC_ERFC_EXACT = 1 };
 
+/* Sweep kernel selection (vmc_ensemble_set_kernel). */
+enum { VMC_KERNEL_AUTO = 0, VMC_KERNEL_GENERIC = 1, VMC_KERNEL_REGISTER = 2 };
+
 /* Sweep flags. */
 enum { VMC_SWEEP_MEASURE = 1, VMC_SWEEP_ADAPT = 2 };
 
@@ -154,10 +157,30 @@ int vmc_ensemble_get_positions(vmc_ensemble *e, double *pos, void *stream);
 int vmc_ensemble_sweep(vmc_ensemble *e, int32_t n_sweeps, int32_t flags, int32_t block_size,
                        void *stream);
 
+/* Choose the sweep kernel: VMC_KERNEL_GENERIC keeps the inverses in LDS (or
+ * HBM) for any N; VMC_KERNEL_REGISTER keeps each inverse row in a lane's
+ * registers (at most 32 electrons per spin, equal spins or polarised).
+ * VMC_KERNEL_AUTO (default) takes the register kernel whenever it applies.
+ * No reference counterpart: the reference has a single CPU code path; this
+ * is the autotuner's knob.  VMC_ERR_INVALID if the system does not fit the
+ * requested kernel. */
+int vmc_ensemble_set_kernel(vmc_ensemble *e, int32_t kernel);
+/* The kernel the next sweep launch will use (VMC_KERNEL_GENERIC or _REGISTER). */
+int vmc_ensemble_get_kernel(const vmc_ensemble *e, int32_t *kernel);
+/* Register-kernel build variant: 0 = unbounded registers, 1 = at most 128
+ * VGPRs (4 waves per SIMD, some spills).  No reference counterpart; an
+ * autotuner knob. */
+int vmc_ensemble_set_rr_variant(vmc_ensemble *e, int32_t variant);
+
 /* Zero the per-walker statistics (start of a measurement phase). */
 int vmc_ensemble_reset_stats(vmc_ensemble *e, void *stream);
 /* Copy per-walker statistics to host: [W][VMC_STAT_COUNT]. */
 int vmc_ensemble_read_stats(vmc_ensemble *e, double *stats, void *stream);
+/* Copy per-walker inverse drift to host: [W], the largest max-norm of
+ * I - D^-1 D met when a walker's mixed-precision inverse came due for its
+ * FP64 re-inversion (register kernel; 0 until one happens).  No reference
+ * counterpart: the reference works in FP64 throughout and never re-inverts. */
+int vmc_ensemble_read_drift(vmc_ensemble *e, double *drift, void *stream);
 /* Copy per-walker step sizes to host: [W]. */
 int vmc_ensemble_read_steps(vmc_ensemble *e, double *steps, void *stream);
 /* Global sweep counter (the Philox stream position) and walker count. */

@@ -39,12 +39,19 @@ def test_init_positions_match_oracle(vmc, oracle, name, factory):
         np.testing.assert_array_equal(gpu[w], pos)
 
 
+KERNELS = [("generic", 1, 0), ("register", 2, 0), ("register_o4", 2, 1)]
+
+
+@pytest.mark.parametrize("kname,kernel,variant", KERNELS, ids=[k[0] for k in KERNELS])
 @pytest.mark.parametrize("name,factory", SYSTEMS, ids=[s[0] for s in SYSTEMS])
-def test_fp64_sweeps_follow_oracle_twin(vmc, oracle, name, factory):
+def test_fp64_sweeps_follow_oracle_twin(vmc, oracle, name, factory, kname, kernel, variant):
     s = factory(vmc)
     W, seed, sweeps = 4, 987654321, 6
     step = 0.3 * s.box_length / 10.0 * 3.0
     ens = vmc.Ensemble(s, W, seed, step0=step)
+    ens.set_kernel(kernel)
+    ens.set_rr_variant(variant)
+    assert ens.kernel() == kernel
     ens.init_positions()
     osys = oracle_system(oracle, s)
     walkers = []
@@ -66,11 +73,13 @@ def test_fp64_sweeps_follow_oracle_twin(vmc, oracle, name, factory):
     assert np.all(ens.stats()[:, vmc.STAT["PROPOSED"]] == sweeps * s.N)
 
 
-def test_adaptation_matches_reference_rule(vmc, oracle):
+@pytest.mark.parametrize("kernel", [1, 2], ids=["generic", "register"])
+def test_adaptation_matches_reference_rule(vmc, oracle, kernel):
     """Warmup step rule s *= exp(0.25 (acc - 0.5)), s <= L/2 (simulation.cpp:178-194)."""
     s = vmc.System.heg(14, 1.0, 1.0)
     W, seed = 3, 5
     ens = vmc.Ensemble(s, W, seed, step0=0.05)
+    ens.set_kernel(kernel)
     ens.init_positions()
     osys = oracle_system(oracle, s)
     walkers = [oracle.Walker(osys, oracle.init_positions(osys, seed, w)[0]) for w in range(W)]
@@ -83,11 +92,13 @@ def test_adaptation_matches_reference_rule(vmc, oracle):
     np.testing.assert_allclose(ens.steps(), steps, rtol=1e-12)
 
 
-def test_zero_step_accepts_everything_and_keeps_energy(vmc):
+@pytest.mark.parametrize("kernel", [1, 2], ids=["generic", "register"])
+def test_zero_step_accepts_everything_and_keeps_energy(vmc, kernel):
     """Zero-step proposals are always accepted and leave E_L unchanged
     (test_simulation.cpp:91-109)."""
     s = vmc.System.reference(7, 6.0)
     ens = vmc.Ensemble(s, 8, 314159, step0=0.0)
+    ens.set_kernel(kernel)
     ens.init_positions()
     ens.sweep(1, measure=True)
     e0 = ens.stats()[:, vmc.STAT["LAST_E"]].copy()
@@ -105,3 +116,86 @@ def test_large_step_rejects(vmc):
     ens.sweep(15)
     st = ens.stats()
     assert np.all(st[:, vmc.STAT["ACCEPTED"]] < st[:, vmc.STAT["PROPOSED"]])
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+def test_register_and_generic_kernels_agree(vmc, prec):
+    """Both sweep kernels on the same Philox streams: FP64 chains coincide to
+    rounding; mixed chains (different FP32 operation orders) stay statistically
+    equal and their local energies agree to the mixed budget while the
+    trajectories coincide."""
+    s = vmc.System.heg(54, 2.0, 1.0, precision=prec, reinvert_interval=64 if prec else 0)
+    W, seed = 64, 4242
+    out = []
+    for kernel in (1, 2):
+        ens = vmc.Ensemble(s, W, seed, step0=0.6)
+        ens.set_kernel(kernel)
+        ens.init_positions()
+        ens.sweep(3, measure=True)
+        out.append((ens.positions(), ens.stats()))
+    (p1, s1), (p2, s2) = out
+    if prec == 0:
+        np.testing.assert_allclose(p1, p2, atol=1e-9)
+        np.testing.assert_allclose(s1[:, vmc.STAT["LAST_E"]], s2[:, vmc.STAT["LAST_E"]], rtol=1e-9,
+                                   atol=1e-9)
+        assert np.array_equal(s1[:, vmc.STAT["ACCEPTED"]], s2[:, vmc.STAT["ACCEPTED"]])
+    else:
+        close = np.abs(p1 - p2).reshape(W, -1).max(1) < 1e-4
+        assert close.mean() > 0.8
+        e1, e2 = s1[close, vmc.STAT["LAST_E"]], s2[close, vmc.STAT["LAST_E"]]
+        assert np.max(np.abs(e1 - e2) / np.maximum(np.abs(e1), 1.0)) < 1e-4
+
+
+def test_fp64_reinversion_keeps_the_oracle_twin(vmc, oracle):
+    """The post-sweep FP64 re-inversion (Newton-Schulz on the matrix cores)
+    replaces an inverse that the reference keeps by Sherman-Morrison only
+    (slater_plane_wave.cpp accept path); in FP64 both are the same matrix to
+    rounding, so a chain that re-inverts every 16 accepted moves still follows
+    the oracle twin move for move, and the drift found before each
+    re-inversion is FP64 rounding."""
+    s = vmc.System.heg(54, 2.0, 1.0, reinvert_interval=16)
+    W, seed, sweeps = 4, 77, 6
+    step = 0.6
+    ens = vmc.Ensemble(s, W, seed, step0=step)
+    ens.set_kernel(2)
+    ens.init_positions()
+    osys = oracle_system(oracle, s)
+    walkers = [oracle.Walker(osys, oracle.init_positions(osys, seed, w)[0]) for w in range(W)]
+    for sweep in range(sweeps):
+        ens.sweep(1, measure=True, adapt=False)
+        st = ens.stats()
+        gpu_pos = ens.positions()
+        for w in range(W):
+            _, _, e = walkers[w].sweep(seed, w, sweep, step, move_kind=s.move_kind, adapt=False,
+                                       measure=True)
+            np.testing.assert_allclose(gpu_pos[w], walkers[w].positions(), rtol=0, atol=1e-9)
+            assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-9 * max(abs(e), 1e-3 * s.N)
+    st = ens.stats()
+    assert np.all(st[:, vmc.STAT["REINV"]] >= 1)
+    d = ens.drift()
+    assert np.all(d > 0) and np.all(d < 1e-11), d
+
+
+def test_mixed_reinversion_bounds_the_drift(vmc, oracle):
+    """Mixed precision: FP32 Sherman-Morrison drift is measured at every FP64
+    re-inversion (max |I - D^-1 D| over the K accepted moves before it) and
+    stays far below the mixed-precision budget; the local energy after many
+    sweeps still matches the FP64 oracle at the walker's own positions to
+    that budget (relative 1e-5 of the sum of the magnitudes of its parts)."""
+    s = vmc.System.heg(54, 2.0, 1.0, precision=1, reinvert_interval=64)
+    W, seed = 64, 2026
+    ens = vmc.Ensemble(s, W, seed, step0=0.6)
+    ens.set_kernel(2)
+    ens.init_positions()
+    ens.sweep(40, measure=True, adapt=False)
+    st = ens.stats()
+    assert np.all(st[:, vmc.STAT["REINV"]] >= 5)
+    d = ens.drift()
+    assert np.all(d > 0) and np.all(d < 1e-3), d.max()
+    pos = ens.positions()
+    osys = oracle_system(oracle, s)
+    for w in range(0, W, 8):
+        wk = oracle.Walker(osys, pos[w])
+        e, kin, _ = wk.local_energy()
+        scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
+        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale, (w, st[w, vmc.STAT["LAST_E"]], e)

@@ -43,6 +43,20 @@ struct DevSys {
     const double *gw;    // [ng] weights 8 pi^2 / G^2 exp(-G^2 / 4 alpha^2)
 };
 
+// Per-ensemble device state (walker-major SoA in HBM).
+struct DevState {
+    int W;
+    double *pos;      // [W][3][N]
+    void *inv;        // Real [W][ninv]
+    double *sigma;    // [W]
+    double *stats;    // [W][kStatCount]
+    double *logdet;   // [W][2]
+    int *since;       // [W] accepted moves since the last FP64 re-inversion
+    int *flags;       // [W] bit0: singular determinant met
+    double *scratch;  // [W][nmax^2] FP64 workspace
+    double *drift;    // [W] largest |I - D^-1 D| met before a re-inversion
+};
+
 // -------------------------------------------------------------- reductions
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -123,6 +137,28 @@ __device__ inline double rsqrt_r<double>(double x) { return 1.0 / sqrt(x); }
 template <>
 __device__ inline float rsqrt_r<float>(float x) { return 1.0f / sqrtf(x); }
 
+// Working-precision sqrt, reciprocal and exp: IEEE in double; in float the
+// hardware v_sqrt_f32 / v_rcp_f32 / v_exp_f32 (about 1 ulp), well inside the
+// mixed path's error budget.
+template <typename Real>
+__device__ __forceinline__ Real wsqrt(Real x);
+template <>
+__device__ __forceinline__ double wsqrt<double>(double x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float wsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <typename Real>
+__device__ __forceinline__ Real wrcp(Real x);
+template <>
+__device__ __forceinline__ double wrcp<double>(double x) { return 1.0 / x; }
+template <>
+__device__ __forceinline__ float wrcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <typename Real>
+__device__ __forceinline__ Real wexp(Real x);
+template <>
+__device__ __forceinline__ double wexp<double>(double x) { return exp(x); }
+template <>
+__device__ __forceinline__ float wexp<float>(float x) { return __expf(x); }
+
 // Abramowitz & Stegun 7.1.26 with the reference's constants
 // (energy_tracking.cpp:101-107,133-137).
 template <typename Real>
@@ -130,9 +166,9 @@ __device__ inline Real erfc_as7126(Real x) {
     const Real p = (Real)0.3275911;
     const Real a1 = (Real)0.254829592, a2 = (Real)-0.284496736, a3 = (Real)1.421413741,
                a4 = (Real)-1.453152027, a5 = (Real)1.061405429;
-    const Real t = (Real)1 / ((Real)1 + p * x);
+    const Real t = wrcp<Real>((Real)1 + p * x);
     const Real tau = t * (a1 + t * (a2 + t * (a3 + t * (a4 + t * a5))));
-    return tau * exp(-x * x);
+    return tau * wexp<Real>(-x * x);
 }
 
 template <typename Real>

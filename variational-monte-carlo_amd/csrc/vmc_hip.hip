@@ -10,6 +10,7 @@
 //   since   int    [W]         accepted moves since the last FP64 re-inversion
 //   flags   int    [W]         bit0: singular determinant met
 //   scratch double [W][nmax^2] FP64 Gauss-Jordan workspace
+//   drift   double [W]         largest |I - D^-1 D| met at a re-inversion
 // One wavefront (one 64-lane workgroup) runs one walker; its positions,
 // Jastrow row sums and draws stay in LDS for a whole launch, and so do the
 // inverses when they fit.
@@ -17,6 +18,7 @@
 #include "vmc_device.hpp"
 #include "vmc_tables.hpp"
 #include "vmc_walker.hpp"
+#include "vmc_rr_launch.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -30,18 +32,6 @@
 #include <vector>
 
 namespace vmc {
-
-struct DevState {
-    int W;
-    double *pos;
-    void *inv;
-    double *sigma;
-    double *stats;
-    double *logdet;
-    int *since;
-    int *flags;
-    double *scratch;
-};
 
 constexpr size_t kMaxLdsBytes = 64 * 1024;
 
@@ -492,6 +482,8 @@ struct vmc_ensemble {
     uint64_t sweeps_done = 0;
     double step0 = 0.0;
     bool positions_ready = false;
+    int kernel = VMC_KERNEL_AUTO;
+    int rr_variant = 0;
 
     ~vmc_ensemble() {
         if (st.pos) (void)hipFree(st.pos);
@@ -502,6 +494,7 @@ struct vmc_ensemble {
         if (st.since) (void)hipFree(st.since);
         if (st.flags) (void)hipFree(st.flags);
         if (st.scratch) (void)hipFree(st.scratch);
+        if (st.drift) (void)hipFree(st.drift);
     }
 };
 
@@ -517,9 +510,21 @@ int launch_init(vmc_ensemble *e, int draw, hipStream_t stream) {
     return VMC_OK;
 }
 
+bool rr_supported(const DevSys &d) { return rr_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng); }
+
 template <typename Real>
 int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipStream_t stream) {
     const DevSys &d = e->tables.dev;
+    if (e->kernel != VMC_KERNEL_GENERIC && rr_supported(d)) {
+        const unsigned grid = static_cast<unsigned>(e->W);
+        const unsigned long long s0 = (unsigned long long)e->sweeps_done;
+        const bool ok = sizeof(Real) == 4
+                            ? rr_launch_f32(d, e->st, grid, n_sweeps, s0, flags, block_size, e->rr_variant, stream)
+                            : rr_launch_f64(d, e->st, grid, n_sweeps, s0, flags, block_size, e->rr_variant, stream);
+        if (!ok) return fail(VMC_ERR_INVALID, "no register-resident instantiation for this system");
+        VMC_HIP_TRY(hipGetLastError());
+        return VMC_OK;
+    }
     const size_t lds_in = lds_for(d, sizeof(Real), true);
     const unsigned grid = static_cast<unsigned>(e->W);
     if (lds_in <= kMaxLdsBytes) {
@@ -590,7 +595,7 @@ int vmc_ensemble_create(const vmc_system *sys, int64_t n_walkers, uint64_t seed,
     if ((rc = dev_alloc(&st.pos, W * 3 * d.N)) || (rc = dev_alloc(reinterpret_cast<unsigned char **>(&st.inv), W * d.ninv * e->tables.real_bytes)) ||
         (rc = dev_alloc(&st.sigma, W)) || (rc = dev_alloc(&st.stats, W * kStatCount)) ||
         (rc = dev_alloc(&st.logdet, W * 2)) || (rc = dev_alloc(&st.since, W)) ||
-        (rc = dev_alloc(&st.flags, W)) || (rc = dev_alloc(&st.scratch, W * d.nmax * d.nmax))) {
+        (rc = dev_alloc(&st.flags, W)) || (rc = dev_alloc(&st.scratch, W * d.nmax * d.nmax)) || (rc = dev_alloc(&st.drift, W))) {
         delete e;
         return rc;
     }
@@ -599,7 +604,8 @@ int vmc_ensemble_create(const vmc_system *sys, int64_t n_walkers, uint64_t seed,
         hipMemset(st.stats, 0, sizeof(double) * W * kStatCount) != hipSuccess ||
         hipMemset(st.since, 0, sizeof(int) * W) != hipSuccess ||
         hipMemset(st.flags, 0, sizeof(int) * W) != hipSuccess ||
-        hipMemset(st.logdet, 0, sizeof(double) * W * 2) != hipSuccess) {
+        hipMemset(st.logdet, 0, sizeof(double) * W * 2) != hipSuccess ||
+        hipMemset(st.drift, 0, sizeof(double) * W) != hipSuccess) {
         delete e;
         return fail(VMC_ERR_DEVICE, "ensemble initialisation copy failed");
     }
@@ -679,6 +685,30 @@ int vmc_ensemble_sweep(vmc_ensemble *e, int32_t n_sweeps, int32_t flags, int32_t
     return VMC_OK;
 }
 
+int vmc_ensemble_set_kernel(vmc_ensemble *e, int32_t kernel) {
+    if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
+    if (kernel != VMC_KERNEL_AUTO && kernel != VMC_KERNEL_GENERIC && kernel != VMC_KERNEL_REGISTER)
+        return fail(VMC_ERR_INVALID, "unknown kernel %d", kernel);
+    if (kernel == VMC_KERNEL_REGISTER && !rr_supported(e->tables.dev))
+        return fail(VMC_ERR_INVALID, "register kernel needs 7, 19 or 27 electrons per spin");
+    e->kernel = kernel;
+    return VMC_OK;
+}
+
+int vmc_ensemble_set_rr_variant(vmc_ensemble *e, int32_t variant) {
+    if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
+    if (variant != 0 && variant != 1) return fail(VMC_ERR_INVALID, "unknown variant %d", variant);
+    e->rr_variant = variant;
+    return VMC_OK;
+}
+
+int vmc_ensemble_get_kernel(const vmc_ensemble *e, int32_t *kernel) {
+    if (!e || !kernel) return fail(VMC_ERR_INVALID, "NULL argument");
+    *kernel = (e->kernel != VMC_KERNEL_GENERIC && rr_supported(e->tables.dev)) ? VMC_KERNEL_REGISTER
+                                                                               : VMC_KERNEL_GENERIC;
+    return VMC_OK;
+}
+
 int vmc_ensemble_reset_stats(vmc_ensemble *e, void *stream) {
     if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
     VMC_HIP_TRY(hipMemsetAsync(e->st.stats, 0, sizeof(double) * e->W * kStatCount, static_cast<hipStream_t>(stream)));
@@ -689,6 +719,14 @@ int vmc_ensemble_read_stats(vmc_ensemble *e, double *stats, void *stream) {
     if (!e || !stats) return fail(VMC_ERR_INVALID, "NULL argument");
     hipStream_t s = static_cast<hipStream_t>(stream);
     VMC_HIP_TRY(hipMemcpyAsync(stats, e->st.stats, sizeof(double) * e->W * kStatCount, hipMemcpyDeviceToHost, s));
+    VMC_HIP_TRY(hipStreamSynchronize(s));
+    return VMC_OK;
+}
+
+int vmc_ensemble_read_drift(vmc_ensemble *e, double *drift, void *stream) {
+    if (!e || !drift) return fail(VMC_ERR_INVALID, "NULL argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    VMC_HIP_TRY(hipMemcpyAsync(drift, e->st.drift, sizeof(double) * e->W, hipMemcpyDeviceToHost, s));
     VMC_HIP_TRY(hipStreamSynchronize(s));
     return VMC_OK;
 }

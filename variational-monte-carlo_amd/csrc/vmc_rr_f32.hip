@@ -1,0 +1,9 @@
+// vmc_rr_f32.hip -- register-resident sweep, mixed precision (FP32 working set).
+#include "vmc_rr_impl.hpp"
+
+namespace vmc {
+bool rr_launch_f32(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,
+                   unsigned long long s0, int flags, int bs, int variant, hipStream_t stream) {
+    return rr_launch<float>(d, st, grid, n_sweeps, s0, flags, bs, variant, stream);
+}
+}  // namespace vmc

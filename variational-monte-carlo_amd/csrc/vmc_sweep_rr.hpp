@@ -1,0 +1,998 @@
+// vmc_sweep_rr.hpp -- register-resident sweep for systems with at most 32
+// electrons per spin (N = 14, 38, 54 unpolarised; polarised up to 32).
+//
+// One wavefront runs one walker.  Lanes 0..31 own the rows of the spin-up
+// transposed inverse, lanes 32..63 those of spin down: lane (h, r) keeps
+// invT_h[r][0..n) in VGPRs together with electron r of spin h (position,
+// Jastrow row sum U, the sweep's proposal and its axis phasors).  Per move:
+//   - the proposal of electron p and its axis phasors e^{2 pi i u'} are read
+//     from lane p (v_readlane), every orbital of the row is a product of axis
+//     phasors (no transcendental on the move's critical path);
+//   - the orbital row is broadcast through 32 LDS words and every row lane
+//     forms s_k = sum_m phi_m invT[k][m]; the ratio is s_p (v_readlane);
+//   - the Jastrow new-pair sum is a DPP row reduction plus four readlanes, so
+//     the accept decision is a scalar branch;
+//   - on accept, row p is broadcast through LDS and each row lane applies the
+//     Sherman-Morrison update to its own registers.
+// The per-sweep local energy uses the same registers plus an LDS mirror of the
+// positions.
+#pragma once
+
+#include "../../include/vmc_hip.h"
+#include "vmc_device.hpp"
+#include "vmc_walker.hpp"
+
+#include <array>
+
+namespace vmc {
+namespace rr {
+
+struct Orb {
+    int x, y, z, t;
+};
+
+// Closed-shell filling at compile time: canonical n sorted by
+// (|n|^2, nx, ny, nz), orbital 0 = cos(0), then (cos, sin) per nonzero k
+// (slater_plane_wave.cpp:30-152; same rule as fill_shells in vmc_tables.hpp).
+template <int NO>
+__host__ __device__ constexpr std::array<Orb, (NO > 0 ? NO : 1)> shell_orbitals() {
+    constexpr int R = 3;
+    std::array<std::array<int, 4>, (2 * R + 1) * (2 * R + 1) * (2 * R + 1)> c{};
+    int nc = 0;
+    for (int x = -R; x <= R; ++x)
+        for (int y = -R; y <= R; ++y)
+            for (int z = -R; z <= R; ++z) {
+                const bool canon = x != 0 ? x > 0 : (y != 0 ? y > 0 : (z != 0 ? z > 0 : true));
+                if (canon) c[nc++] = {x * x + y * y + z * z, x, y, z};
+            }
+    for (int i = 1; i < nc; ++i) {
+        auto key = c[i];
+        int j = i - 1;
+        while (j >= 0 && (c[j][0] > key[0] || (c[j][0] == key[0] && (c[j][1] > key[1] || (c[j][1] == key[1] && (c[j][2] > key[2] || (c[j][2] == key[2] && c[j][3] > key[3]))))))) {
+            c[j + 1] = c[j];
+            --j;
+        }
+        c[j + 1] = key;
+    }
+    std::array<Orb, (NO > 0 ? NO : 1)> out{};
+    int o = 0;
+    for (int i = 0; i < nc && o < NO; ++i) {
+        out[o++] = Orb{c[i][1], c[i][2], c[i][3], 0};
+        if (c[i][0] != 0 && o < NO) out[o++] = Orb{c[i][1], c[i][2], c[i][3], 1};
+    }
+    return out;
+}
+
+template <int NO>
+__host__ __device__ constexpr int max_component() {
+    constexpr auto t = shell_orbitals<NO>();
+    int m = 0;
+    for (int i = 0; i < NO; ++i) {
+        const int a = t[i].x < 0 ? -t[i].x : t[i].x;
+        const int b = t[i].y < 0 ? -t[i].y : t[i].y;
+        const int c = t[i].z < 0 ? -t[i].z : t[i].z;
+        m = a > m ? a : m;
+        m = b > m ? b : m;
+        m = c > m ? c : m;
+    }
+    return m;
+}
+
+// ------------------------------------------------------- lane primitives
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// Sum over each row of 16 lanes, left in every lane of the row:
+// quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror.
+template <typename T>
+__device__ __forceinline__ T row_sum(T v) {
+    v += dpp<0xB1>(v);
+    v += dpp<0x4E>(v);
+    v += dpp<0x141>(v);
+    v += dpp<0x140>(v);
+    return v;
+}
+
+__device__ __forceinline__ float readlane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+__device__ __forceinline__ double readlane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// Whole-wave sum as a wave-uniform (scalar) value; all 64 lanes must be active.
+template <typename T>
+__device__ __forceinline__ T wave_total(T v) {
+    v = row_sum(v);
+    return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
+}
+
+template <typename Real>
+struct Cx {
+    Real re, im;
+};
+
+template <typename Real>
+__device__ __forceinline__ Cx<Real> cmul(Cx<Real> a, Cx<Real> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// a^n for |n| <= 2 given a and a^2 (conjugate for negative n).
+template <typename Real>
+__device__ __forceinline__ Cx<Real> cpow2(int n, Cx<Real> a1, Cx<Real> a2) {
+    const int m = n < 0 ? -n : n;
+    Cx<Real> r;
+    r.re = m == 0 ? (Real)1 : (m == 1 ? a1.re : a2.re);
+    r.im = m == 0 ? (Real)0 : (m == 1 ? a1.im : a2.im);
+    r.im = n < 0 ? -r.im : r.im;
+    return r;
+}
+
+// Axis phasor e^{2 pi i u} of a coordinate in [0, L).
+template <typename Real>
+__device__ __forceinline__ Cx<Real> axis_phasor(double x, double inv_L) {
+    double t = x * inv_L;
+    t -= rint(t);
+    Cx<Real> p;
+    sincos_2pi<Real>(t, &p.im, &p.re);
+    return p;
+}
+
+// Compile-time power a^n (n >= 0).
+template <int NPOW, typename Real>
+__device__ __forceinline__ Cx<Real> cpow_ct(Cx<Real> a) {
+    if constexpr (NPOW == 0) {
+        return Cx<Real>{(Real)1, (Real)0};
+    } else if constexpr (NPOW == 1) {
+        return a;
+    } else {
+        return cmul(a, cpow_ct<NPOW - 1, Real>(a));
+    }
+}
+
+template <typename Real>
+__device__ __forceinline__ Cx<Real> cconj(Cx<Real> a) {
+    return {a.re, -a.im};
+}
+
+// Orbital phasor e^{2 pi i n.u} from the three axis phasors and their squares.
+template <typename Real>
+__device__ __forceinline__ Cx<Real> orbital_phasor(int nx, int ny, int nz, const Cx<Real> a[3],
+                                                   const Cx<Real> a2[3]) {
+    return cmul(cmul(cpow2(nx, a[0], a2[0]), cpow2(ny, a[1], a2[1])), cpow2(nz, a[2], a2[2]));
+}
+
+template <typename Real>
+__device__ __forceinline__ Real jastrow_pair(Real r, Real a, Real b) {
+    return a * r / ((Real)1 + b * r);
+}
+
+}  // namespace rr
+
+// Maximum G vectors per lane of the register-resident measurement.
+constexpr int kRRMaxGPerLane = 12;
+// Electrons whose axis-power tables are staged in LDS at once.
+constexpr int kRRChunk = 8;
+// Highest power of an axis phasor the structure factor needs (|m| <= 7).
+constexpr int kRRMaxPow = 7;
+
+template <typename Real, int NU, int ND>
+struct RRShared {
+    double pos[3][kWave];          // positions by lane (pair loops of the measurement)
+    double prop[4][kWave];         // this sweep's proposal x, y, z and accept uniform
+    Real pph[6][kWave];            // axis phasors of the proposal (re, im per axis)
+    Real tab[kRRChunk * 3 * (kRRMaxPow + 1) * 2];  // axis powers of a chunk of electrons
+    double stat[kStatCount];
+    double misc[4];                // sigma, log|det| up, log|det| down
+    Real phi[32];                  // orbital row of the pending move
+    Real row[32];                  // inverse row of the moved electron
+    double gjcol[32];
+    int perm[32];
+};
+
+// Loop-invariant system scalars, hoisted once per kernel into registers.
+template <typename Real>
+struct RRConst {
+    double L, half_L, inv_L;
+    Real a_same, a_anti, b, alpha, rcut;
+    int use_cut, erfc_kind;
+    __device__ explicit RRConst(const DevSys &s)
+        : L(s.L), half_L(s.half_L), inv_L(s.inv_L), a_same((Real)s.a_same),
+          a_anti((Real)s.a_anti), b((Real)s.b), alpha((Real)s.alpha), rcut((Real)s.rcut),
+          use_cut(s.use_cut), erfc_kind(s.erfc_kind) {}
+    __device__ double mi(double d) const {
+        return d + L * (double)(d <= -half_L) - L * (double)(d > half_L);
+    }
+    // erfc(alpha r) / r with the r < 1e-12 guard and optional cutoff
+    // (energy_tracking.cpp:101-139).
+    __device__ Real ewald(Real r) const {
+        if (use_cut && !(r < rcut)) return (Real)0;
+        const Real inv_r = (r < (Real)1e-12) ? (Real)1 : wrcp<Real>(r);
+        return erfc_r<Real>(alpha * r, erfc_kind) * inv_r;
+    }
+};
+
+// In-LDS FP64 Gauss-Jordan inversion of the n x n matrix A (A = D^T on entry,
+// invT on exit) by the whole wave; returns log|det| or -inf below the
+// reference's pivot tolerance 1e-12 (matrix.hpp:35-39).
+__device__ inline double rr_gauss_jordan(double *A, double *col, int *perm, int n, int lane) {
+    double logdet = 0.0;
+    for (int c = 0; c < n; ++c) {
+        double best = -1.0;
+        int bi = n;
+        for (int r = c + lane; r < n; r += kWave) {
+            const double a = fabs(A[r * n + c]);
+            if (a > best) { best = a; bi = r; }
+        }
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off, kWave);
+            const int oi = __shfl_xor(bi, off, kWave);
+            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        best = wave_uniform(best);
+        bi = wave_uniform(bi);
+        if (!(best >= 1e-12)) return -INFINITY;
+        if (lane == 0) perm[c] = bi;
+        if (bi != c)
+            for (int j = lane; j < n; j += kWave) {
+                const double t = A[c * n + j];
+                A[c * n + j] = A[bi * n + j];
+                A[bi * n + j] = t;
+            }
+        __syncthreads();
+        const double piv = A[c * n + c];
+        logdet += log(fabs(piv));
+        const double pinv = 1.0 / piv;
+        for (int r = lane; r < n; r += kWave) col[r] = A[r * n + c];
+        __syncthreads();
+        for (int j = lane; j < n; j += kWave) A[c * n + j] = (j == c ? 1.0 : A[c * n + j]) * pinv;
+        __syncthreads();
+        for (int e = lane; e < n * n; e += kWave) {
+            const int r = e / n, j = e - (e / n) * n;
+            if (r == c) continue;
+            A[e] = (j == c ? 0.0 : A[e]) - col[r] * A[c * n + j];
+        }
+        __syncthreads();
+    }
+    for (int c = n - 1; c >= 0; --c) {
+        const int q = perm[c];
+        if (q != c)
+            for (int r = lane; r < n; r += kWave) {
+                const double t = A[r * n + c];
+                A[r * n + c] = A[r * n + q];
+                A[r * n + q] = t;
+            }
+        __syncthreads();
+    }
+    return logdet;
+}
+
+// Per-lane walker state of the register-resident kernel.
+template <typename Real, int NMAX>
+struct RRLane {
+    Real inv[NMAX];   // row r of invT of spin h
+    double x, y, z;   // electron position
+    double U;         // Jastrow row sum
+    int orb;          // orbital r: (nx+4) | (ny+4) << 4 | (nz+4) << 8 | type << 12
+};
+
+__device__ __forceinline__ int orb_nx(int o) { return (o & 15) - 4; }
+__device__ __forceinline__ int orb_ny(int o) { return ((o >> 4) & 15) - 4; }
+__device__ __forceinline__ int orb_nz(int o) { return ((o >> 8) & 15) - 4; }
+__device__ __forceinline__ int orb_t(int o) { return o >> 12; }
+
+__device__ __attribute__((noinline)) double rr_exp_f64(double x);
+
+template <typename Real, int NU, int ND, int GPL>
+struct RRKernel {
+    static constexpr int N = NU + ND;
+    static constexpr int NMAX = NU > ND ? NU : ND;
+    static constexpr auto orb = rr::shell_orbitals<NMAX>();
+    static constexpr int kMaxPow = rr::max_component<NMAX>();
+    using Sh = RRShared<Real, NU, ND>;
+    static_assert(NU <= 32 && ND <= 32, "register-resident path holds at most 32 rows per spin");
+    static_assert(ND == 0 || ND == NU, "register-resident path: polarised or equal spins");
+
+    __device__ static int lane_of(int e) { return e < NU ? e : 32 + (e - NU); }
+
+    // Squares of the axis phasors when some orbital needs |n_c| = 2.
+    __device__ static void squares(const rr::Cx<Real> a[3], rr::Cx<Real> a2[3]) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a2[c] = kMaxPow >= 2 ? rr::cmul(a[c], a[c]) : a[c];
+    }
+
+    // Jastrow row sum, gradient and Laplacian of the lane's electron and its
+    // half of the real-space Ewald sum (jastrow_pade.cpp:57-138,
+    // energy_tracking.cpp:93-144), reading the other positions from LDS.
+    __device__ static void pair_terms(const RRConst<Real> &c, const Sh &sh,
+                                      const RRLane<Real, NMAX> &L, int lane, bool has_e, int h,
+                                      double &U, double g[3], double &lap, double &vreal) {
+        U = 0.0;
+        g[0] = g[1] = g[2] = 0.0;
+        lap = 0.0;
+        vreal = 0.0;
+        const Real m2b = (Real)-2 * c.b;
+        for (int j = 0; j < N; ++j) {
+            const int lj = lane_of(j);
+            const double xj = sh.pos[0][lj], yj = sh.pos[1][lj], zj = sh.pos[2][lj];
+            if (!has_e || lj == lane) continue;
+            const Real dx = (Real)c.mi(L.x - xj);
+            const Real dy = (Real)c.mi(L.y - yj);
+            const Real dz = (Real)c.mi(L.z - zj);
+            const Real r = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+            const Real a = ((lj >> 5) == h) ? c.a_same : c.a_anti;
+            const Real den = wrcp<Real>((Real)1 + c.b * r);
+            U += (double)(a * r * den);
+            vreal += (double)c.ewald(r);
+            if (r < (Real)1e-12) continue;
+            const Real d1 = a * den * den;
+            const Real inv_r = wrcp<Real>(r);
+            const Real gf = d1 * inv_r;
+            g[0] += (double)(gf * dx);
+            g[1] += (double)(gf * dy);
+            g[2] += (double)(gf * dz);
+            lap += (double)(d1 * (m2b * den + (Real)2 * inv_r));
+        }
+        vreal *= 0.5;
+    }
+
+    // Jastrow row sum U_i of the lane's electron (jastrow_pade.cpp:7-55 rows).
+    __device__ static double jastrow_row(const RRConst<Real> &c, const Sh &sh,
+                                         const RRLane<Real, NMAX> &L, int lane, bool has_e, int h) {
+        double U = 0.0;
+        for (int j = 0; j < N; ++j) {
+            const int lj = lane_of(j);
+            const double xj = sh.pos[0][lj], yj = sh.pos[1][lj], zj = sh.pos[2][lj];
+            if (!has_e || lj == lane) continue;
+            const Real dx = (Real)c.mi(L.x - xj);
+            const Real dy = (Real)c.mi(L.y - yj);
+            const Real dz = (Real)c.mi(L.z - zj);
+            const Real r = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+            const Real a = ((lj >> 5) == h) ? c.a_same : c.a_anti;
+            U += (double)(a * r * wrcp<Real>((Real)1 + c.b * r));
+        }
+        return U;
+    }
+
+    // Local energy after a sweep; also refreshes U (drift-free) and returns
+    // kinetic / potential parts.
+    __device__ static Measured measure(const DevSys &s, const RRConst<Real> &cst, Sh &sh,
+                                       RRLane<Real, NMAX> &L, int lane, bool has_e, int h) {
+        double g[3], lap, vreal, U;
+        pair_terms(cst, sh, L, lane, has_e, h, U, g, lap, vreal);
+        if (has_e) L.U = U;
+        // Slater gradient and Laplacian of ln|D_h| at the lane's electron:
+        // sum_m invT[r][m] grad phi_m(r) with phi_m from the axis phasors
+        // (slater_plane_wave.cpp:425-499).
+        double sx = 0.0, sy = 0.0, sz = 0.0, sl = 0.0;
+        if (has_e) {
+            rr::Cx<Real> cur[3], a2[3];
+            cur[0] = rr::axis_phasor<Real>(L.x, s.inv_L);
+            cur[1] = rr::axis_phasor<Real>(L.y, s.inv_L);
+            cur[2] = rr::axis_phasor<Real>(L.z, s.inv_L);
+            squares(cur, a2);
+            const Real ku = (Real)s.kunit;
+            constexpr int nh = NU;  // ND is 0 or NU
+#pragma unroll
+            for (int m = 0; m < NMAX; ++m) {
+                if (m >= nh) break;
+                if (orb[m].t == 1) continue;  // handled with its cos partner below
+                __builtin_amdgcn_sched_barrier(0);
+                const rr::Cx<Real> ph = rr::orbital_phasor<Real>(orb[m].x, orb[m].y, orb[m].z, cur, a2);
+                const Real k2 = ku * ku * (Real)(orb[m].x * orb[m].x + orb[m].y * orb[m].y + orb[m].z * orb[m].z);
+                // cos orbital m: grad = -k sin, lap = -k^2 cos
+                const Real wc = L.inv[m];
+                Real gsc = -wc * ph.im;
+                Real lpc = -wc * ph.re;
+                if (m + 1 < nh && orb[m + 1].t == 1) {
+                    // sin orbital m+1 (same k): grad = k cos, lap = -k^2 sin
+                    const Real ws = L.inv[m + 1];
+                    gsc += ws * ph.re;
+                    lpc -= ws * ph.im;
+                }
+                sx += (double)(gsc * ku * (Real)orb[m].x);
+                sy += (double)(gsc * ku * (Real)orb[m].y);
+                sz += (double)(gsc * ku * (Real)orb[m].z);
+                sl += (double)(lpc * k2);
+            }
+        }
+        const double slap = sl - (sx * sx + sy * sy + sz * sz);
+        const double tx = sx + g[0], ty = sy + g[1], tz = sz + g[2];
+        const double kin_lane = has_e ? (slap + lap + tx * tx + ty * ty + tz * tz) : 0.0;
+        Measured m;
+        m.kinetic = -0.5 * rr::wave_total(kin_lane);
+        m.vreal = rr::wave_total(has_e ? vreal : 0.0);
+        m.vrecip = s.pref * recip(s, sh, L, lane, has_e);
+        m.e_local = m.kinetic + m.vreal + m.vrecip + s.e_const;
+        return m;
+    }
+
+    // sum_G w_G |S_G|^2 with S_G rebuilt from per-electron axis-power tables
+    // staged in LDS (energy_tracking.cpp:72-91,146-191).
+    __device__ static double recip(const DevSys &s, Sh &sh, const RRLane<Real, NMAX> &L, int lane,
+                                   bool has_e) {
+        constexpr int P = kRRMaxPow + 1;
+        int gk[GPL];  // packed (mx, my + 8, mz + 8), -1 when absent
+        Real sre[GPL], sim[GPL];
+#pragma unroll
+        for (int k = 0; k < GPL; ++k) {
+            const int g = lane + kWave * k;
+            const int4 G = g < s.ng ? s.gvec[g] : make_int4(0, 0, 0, 0);
+            gk[k] = g < s.ng ? (G.x | ((G.y + 8) << 4) | ((G.z + 8) << 8)) : -1;
+            sre[k] = (Real)0;
+            sim[k] = (Real)0;
+        }
+        const int lane_e = has_e ? ((lane < 32) ? lane : NU + (lane - 32)) : -1;
+        for (int c0 = 0; c0 < N; c0 += kRRChunk) {
+            const int cn = N - c0 < kRRChunk ? N - c0 : kRRChunk;
+            __syncthreads();
+            if (lane_e >= c0 && lane_e < c0 + cn) {
+                const int i = lane_e - c0;
+                const double u[3] = {L.x, L.y, L.z};
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    const rr::Cx<Real> a = rr::axis_phasor<Real>(u[ax], s.inv_L);
+                    rr::Cx<Real> p{(Real)1, (Real)0};
+                    Real *t = sh.tab + ((i * 3 + ax) * P) * 2;
+#pragma unroll
+                    for (int m = 0; m < P; ++m) {
+                        t[2 * m] = p.re;
+                        t[2 * m + 1] = p.im;
+                        p = rr::cmul(p, a);
+                    }
+                }
+            }
+            __syncthreads();
+            for (int i = 0; i < cn; ++i) {
+                const Real *t = sh.tab + (i * 3 * P) * 2;
+#pragma unroll
+                for (int k = 0; k < GPL; ++k) {
+                    if (gk[k] < 0) continue;
+                    const int mx = gk[k] & 15, my = ((gk[k] >> 4) & 15) - 8, mz = ((gk[k] >> 8) & 15) - 8;
+                    const int ay = my < 0 ? -my : my, az = mz < 0 ? -mz : mz;
+                    const rr::Cx<Real> X{t[2 * mx], t[2 * mx + 1]};
+                    rr::Cx<Real> Y{t[2 * (P + ay)], t[2 * (P + ay) + 1]};
+                    rr::Cx<Real> Z{t[2 * (2 * P + az)], t[2 * (2 * P + az) + 1]};
+                    if (my < 0) Y.im = -Y.im;
+                    if (mz < 0) Z.im = -Z.im;
+                    const rr::Cx<Real> e = rr::cmul(rr::cmul(X, Y), Z);
+                    sre[k] += e.re;
+                    sim[k] += e.im;
+                }
+            }
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < GPL; ++k) {
+            const int g = lane + kWave * k;
+            if (g < s.ng) acc += s.gw[g] * ((double)sre[k] * (double)sre[k] + (double)sim[k] * (double)sim[k]);
+        }
+        __syncthreads();
+        return rr::wave_total(acc);
+    }
+
+    // One move of electron (HP, rp).  Returns 1 if accepted.
+    template <int HP>
+    __device__ static int move(const RRConst<Real> &c, Sh &sh, RRLane<Real, NMAX> &L, int lane,
+                               bool has_e, int h, int r, int rp) {
+        constexpr int NP = HP ? ND : NU;
+        const int lp = HP * 32 + rp;
+        const double xn = sh.prop[0][lp], yn = sh.prop[1][lp], zn = sh.prop[2][lp];
+        rr::Cx<Real> a[3], a2[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            a[k].re = sh.pph[2 * k][lp];
+            a[k].im = sh.pph[2 * k + 1][lp];
+        }
+        squares(a, a2);
+        // Orbital row of the proposal: lane (HP, j) evaluates orbital j.
+        if (h == HP && r < NP) {
+            const rr::Cx<Real> ph = rr::orbital_phasor<Real>(orb_nx(L.orb), orb_ny(L.orb), orb_nz(L.orb), a, a2);
+            sh.phi[r] = orb_t(L.orb) ? ph.im : ph.re;
+        }
+        // Jastrow: new pair terms against every other electron.
+        const Real apair = (h == HP) ? c.a_same : c.a_anti;
+        Real un = (Real)0;
+        if (has_e && lane != lp) {
+            const Real dx = (Real)c.mi(xn - L.x);
+            const Real dy = (Real)c.mi(yn - L.y);
+            const Real dz = (Real)c.mi(zn - L.z);
+            const Real rn = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+            un = apair * rn * wrcp<Real>((Real)1 + c.b * rn);
+        }
+        __syncthreads();
+        // s_k = sum_m phi_m invT[k][m] for every row k of spin HP.
+        Real sk0 = (Real)0, sk1 = (Real)0;
+        if (h == HP && r < NP) {
+#pragma unroll
+            for (int m = 0; m < NP; m += 2) {
+                sk0 += sh.phi[m] * L.inv[m];
+                if (m + 1 < NP) sk1 += sh.phi[m + 1] * L.inv[m + 1];
+            }
+        }
+        const Real sk = sk0 + sk1;
+        const double R = (double)rr::readlane(sk, lp);
+        const double jn = rr::wave_total((double)un);
+        const double dj = jn - rr::readlane(L.U, lp);
+        const double ua = sh.prop[3][lp];
+        // Metropolis test u < R^2 exp(2 dJ) (simulation.cpp:125-131).  The
+        // hardware exp2 (relative error < 1e-6 here) decides unless u lies
+        // within 1e-5 of the threshold, where the exact FP64 exp decides, so
+        // the outcome equals the FP64 comparison.
+        const double r2 = R * R;
+        const float e32 = __builtin_amdgcn_exp2f((float)(2.0 * dj * 1.4426950408889634));
+        const double approx = r2 * (double)e32;
+        bool acc;
+        if (!(approx == approx)) {
+            acc = false;
+        } else if (fabs(ua - approx) > 1e-5 * approx || !(fabs(dj) < 40.0)) {
+            acc = ua < approx;
+        } else {
+            acc = ua < r2 * rr_exp_f64(2.0 * dj);
+        }
+        if (!acc) return 0;
+        // Accept (simulation.cpp:133-145).
+        if (lane == lp) {
+#pragma unroll
+            for (int m = 0; m < NP; ++m) sh.row[m] = L.inv[m];
+        }
+        const double xo = rr::readlane(L.x, lp), yo = rr::readlane(L.y, lp), zo = rr::readlane(L.z, lp);
+        __syncthreads();
+        if (h == HP && r < NP) {
+            const Real inv_ratio = (Real)(1.0 / R);
+            if (r == rp) {
+#pragma unroll
+                for (int m = 0; m < NP; ++m) L.inv[m] *= inv_ratio;
+            } else {
+                const Real f = sk * inv_ratio;
+#pragma unroll
+                for (int m = 0; m < NP; ++m) L.inv[m] -= sh.row[m] * f;
+            }
+        }
+        if (has_e && lane != lp) {
+            const Real dx = (Real)c.mi(xo - L.x);
+            const Real dy = (Real)c.mi(yo - L.y);
+            const Real dz = (Real)c.mi(zo - L.z);
+            const Real ro = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+            L.U += (double)un - (double)(apair * ro * wrcp<Real>((Real)1 + c.b * ro));
+        }
+        if (lane == lp) {
+            L.U = jn;
+            L.x = xn;
+            L.y = yn;
+            L.z = zn;
+            sh.pos[0][lane] = xn;
+            sh.pos[1][lane] = yn;
+            sh.pos[2][lane] = zn;
+        }
+        return 1;
+    }
+};
+
+// Philox draws (tags MOVE and ACCP, counter {electron, walker, sweep}),
+// Box-Muller or uniform-cube displacement, wrap into [0, L) and the axis
+// phasors of the proposal, written to the lane's LDS slots.  Out of line so
+// the transcendental constants do not occupy registers across the sweep.
+template <typename Real>
+__device__ __attribute__((noinline)) void rr_proposal(const DevSys &s, double sigma, double x0,
+                                                      double y0, double z0, int e, int w,
+                                                      uint32_t sweep, int lane, double *prop,
+                                                      Real *pph) {
+    const uint4 rm = philox4x32_10(make_uint4((uint32_t)e, (uint32_t)w, sweep, kTagMove), s.key0, s.key1);
+    const uint4 ra = philox4x32_10(make_uint4((uint32_t)e, (uint32_t)w, sweep, kTagAccept), s.key0, s.key1);
+    double dx, dy, dz;
+    if (s.move_kind == 1) {
+        const double rho0 = sqrt(-2.0 * log(u01(rm.x)));
+        const double rho1 = sqrt(-2.0 * log(u01(rm.z)));
+        double s0, c0, s1, c1;
+        sincospi(2.0 * u01(rm.y), &s0, &c0);
+        sincospi(2.0 * u01(rm.w), &s1, &c1);
+        dx = rho0 * c0;
+        dy = rho0 * s0;
+        dz = rho1 * c1;
+    } else {
+        dx = 2.0 * u01(rm.x) - 1.0;
+        dy = 2.0 * u01(rm.y) - 1.0;
+        dz = 2.0 * u01(rm.z) - 1.0;
+    }
+    const double x = x0 + sigma * dx, y = y0 + sigma * dy, z = z0 + sigma * dz;
+    const double q[3] = {x - s.L * floor(x * s.inv_L), y - s.L * floor(y * s.inv_L),
+                         z - s.L * floor(z * s.inv_L)};
+    for (int k = 0; k < 3; ++k) {
+        prop[k * kWave + lane] = q[k];
+        const rr::Cx<Real> ph = rr::axis_phasor<Real>(q[k], s.inv_L);
+        pph[(2 * k) * kWave + lane] = ph.re;
+        pph[(2 * k + 1) * kWave + lane] = ph.im;
+    }
+    prop[3 * kWave + lane] = u01(ra.x);
+}
+
+__device__ __attribute__((noinline)) double rr_exp_f64(double x) { return exp(x); }
+
+template <typename Real, int NU, int ND, int GPL>
+__device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &st, int n_sweeps,
+                                              unsigned long long sweep0, int flags) {
+    using K = RRKernel<Real, NU, ND, GPL>;
+    constexpr int NMAX = K::NMAX;
+    constexpr int N = NU + ND;
+    __shared__ RRShared<Real, NU, ND> sh;
+    const RRConst<Real> cst(s);
+    const int w = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int h = lane >> 5, r = lane & 31;
+    const int nh = h ? ND : NU;
+    const bool has_e = r < nh;
+    const int e = h ? NU + r : r;
+
+    RRLane<Real, NMAX> L;
+    const Real *inv_hbm = reinterpret_cast<const Real *>(st.inv) + (size_t)w * s.ninv;
+    const double *gp = st.pos + (size_t)w * 3 * N;
+#pragma unroll
+    for (int m = 0; m < NMAX; ++m)
+        L.inv[m] = (has_e && m < nh) ? inv_hbm[s.inv_off[h] + r * nh + m] : (Real)0;
+    L.x = has_e ? gp[e] : 0.0;
+    L.y = has_e ? gp[N + e] : 0.0;
+    L.z = has_e ? gp[2 * N + e] : 0.0;
+    sh.pos[0][lane] = L.x;
+    sh.pos[1][lane] = L.y;
+    sh.pos[2][lane] = L.z;
+    if (lane < kStatCount) sh.stat[lane] = st.stats[(size_t)w * kStatCount + lane];
+    if (lane == 0) {
+        sh.misc[0] = st.sigma[w];
+        sh.misc[1] = st.logdet[2 * w + 0];
+        sh.misc[2] = st.logdet[2 * w + 1];
+    }
+    L.orb = 0;
+#pragma unroll
+    for (int m = 0; m < NMAX; ++m)
+        if (r == m)
+            L.orb = (K::orb[m].x + 4) | ((K::orb[m].y + 4) << 4) | ((K::orb[m].z + 4) << 8) | (K::orb[m].t << 12);
+    __syncthreads();
+    L.U = K::jastrow_row(cst, sh, L, lane, has_e, h);
+    int since = st.since[w];
+    int wflags = st.flags[w];
+    int n_acc = 0;
+
+    for (int sw = 0; sw < n_sweeps; ++sw) {
+        const uint32_t sweep = (uint32_t)(sweep0 + (unsigned long long)sw);
+        // Proposal of the lane's electron for this sweep (electron order is
+        // fixed, so its position is current until its own move); staged in
+        // LDS for the move that reads it.
+        if (has_e) rr_proposal<Real>(s, sh.misc[0], L.x, L.y, L.z, e, w, sweep, lane, &sh.prop[0][0], &sh.pph[0][0]);
+        __syncthreads();
+        int acc = 0;
+#pragma unroll 1
+        for (int p = 0; p < N; ++p) {
+            const int accepted = (ND == 0 || p < NU) ? K::template move<0>(cst, sh, L, lane, has_e, h, r, p)
+                                                     : K::template move<(ND > 0 ? 1 : 0)>(cst, sh, L, lane, has_e, h, r, p - NU);
+            acc += accepted;
+        }
+        n_acc += acc;
+        since += acc;
+        if (flags & VMC_SWEEP_ADAPT) {
+            const double f = rr_exp_f64(0.25 * ((double)acc / (double)N - 0.5));
+            if (lane == 0) {
+                const double sigma = sh.misc[0] * f;
+                sh.misc[0] = sigma > 0.5 * s.L ? 0.5 * s.L : sigma;
+            }
+        }
+    }
+    Real *inv_out = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
+    if (has_e) {
+#pragma unroll
+        for (int m = 0; m < NMAX; ++m)
+            if (m < nh) inv_out[s.inv_off[h] + r * nh + m] = L.inv[m];
+        double *gq = st.pos + (size_t)w * 3 * N;
+        gq[e] = L.x;
+        gq[N + e] = L.y;
+        gq[2 * N + e] = L.z;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        sh.stat[6] += (double)n_acc;
+        sh.stat[7] += (double)n_sweeps * (double)N;
+
+        st.sigma[w] = sh.misc[0];
+        st.since[w] = since;
+        st.flags[w] = wflags;
+        st.logdet[2 * w + 0] = sh.misc[1];
+        st.logdet[2 * w + 1] = sh.misc[2];
+    }
+    __syncthreads();
+    if (lane < kStatCount) st.stats[(size_t)w * kStatCount + lane] = sh.stat[lane];
+}
+
+// The sweep kernel, without and with a 4-waves-per-SIMD occupancy bound (the
+// bound caps VGPRs at 128 at the price of some spills; the autotuner picks).
+template <typename Real, int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave) void k_sweep_rr(DevSys s, DevState st, int n_sweeps,
+                                                    unsigned long long sweep0, int flags) {
+    sweep_rr_body<Real, NU, ND, GPL>(s, st, n_sweeps, sweep0, flags);
+}
+
+template <typename Real, int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave, 4) void k_sweep_rr_o4(DevSys s, DevState st, int n_sweeps,
+                                                          unsigned long long sweep0, int flags) {
+    sweep_rr_body<Real, NU, ND, GPL>(s, st, n_sweeps, sweep0, flags);
+}
+
+typedef double rr_v4d __attribute__((ext_vector_type(4)));
+
+// C (32 x 32) = A * B for FP64 row-major LDS matrices of row stride kNSStride,
+// on the matrix cores: 2 x 2 output tiles of v_mfma_f64_16x16x4f64, 8 k-steps.
+// Operand layout of that instruction: lane l supplies A[l & 15][l >> 4] and
+// B[l >> 4][l & 15]; output register q of lane l is C[4q + (l >> 4)][l & 15].
+constexpr int kNSStride = 33;
+__device__ __forceinline__ void ns_gemm32(const double *A, const double *B, rr_v4d acc[4],
+                                          int lane) {
+    const int lo = lane & 15, hi = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = rr_v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+        const int k = 4 * kk + hi;
+        const double a0 = A[lo * kNSStride + k], a1 = A[(16 + lo) * kNSStride + k];
+        const double b0 = B[k * kNSStride + lo], b1 = B[k * kNSStride + 16 + lo];
+        acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[2], 0, 0, 0);
+        acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[3], 0, 0, 0);
+    }
+}
+
+// Row / column of accumulator q of tile t for this lane.
+__device__ __forceinline__ int ns_row(int t, int q, int lane) { return 16 * (t >> 1) + 4 * q + (lane >> 4); }
+__device__ __forceinline__ int ns_col(int t, int lane) { return 16 * (t & 1) + (lane & 15); }
+
+// Shared memory of the post-sweep kernel: the sweep layout plus two FP64
+// 32 x 33 matrices for the Newton-Schulz refinement (the axis-power table of
+// the measurement aliases the first one).
+template <typename Real, int NU, int ND>
+struct RRPostShared {
+    RRShared<Real, NU, ND> base;
+    double E[32 * kNSStride];
+    double Y[32 * kNSStride];
+};
+
+// Re-inversion of spin HP in FP64 on the matrix cores (Newton-Schulz):
+// with E = D^T (E[m][i] = phi_m(r_i), built from the positions in FP64) and
+// Y = invT, Z = E Y measures the drift |I - Z| (= |I - D^-1 D|), and
+// Y <- Y (2 I - Z) squares it; the FP32-drifted inverse of the sweep comes back
+// to FP64 accuracy in one step (two if the drift exceeds 1e-3).  A drift of
+// 0.5 or more (or a non-finite one) falls back to Gauss-Jordan from scratch.
+// Returns the drift found before refinement (negative if singular); a
+// Gauss-Jordan rebuild also refreshes log|det D| into `logdet`.
+template <typename Real, int NU, int ND, int GPL, int HP>
+__device__ double ns_reinvert(const DevSys &s, RRPostShared<Real, NU, ND> &ps,
+                              RRLane<Real, (NU > ND ? NU : ND)> &L, int lane, int h, int r,
+                              double &logdet) {
+    constexpr int NP = HP ? ND : NU;
+    constexpr int NMAX = NU > ND ? NU : ND;
+    constexpr auto orb = rr::shell_orbitals<NMAX>();
+    // E: lane (HP, i) writes column i (its electron's D row) in FP64.
+    for (int e = lane; e < 32 * kNSStride; e += kWave) {
+        ps.E[e] = 0.0;
+        ps.Y[e] = 0.0;
+    }
+    __syncthreads();
+    if (h == HP && r < NP) {
+        rr::Cx<double> a[3], a2[3];
+        a[0] = rr::axis_phasor<double>(L.x, s.inv_L);
+        a[1] = rr::axis_phasor<double>(L.y, s.inv_L);
+        a[2] = rr::axis_phasor<double>(L.z, s.inv_L);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a2[c] = rr::cmul(a[c], a[c]);
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            const rr::Cx<double> ph = rr::orbital_phasor<double>(orb[m].x, orb[m].y, orb[m].z, a, a2);
+            ps.E[m * kNSStride + r] = orb[m].t ? ph.im : ph.re;
+        }
+#pragma unroll
+        for (int m = 0; m < NP; ++m) ps.Y[r * kNSStride + m] = (double)L.inv[m];
+    }
+    __syncthreads();
+    double drift0 = -1.0;
+    for (int it = 0; it < 2; ++it) {
+        rr_v4d z[4];
+        ns_gemm32(ps.E, ps.Y, z, lane);
+        double dmax = 0.0;
+        bool finite = true;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = ns_row(t, q, lane), j = ns_col(t, lane);
+                const double d = (i == j ? 1.0 : 0.0) - z[t][q];
+                if (i < NP && j < NP) {
+                    dmax = fabs(d) > dmax ? fabs(d) : dmax;
+                    finite = finite && (d == d);
+                }
+            }
+        // wave max of the drift
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, off, kWave));
+        dmax = wave_uniform(dmax);
+        const int all_finite = __syncthreads_and(finite ? 1 : 0);
+        if (it == 0) drift0 = all_finite ? dmax : INFINITY;
+        if (!all_finite || !(dmax < 0.5)) {
+            drift0 = INFINITY;
+            break;
+        }
+        if (it == 1 && dmax < 1e-13) break;
+        __syncthreads();
+        // W = 2 I - Z into E's storage, then Y <- Y W.
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = ns_row(t, q, lane), j = ns_col(t, lane);
+                ps.E[i * kNSStride + j] = (i == j ? 2.0 : 0.0) - z[t][q];
+            }
+        __syncthreads();
+        rr_v4d y[4];
+        ns_gemm32(ps.Y, ps.E, y, lane);
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) ps.Y[ns_row(t, q, lane) * kNSStride + ns_col(t, lane)] = y[t][q];
+        __syncthreads();
+        if (dmax < 1e-3) break;
+        // second refinement: rebuild E (overwritten by W)
+        if (h == HP && r < NP) {
+            rr::Cx<double> a[3], a2[3];
+            a[0] = rr::axis_phasor<double>(L.x, s.inv_L);
+            a[1] = rr::axis_phasor<double>(L.y, s.inv_L);
+            a[2] = rr::axis_phasor<double>(L.z, s.inv_L);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) a2[c] = rr::cmul(a[c], a[c]);
+#pragma unroll
+            for (int m = 0; m < NP; ++m) {
+                const rr::Cx<double> ph = rr::orbital_phasor<double>(orb[m].x, orb[m].y, orb[m].z, a, a2);
+                ps.E[m * kNSStride + r] = orb[m].t ? ph.im : ph.re;
+            }
+        }
+        for (int e = lane; e < 32 * kNSStride; e += kWave) {
+            const int i = e / kNSStride, j = e - i * kNSStride;
+            if (i >= NP || j >= NP) ps.E[e] = 0.0;
+        }
+        __syncthreads();
+    }
+    if (!(drift0 < 0.5)) {
+        // Gauss-Jordan from scratch on A = D^T (row stride NP) in E's storage.
+        __syncthreads();
+        if (h == HP && r < NP) {
+            rr::Cx<double> a[3], a2[3];
+            a[0] = rr::axis_phasor<double>(L.x, s.inv_L);
+            a[1] = rr::axis_phasor<double>(L.y, s.inv_L);
+            a[2] = rr::axis_phasor<double>(L.z, s.inv_L);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) a2[c] = rr::cmul(a[c], a[c]);
+#pragma unroll
+            for (int m = 0; m < NP; ++m) {
+                const rr::Cx<double> ph = rr::orbital_phasor<double>(orb[m].x, orb[m].y, orb[m].z, a, a2);
+                ps.E[m * NP + r] = orb[m].t ? ph.im : ph.re;
+            }
+        }
+        __syncthreads();
+        const double ld = rr_gauss_jordan(ps.E, ps.base.gjcol, ps.base.perm, NP, lane);
+        if (!(ld > -INFINITY)) return -1.0;
+        logdet = ld;
+        if (h == HP && r < NP) {
+#pragma unroll
+            for (int m = 0; m < NP; ++m) L.inv[m] = (Real)ps.E[r * NP + m];
+        }
+        __syncthreads();
+        return drift0;
+    }
+    if (h == HP && r < NP) {
+#pragma unroll
+        for (int m = 0; m < NP; ++m) L.inv[m] = (Real)ps.Y[r * kNSStride + m];
+    }
+    __syncthreads();
+    return drift0;
+}
+
+// After a sweep: the periodic FP64 re-inversion of walkers whose accepted
+// moves since the last one reached K (flag VMC_POST_REINVERT), then the local
+// energy (flag VMC_POST_MEASURE) with the same lane layout as the sweep,
+// feeding the FP64 accumulators and blocking (blocking_analysis.cpp:17-39).
+// Stat VMC_STAT_REINV counts re-inversions; the largest drift |I - D^-1 D|
+// met before a re-inversion is kept per walker in st.drift.
+constexpr int kPostMeasure = 1;
+constexpr int kPostReinvert = 2;
+
+template <typename Real, int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave) void k_post_rr(DevSys s, DevState st, int block_size, int flags) {
+    using K = RRKernel<Real, NU, ND, GPL>;
+    constexpr int NMAX = K::NMAX;
+    constexpr int N = NU + ND;
+    __shared__ RRPostShared<Real, NU, ND> ps;
+    RRShared<Real, NU, ND> &sh = ps.base;
+    const RRConst<Real> cst(s);
+    const int w = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int h = lane >> 5, r = lane & 31;
+    const int nh = h ? ND : NU;
+    const bool has_e = r < nh;
+    const int e = h ? NU + r : r;
+    RRLane<Real, NMAX> L;
+    Real *inv_hbm = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
+    const double *gp = st.pos + (size_t)w * 3 * N;
+#pragma unroll
+    for (int m = 0; m < NMAX; ++m)
+        L.inv[m] = (has_e && m < nh) ? inv_hbm[s.inv_off[h] + r * nh + m] : (Real)0;
+    L.x = has_e ? gp[e] : 0.0;
+    L.y = has_e ? gp[N + e] : 0.0;
+    L.z = has_e ? gp[2 * N + e] : 0.0;
+    sh.pos[0][lane] = L.x;
+    sh.pos[1][lane] = L.y;
+    sh.pos[2][lane] = L.z;
+    __syncthreads();
+    if ((flags & kPostReinvert) && s.reinv_k > 0 && st.since[w] >= s.reinv_k) {
+        double ld0 = st.logdet[2 * w + 0], ld1 = st.logdet[2 * w + 1];
+        double d = ns_reinvert<Real, NU, ND, GPL, 0>(s, ps, L, lane, h, r, ld0);
+        if constexpr (ND > 0) {
+            const double d1 = ns_reinvert<Real, NU, ND, GPL, 1>(s, ps, L, lane, h, r, ld1);
+            d = (d < 0.0 || d1 < 0.0) ? -1.0 : fmax(d, d1);
+        }
+        if (has_e) {
+#pragma unroll
+            for (int m = 0; m < NMAX; ++m)
+                if (m < nh) inv_hbm[s.inv_off[h] + r * nh + m] = L.inv[m];
+        }
+        if (lane == 0) {
+            st.since[w] = 0;
+            st.logdet[2 * w + 0] = ld0;
+            st.logdet[2 * w + 1] = ld1;
+            if (d < 0.0) st.flags[w] |= 1;
+            double *t = st.stats + (size_t)w * kStatCount;
+            t[15] += 1.0;
+            if (d > st.drift[w]) st.drift[w] = d;
+        }
+        __syncthreads();
+    }
+    if (!(flags & kPostMeasure)) return;
+    const Measured m = K::measure(s, cst, sh, L, lane, has_e, h);
+    if (lane == 0) {
+        double *t = st.stats + (size_t)w * kStatCount;
+        const double en = m.e_local;
+        const double pot = m.vreal + m.vrecip + s.e_const;
+        t[0] += 1.0;
+        t[1] += en;
+        t[2] += en * en;
+        t[3] += m.kinetic;
+        t[4] += pot;
+        t[5] = en;
+        t[13] = m.kinetic;
+        t[14] = pot;
+        t[8] += en;
+        t[9] += 1.0;
+        if (t[9] >= (double)block_size) {
+            const double bm = t[8] / (double)block_size;
+            t[10] += 1.0;
+            const double delta = bm - t[11];
+            t[11] += delta / t[10];
+            t[12] += delta * (bm - t[11]);
+            t[8] = 0.0;
+            t[9] = 0.0;
+        }
+    }
+}
+
+}  // namespace vmc

@@ -27,6 +27,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 MOVE_UNIFORM, MOVE_GAUSSIAN = 0, 1
 ERFC_AS7126, ERFC_EXACT = 0, 1
 SWEEP_MEASURE, SWEEP_ADAPT = 1, 2
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_REGISTER = 0, 1, 2
 
 STAT = dict(NMEAS=0, SUM_E=1, SUM_E2=2, SUM_KIN=3, SUM_POT=4, LAST_E=5, ACCEPTED=6,
             PROPOSED=7, BLK_SUM=8, BLK_IN=9, BLK_N=10, BLK_MEAN=11, BLK_M2=12, LAST_KIN=13,
@@ -150,8 +151,12 @@ def lib():
             "vmc_ensemble_get_positions": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_sweep": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _VP]),
             "vmc_ensemble_reset_stats": (C.c_int, [_VP, _VP]),
+            "vmc_ensemble_set_kernel": (C.c_int, [_VP, C.c_int32]),
+            "vmc_ensemble_get_kernel": (C.c_int, [_VP, _ip]),
+            "vmc_ensemble_set_rr_variant": (C.c_int, [_VP, C.c_int32]),
             "vmc_ensemble_read_stats": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_read_steps": (C.c_int, [_VP, _dp, _VP]),
+            "vmc_ensemble_read_drift": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_info": (C.c_int, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
             "vmc_frozen_eval": (C.c_int, [sp, C.c_int32, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _dp,
                                           _VP]),
@@ -226,6 +231,17 @@ class Ensemble:
         flags = (SWEEP_MEASURE if measure else 0) | (SWEEP_ADAPT if adapt else 0)
         _check(lib().vmc_ensemble_sweep(self._h, n_sweeps, flags, block_size, self.stream))
 
+    def set_kernel(self, kernel):
+        _check(lib().vmc_ensemble_set_kernel(self._h, kernel))
+
+    def kernel(self):
+        k = C.c_int32()
+        _check(lib().vmc_ensemble_get_kernel(self._h, C.byref(k)))
+        return k.value
+
+    def set_rr_variant(self, variant):
+        _check(lib().vmc_ensemble_set_rr_variant(self._h, variant))
+
     def reset_stats(self):
         _check(lib().vmc_ensemble_reset_stats(self._h, self.stream))
 
@@ -237,6 +253,13 @@ class Ensemble:
     def steps(self):
         s = np.zeros(self.W)
         _check(lib().vmc_ensemble_read_steps(self._h, _ptr(s), self.stream))
+        return s
+
+    def drift(self):
+        """Largest |I - D^-1 D| (max norm) met per walker when its
+        mixed-precision inverse came due for the FP64 re-inversion."""
+        s = np.zeros(self.W)
+        _check(lib().vmc_ensemble_read_drift(self._h, _ptr(s), self.stream))
         return s
 
     def sweeps_done(self):
