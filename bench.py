@@ -1,20 +1,20 @@
 #!/usr/bin/env python3
 """Headline benchmark: electron-moves/s of the VMC sweep at N = 54
 (27 up + 27 down), r_s = 2, b = 1, 4096 walkers on one MI355X, mixed
-precision against the plain FP64 path, with the CPU baseline timed on the
-host's cores in the same run.
+precision against the FP64 path, with the CPU baseline timed on the host's
+cores in the same run.
 
-A step is one sweep of every walker (each electron proposed once) followed by
-the local-energy measurement; walker state stays resident in HBM/LDS and the
-inputs are on the device before the timed region starts.  Prints one JSON
-line.
+A step is one sweep of every walker (each electron proposed once, in order)
+followed by the FP64 re-inversion of the walkers due for it (mixed precision)
+and the local energy of every walker.  Walker state is
+resident in HBM before the timed region starts.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
 import json
-import math
 import os
+import re
 import sys
 import time
 
@@ -23,69 +23,129 @@ sys.path.insert(0, os.path.join(ROOT, "variational-monte-carlo_amd"))
 
 N_ELECTRONS, RS, JB, WALKERS = 54, 2.0, 1.0, 4096
 SEED = 20251202
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_measure_rr_N54_W4096_traffic.md")
 
-# MI355X published peaks.
+# MI355X published dense peaks.
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 
 
-def flops_per_move(n, nk, N, ng, p):
-    """Algorithmic flops of one proposed move (add/mul = 1, FMA = 2,
-    transcendentals not counted): orbital phases 6 n_k, ratio 2 n,
-    Jastrow pairs 40 (N-1), on accept p [SM 4 n^2 + rho_k 26 N_G + real-space
-    60 (N-1)], per-sweep local energy [16 n^2 + 30 N (N-1)] / N."""
-    return (6 * nk + 2 * n + 40 * (N - 1) + p * (4 * n * n + 26 * ng + 60 * (N - 1)) +
-            (16 * n * n + 30 * N * (N - 1)) / N)
+def measure_flops_per_walker(n, N, ng):
+    """Algorithmic flops of the local-energy kernel for one walker (add/mul =
+    1, FMA = 2; sqrt, reciprocal, exp and erfc counted as 1 each):
+      ordered pairs N(N-1) x 40   (min image 3, r^2 5, r 1, 1/(1+br) 2, u 3,
+                                   u', u'' 6, gradient 6, Laplacian 5,
+                                   erfc(ar)/r 9)
+      Slater terms N n x 14        (orbital phase 6, gradient 6, Laplacian 2)
+      structure factor N N_G x 14 + 4 N_G   (e^{iG.r} from axis powers 12,
+                                              accumulate 2; w |S|^2 4)"""
+    return 40 * N * (N - 1) + 14 * N * n + 14 * N * ng + 4 * ng
 
 
-def gpu_run(V, torch, precision, steps, warmup, sweeps_per_launch, reinvert):
+def reinvert_flops_per_walker(n, ndet=2):
+    """Algorithmic flops of one FP64 Newton-Schulz step per walker: E Y and
+    Y (2I - E Y), 2 n^3 each, per determinant."""
+    return ndet * 4 * n ** 3
+
+
+def sweep_flops_per_move(n, N, p):
+    """Algorithmic flops of one proposed move in the sweep kernel: orbital
+    row 6 n, ratio 2 n, new Jastrow pair terms 15 (N-1); on accept (rate p)
+    Sherman-Morrison 4 n^2 and the Jastrow row-sum update 15 (N-1)."""
+    return 6 * n + 2 * n + 15 * (N - 1) + p * (4 * n * n + 15 * (N - 1))
+
+
+def read_traffic():
+    """HBM bytes per local-energy kernel launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            text = f.read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def pct(xs, q):
+    s = sorted(xs)
+    return s[min(len(s) - 1, int(q * len(s)))]
+
+
+def gpu_run(V, torch, precision, steps, warmup, reinvert, kernel):
     s = V.System.heg(N_ELECTRONS, RS, JB, precision=precision, reinvert_interval=reinvert)
     (nk0, _), ng, _ = V.system_sizes(s)
     stream = torch.cuda.Stream()
     ens = V.Ensemble(s, WALKERS, SEED, step0=0.3 * RS, stream=stream.cuda_stream)
+    ens.set_kernel(kernel)
     ens.init_positions()
     # Equilibration with step adaptation (untimed), then the measured sweeps.
     ens.sweep(max(warmup, 1), measure=False, adapt=True)
     ens.reset_stats()
-    launches = math.ceil(steps / sweeps_per_launch)
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(launches)]
+    # Timed region 1: the throughput, with no events between launches.
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    done = 0
-    for i in range(launches):
-        k = min(sweeps_per_launch, steps - done)
-        starts[i].record(stream)
-        ens.sweep(k, measure=True, adapt=False, block_size=10)
-        ends[i].record(stream)
-        done += k
+    for i in range(steps):
+        ens.sweep(1, measure=True, adapt=False, block_size=10)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    launch_ms = [a.elapsed_time(b) for a, b in zip(starts, ends)]
+    # Timed region 2: the same steps again with HIP events on the launch
+    # stream around every step and every kernel (latency and kernel times).
+    ens.timing()  # drop anything recorded so far
+    ens.set_timing(True)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        ens.sweep(1, measure=True, adapt=False, block_size=10)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall_ev = time.perf_counter() - t1
+    ens.set_timing(False)
+    kt = ens.timing()
+    step_ms = [a.elapsed_time(b) for a, b in zip(starts, ends)]
     st = ens.stats()
+    drift = ens.drift()
     acc = float(st[:, V.STAT["ACCEPTED"]].sum() / st[:, V.STAT["PROPOSED"]].sum())
+    reinv = float(st[:, V.STAT["REINV"]].sum()) / 2.0  # both timed regions
     mean, se = V.blocking_summary(st)
-    moves = WALKERS * N_ELECTRONS * steps
-    n = N_ELECTRONS // 2
-    f_move = flops_per_move(n, nk0, N_ELECTRONS, ng, acc)
-    # Per-launch algorithmic flops over the event-timed launch durations.
-    per_launch = [(min(sweeps_per_launch, steps - i * sweeps_per_launch)) for i in range(launches)]
-    tflops = [f_move * WALKERS * N_ELECTRONS * k / (ms * 1e-3) / 1e12
-              for k, ms in zip(per_launch, launch_ms)]
+    used = ens.kernel()
     ens.close()
-    srt = sorted(launch_ms)
-    return dict(moves_per_s=moves / wall, wall_s=wall, acceptance=acc, e_per_electron=mean /
-                N_ELECTRONS, se_per_electron=None if se is None else se / N_ELECTRONS,
-                launch_ms_avg=sum(launch_ms) / len(launch_ms),
-                launch_ms_p50=srt[len(srt) // 2], launch_ms_p99=srt[min(len(srt) - 1,
-                                                                       int(0.99 * len(srt)))],
-                tflops=sum(tflops) / len(tflops), flops_per_move=f_move, sweeps=steps,
-                sweeps_per_launch=sweeps_per_launch)
+    n = N_ELECTRONS // 2
+    moves = WALKERS * N_ELECTRONS * steps
+    out = dict(moves_per_s=moves / wall, wall_s=wall, moves_per_s_with_events=moves / wall_ev,
+               acceptance=acc,
+               e_per_electron=mean / N_ELECTRONS,
+               se_per_electron=None if se is None else se / N_ELECTRONS,
+               step_ms_p50=pct(step_ms, 0.5), step_ms_p99=pct(step_ms, 0.99),
+               step_ms_avg=sum(step_ms) / len(step_ms), kernel=used,
+               reinversions_per_walker_step=reinv / (WALKERS * steps),
+               drift_max=float(drift.max()))
+    if kt["measure"][1]:
+        ms = kt["measure"][0] / kt["measure"][1]
+        f = measure_flops_per_walker(n, N_ELECTRONS, ng)
+        out.update(measure_ms_avg=ms, measure_launches=kt["measure"][1],
+                   measure_tflops=f * WALKERS / (ms * 1e-3) / 1e12, measure_flops_per_walker=f)
+    if kt["reinvert"][1]:
+        ms = kt["reinvert"][0] / kt["reinvert"][1]
+        f = reinvert_flops_per_walker(n) * out["reinversions_per_walker_step"]
+        out.update(reinvert_ms_avg=ms, reinvert_launches=kt["reinvert"][1],
+                   reinvert_tflops=f * WALKERS / (ms * 1e-3) / 1e12)
+    if kt["sweep"][1]:
+        sweep_ms = kt["sweep"][0] / kt["sweep"][1]
+        f = sweep_flops_per_move(n, N_ELECTRONS, acc)
+        out.update(sweep_ms_avg=sweep_ms, sweep_launches=kt["sweep"][1],
+                   sweep_tflops=f * WALKERS * N_ELECTRONS / (sweep_ms * 1e-3) / 1e12,
+                   sweep_flops_per_move=f)
+    return out
 
 
 def cpu_baseline(target_s):
-    """The oracle's sweep twin, one chain per host core, sized to about
-    target_s seconds of wall time."""
+    """The oracle's FP64 sweep twin (same system, proposals and per-sweep E_L
+    as the GPU path), one chain per host core, sized to about target_s
+    seconds of wall time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -95,7 +155,7 @@ def cpu_baseline(target_s):
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
-    rate1, el = O.bench_threads(s, SEED, 0.6, 1, 20)
+    rate1, _ = O.bench_threads(s, SEED, 0.6, 1, 20)
     sweeps = max(20, int(target_s * rate1 / N_ELECTRONS))
     rate, wall = O.bench_threads(s, SEED, 0.6, cores, sweeps)
     return dict(value=rate, unit="electron-moves/s", cores=cores, kind="port",
@@ -104,36 +164,71 @@ def cpu_baseline(target_s):
                 single_core=rate1)
 
 
+def reference_cpu(target_s):
+    """The reference's own Simulation::run, compiled from its sources into
+    oracle/_ref, on one core: its single determinant at N=27 (the closest
+    closed shell to one spin of the workload), r_s=2, with the reference's
+    Ewald rules and E_L after every move as the reference does.  None when
+    that library was not built."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    if not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libvmc_ref.so")):
+        return None
+    n = 27
+    L = O.box_length_from_rs(RS, n)
+    t0 = time.perf_counter()
+    O.ref_simulation_run(n, L, 10, 200, 0.6, 11, 10)
+    dt = time.perf_counter() - t0
+    steps = max(200, int(target_s / dt * 210))
+    t0 = time.perf_counter()
+    O.ref_simulation_run(n, L, 10, steps, 0.6, 11, 10)
+    dt = time.perf_counter() - t0
+    return dict(value=(steps + 10) / dt, unit="electron-moves/s", cores=1,
+                sample=f"reference Simulation::run, N=27 polarised, r_s=2, {steps} measured moves, "
+                       f"{dt:.1f} s")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--sweeps-per-launch", type=int, default=10)
     ap.add_argument("--reinvert", type=int, default=64,
                     help="accepted moves between FP64 re-inversions in the mixed path")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--mixed-only", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain device copy of 1 GiB (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU: walkers are independent chains and one MI355X runs the "
-              f"whole 4096-walker ensemble; --gpus {a.gpus} is not supported.", file=sys.stderr)
+        print(f"bench.py measures one GPU: the walkers are independent chains and one MI355X holds "
+              f"the whole 4096-walker ensemble; --gpus {a.gpus} is not supported.", file=sys.stderr)
         sys.exit(2)
     import torch
     import vmc_amd as V
 
     V.device_check()
+    if a.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     res = {}
     if not a.fp64_only:
-        res["mixed"] = gpu_run(V, torch, V.PREC_MIXED, a.steps, a.warmup, a.sweeps_per_launch,
-                               a.reinvert)
+        res["mixed"] = gpu_run(V, torch, V.PREC_MIXED, a.steps, a.warmup, a.reinvert,
+                               V.KERNEL_AUTO)
     if not a.mixed_only:
-        res["fp64"] = gpu_run(V, torch, V.PREC_FP64, a.steps, a.warmup, a.sweeps_per_launch, 0)
-    head = res.get("mixed", res.get("fp64"))
+        res["fp64"] = gpu_run(V, torch, V.PREC_FP64, a.steps, a.warmup, 0, V.KERNEL_AUTO)
+        res["fp64_generic"] = gpu_run(V, torch, V.PREC_FP64, a.steps, a.warmup, 0,
+                                      V.KERNEL_GENERIC)
     head_name = "mixed" if "mixed" in res else "fp64"
+    head = res[head_name]
     peak = PEAK_FP32_TFLOPS if head_name == "mixed" else PEAK_FP64_TFLOPS
+    traffic = read_traffic() if head_name == "mixed" else None
     line = {
         "metric": "electron-moves/s at N=54, 4096 walkers, FP32 vs FP64; p50/p99 solve latency",
         "value": head["moves_per_s"],
@@ -145,33 +240,59 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (f64 accumulators and re-inversion)" if head_name == "mixed" else "f64",
-        "data": "synthetic: Philox uniform starts, 50 adaptive equilibration sweeps",
+        "dtype": "f32" if head_name == "mixed" else "f64",
+        "data": "synthetic: Philox uniform starts per walker, adaptive equilibration sweeps",
         "config": {"workload": "HEG VMC sweep N=54 (27+27), r_s=2, b=1, 4096 walkers, "
-                               "alpha=5/L, |n|^2<=25, rcut=L/2, Gaussian moves, E_L per sweep",
-                   "precision": head_name, "sweeps_per_launch": a.sweeps_per_launch,
-                   "reinvert_interval": a.reinvert if head_name == "mixed" else 0},
-        "latency_ms": {"p50": head["launch_ms_p50"], "p99": head["launch_ms_p99"],
-                       "what": f"one launch of {a.sweeps_per_launch} sweeps of all 4096 walkers"},
-        "roofline": {"bound": "mfma", "engine": "valu (no MFMA in the sweep)",
-                     "achieved": head["tflops"], "peak": peak, "unit": "TFLOP/s",
-                     "frac": head["tflops"] / peak, "traffic": None,
-                     "flops_per_move": head["flops_per_move"]},
+                               "alpha=5/L, |n|^2<=25 (257 G), rcut=L/2, Gaussian moves, "
+                               "E_L every sweep",
+                   "precision": "mixed: FP32 working set, FP64 accumulators and re-inversion"
+                   if head_name == "mixed" else "fp64",
+                   "reinvert_interval": a.reinvert if head_name == "mixed" else 0,
+                   "kernel": "register" if head["kernel"] == V.KERNEL_REGISTER else "generic"},
+        "value_with_kernel_events": head["moves_per_s_with_events"],
+        "latency_ms": {"p50": head["step_ms_p50"], "p99": head["step_ms_p99"],
+                       "what": "one step (sweep, re-inversion, local energy) of all 4096 walkers, "
+                               "HIP events on the launch stream"},
     }
+    if "measure_ms_avg" in head:
+        line["roofline"] = {
+            "kernel": "k_measure_rr (local energy of every walker)",
+            "bound": "mfma", "engine": "VALU-issue bound; peak is the dense figure of the "
+                                       "working precision",
+            "achieved": head["measure_tflops"], "peak": peak, "unit": "TFLOP/s",
+            "frac": head["measure_tflops"] / peak, "traffic": traffic,
+            "launch_ms_avg": head["measure_ms_avg"],
+            "flops_per_walker": head["measure_flops_per_walker"],
+            "sweep_kernel": {"launch_ms_avg": head["sweep_ms_avg"],
+                             "achieved": head["sweep_tflops"], "frac": head["sweep_tflops"] / peak,
+                             "flops_per_move": head["sweep_flops_per_move"]}}
+        if "reinvert_ms_avg" in head:
+            line["roofline"]["reinvert_kernel"] = {
+                "launch_ms_avg": head["reinvert_ms_avg"], "achieved": head["reinvert_tflops"],
+                "peak": PEAK_FP64_TFLOPS, "frac": head["reinvert_tflops"] / PEAK_FP64_TFLOPS}
+    for k in ("fp64", "fp64_generic"):
+        if k in res and head_name != k:
+            r = res[k]
+            line[k] = {"value": r["moves_per_s"], "step_ms_p50": r["step_ms_p50"],
+                       "step_ms_p99": r["step_ms_p99"],
+                       "kernel": "register" if r["kernel"] == V.KERNEL_REGISTER else "generic"}
+            if "measure_ms_avg" in r:
+                line[k].update(measure_ms_avg=r["measure_ms_avg"], sweep_ms_avg=r["sweep_ms_avg"],
+                               measure_frac_fp64_peak=r["measure_tflops"] / PEAK_FP64_TFLOPS)
     if "mixed" in res and "fp64" in res:
-        line["fp64"] = {"value": res["fp64"]["moves_per_s"],
-                        "tflops": res["fp64"]["tflops"],
-                        "frac_fp64_peak": res["fp64"]["tflops"] / PEAK_FP64_TFLOPS,
-                        "launch_ms_p50": res["fp64"]["launch_ms_p50"],
-                        "launch_ms_p99": res["fp64"]["launch_ms_p99"],
-                        "e_per_electron": res["fp64"]["e_per_electron"],
-                        "se_per_electron": res["fp64"]["se_per_electron"]}
         line["mixed_over_fp64"] = res["mixed"]["moves_per_s"] / res["fp64"]["moves_per_s"]
+        line["mixed_over_fp64_generic"] = (res["mixed"]["moves_per_s"] /
+                                           res["fp64_generic"]["moves_per_s"])
     line["physics"] = {k: {"e_per_electron": v["e_per_electron"],
                            "se_per_electron": v["se_per_electron"],
-                           "acceptance": v["acceptance"]} for k, v in res.items()}
+                           "acceptance": v["acceptance"],
+                           "reinversions_per_walker_step": v["reinversions_per_walker_step"],
+                           "drift_max": v["drift_max"]} for k, v in res.items()}
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+        ref = reference_cpu(min(a.cpu_seconds, 5.0))
+        if ref is not None:
+            line["cpu_reference"] = ref
     print(json.dumps(line))
 
 

@@ -181,6 +181,16 @@ int vmc_ensemble_read_stats(vmc_ensemble *e, double *stats, void *stream);
  * FP64 re-inversion (register kernel; 0 until one happens).  No reference
  * counterpart: the reference works in FP64 throughout and never re-inverts. */
 int vmc_ensemble_read_drift(vmc_ensemble *e, double *drift, void *stream);
+/* Per-kernel device time: while enabled, every sweep launch records HIP
+ * events on its own stream around each kernel.  read_timing waits for them
+ * and returns (then forgets) the summed milliseconds and launch counts per
+ * kernel kind, arrays of VMC_TIMING_KINDS: [0] sweep kernel (the generic
+ * kernel's fused sweep, re-inversion and measurement included), [1] FP64
+ * re-inversion kernel, [2] local-energy kernel.  No reference counterpart
+ * (instrumentation). */
+#define VMC_TIMING_KINDS 3
+int vmc_ensemble_set_timing(vmc_ensemble *e, int32_t enable);
+int vmc_ensemble_read_timing(vmc_ensemble *e, double *ms, int64_t *launches);
 /* Copy per-walker step sizes to host: [W]. */
 int vmc_ensemble_read_steps(vmc_ensemble *e, double *steps, void *stream);
 /* Global sweep counter (the Philox stream position) and walker count. */

@@ -199,3 +199,20 @@ def test_mixed_reinversion_bounds_the_drift(vmc, oracle):
         e, kin, _ = wk.local_energy()
         scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
         assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale, (w, st[w, vmc.STAT["LAST_E"]], e)
+
+
+def test_kernel_timing_counts_every_launch(vmc):
+    """The per-kernel event timer sees one sweep, one re-inversion and one
+    local-energy launch per measured sweep of the register path, and
+    forgets them once read."""
+    s = vmc.System.heg(54, 2.0, 1.0, precision=1, reinvert_interval=64)
+    ens = vmc.Ensemble(s, 32, 9, step0=0.6)
+    ens.set_kernel(2)
+    ens.init_positions()
+    ens.set_timing(True)
+    ens.sweep(3, measure=True)
+    ens.sweep(2, measure=False)
+    t = ens.timing()
+    assert t["sweep"][1] == 5 and t["reinvert"][1] == 5 and t["measure"][1] == 3
+    assert all(v[0] > 0 for v in t.values())
+    assert all(v[1] == 0 for v in ens.timing().values())

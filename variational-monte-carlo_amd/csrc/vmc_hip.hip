@@ -484,6 +484,7 @@ struct vmc_ensemble {
     bool positions_ready = false;
     int kernel = VMC_KERNEL_AUTO;
     int rr_variant = 0;
+    KernelTimer timer;
 
     ~vmc_ensemble() {
         if (st.pos) (void)hipFree(st.pos);
@@ -519,14 +520,16 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
         const unsigned grid = static_cast<unsigned>(e->W);
         const unsigned long long s0 = (unsigned long long)e->sweeps_done;
         const bool ok = sizeof(Real) == 4
-                            ? rr_launch_f32(d, e->st, grid, n_sweeps, s0, flags, block_size, e->rr_variant, stream)
-                            : rr_launch_f64(d, e->st, grid, n_sweeps, s0, flags, block_size, e->rr_variant, stream);
+                            ? rr_launch_f32(d, e->st, grid, n_sweeps, s0, flags, block_size, e->rr_variant, stream, e->timer)
+                            : rr_launch_f64(d, e->st, grid, n_sweeps, s0, flags, block_size, e->rr_variant, stream, e->timer);
         if (!ok) return fail(VMC_ERR_INVALID, "no register-resident instantiation for this system");
         VMC_HIP_TRY(hipGetLastError());
         return VMC_OK;
     }
     const size_t lds_in = lds_for(d, sizeof(Real), true);
     const unsigned grid = static_cast<unsigned>(e->W);
+    hipEvent_t ev = nullptr;
+    const bool timed = e->timer.begin(stream, ev);
     if (lds_in <= kMaxLdsBytes) {
         hipLaunchKernelGGL((k_sweep<Real, true>), dim3(grid), dim3(kWave), lds_in, stream, d, e->st,
                            n_sweeps, (unsigned long long)e->sweeps_done, flags, block_size);
@@ -536,6 +539,7 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
         hipLaunchKernelGGL((k_sweep<Real, false>), dim3(grid), dim3(kWave), lds_out, stream, d,
                            e->st, n_sweeps, (unsigned long long)e->sweeps_done, flags, block_size);
     }
+    if (timed) e->timer.end(stream, ev, kTimeSweep);
     VMC_HIP_TRY(hipGetLastError());
     return VMC_OK;
 }
@@ -721,6 +725,36 @@ int vmc_ensemble_read_stats(vmc_ensemble *e, double *stats, void *stream) {
     VMC_HIP_TRY(hipMemcpyAsync(stats, e->st.stats, sizeof(double) * e->W * kStatCount, hipMemcpyDeviceToHost, s));
     VMC_HIP_TRY(hipStreamSynchronize(s));
     return VMC_OK;
+}
+
+int vmc_ensemble_set_timing(vmc_ensemble *e, int32_t enable) {
+    if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
+    e->timer.on = enable != 0;
+    return VMC_OK;
+}
+
+int vmc_ensemble_read_timing(vmc_ensemble *e, double *ms, int64_t *launches) {
+    if (!e || !ms || !launches) return fail(VMC_ERR_INVALID, "NULL argument");
+    for (int k = 0; k < kTimeKinds; ++k) {
+        ms[k] = 0.0;
+        launches[k] = 0;
+    }
+    KernelTimer &t = e->timer;
+    int rc = VMC_OK;
+    for (size_t i = 0; i < t.start.size(); ++i) {
+        float dt = 0.0f;
+        if (hipEventSynchronize(t.stop[i]) != hipSuccess ||
+            hipEventElapsedTime(&dt, t.start[i], t.stop[i]) != hipSuccess)
+            rc = fail(VMC_ERR_DEVICE, "kernel timing event failed");
+        ms[t.kind[i]] += dt;
+        launches[t.kind[i]] += 1;
+        t.pool.push_back(t.start[i]);
+        t.pool.push_back(t.stop[i]);
+    }
+    t.start.clear();
+    t.stop.clear();
+    t.kind.clear();
+    return rc;
 }
 
 int vmc_ensemble_read_drift(vmc_ensemble *e, double *drift, void *stream) {

@@ -3,7 +3,8 @@
 
 namespace vmc {
 bool rr_launch_f32(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,
-                   unsigned long long s0, int flags, int bs, int variant, hipStream_t stream) {
-    return rr_launch<float>(d, st, grid, n_sweeps, s0, flags, bs, variant, stream);
+                   unsigned long long s0, int flags, int bs, int variant, hipStream_t stream,
+                   KernelTimer &timer) {
+    return rr_launch<float>(d, st, grid, n_sweeps, s0, flags, bs, variant, stream, timer);
 }
 }  // namespace vmc

@@ -6,7 +6,48 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace vmc {
+
+// Kernel kinds timed by KernelTimer (vmc_ensemble_read_timing order).
+enum { kTimeSweep = 0, kTimeReinv = 1, kTimeMeasure = 2, kTimeKinds = 3 };
+
+// Device time per kernel kind from HIP events recorded on the launch stream
+// around each launch while enabled; summed when read.
+struct KernelTimer {
+    bool on = false;
+    std::vector<hipEvent_t> start, stop;
+    std::vector<int> kind;
+    std::vector<hipEvent_t> pool;
+
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t ev = pool.back();
+            pool.pop_back();
+            return ev;
+        }
+        hipEvent_t ev = nullptr;
+        return hipEventCreate(&ev) == hipSuccess ? ev : nullptr;
+    }
+    // Records the start of a launch; returns false when not timing.
+    bool begin(hipStream_t s, hipEvent_t &ev) {
+        if (!on || !(ev = take())) return false;
+        return hipEventRecord(ev, s) == hipSuccess;
+    }
+    void end(hipStream_t s, hipEvent_t ev0, int k) {
+        hipEvent_t ev1 = take();
+        if (!ev1 || hipEventRecord(ev1, s) != hipSuccess) return;
+        start.push_back(ev0);
+        stop.push_back(ev1);
+        kind.push_back(k);
+    }
+    ~KernelTimer() {
+        for (hipEvent_t ev : start) (void)hipEventDestroy(ev);
+        for (hipEvent_t ev : stop) (void)hipEventDestroy(ev);
+        for (hipEvent_t ev : pool) (void)hipEventDestroy(ev);
+    }
+};
 
 // Sizes with a register-resident instantiation: 7, 19 or 27 electrons per
 // spin, unpolarised with at most 5 G vectors per lane or polarised with at
@@ -20,8 +61,10 @@ inline bool rr_supported_sizes(int ndet, int n_up, int n_dn, int ng) {
 // Launches n_sweeps sweeps (and, with VMC_SWEEP_MEASURE, one measurement per
 // sweep).  variant 0: no occupancy bound; 1: at most 128 VGPRs.
 bool rr_launch_f32(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,
-                   unsigned long long s0, int flags, int bs, int variant, hipStream_t stream);
+                   unsigned long long s0, int flags, int bs, int variant, hipStream_t stream,
+                   KernelTimer &timer);
 bool rr_launch_f64(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,
-                   unsigned long long s0, int flags, int bs, int variant, hipStream_t stream);
+                   unsigned long long s0, int flags, int bs, int variant, hipStream_t stream,
+                   KernelTimer &timer);
 
 }  // namespace vmc

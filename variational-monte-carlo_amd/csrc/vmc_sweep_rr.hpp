@@ -586,10 +586,9 @@ struct RRKernel {
 
 // Philox draws (tags MOVE and ACCP, counter {electron, walker, sweep}),
 // Box-Muller or uniform-cube displacement, wrap into [0, L) and the axis
-// phasors of the proposal, written to the lane's LDS slots.  Out of line so
-// the transcendental constants do not occupy registers across the sweep.
+// phasors of the proposal, written to the lane's LDS slots.
 template <typename Real>
-__device__ __attribute__((noinline)) void rr_proposal(const DevSys &s, double sigma, double x0,
+__device__ __forceinline__ void rr_proposal(const DevSys &s, double sigma, double x0,
                                                       double y0, double z0, int e, int w,
                                                       uint32_t sweep, int lane, double *prop,
                                                       Real *pph) {
@@ -625,7 +624,7 @@ __device__ __attribute__((noinline)) void rr_proposal(const DevSys &s, double si
 __device__ __attribute__((noinline)) double rr_exp_f64(double x) { return exp(x); }
 
 template <typename Real, int NU, int ND, int GPL>
-__device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &st, int n_sweeps,
+__device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &st,
                                               unsigned long long sweep0, int flags) {
     using K = RRKernel<Real, NU, ND, GPL>;
     constexpr int NMAX = K::NMAX;
@@ -666,30 +665,27 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
     L.U = K::jastrow_row(cst, sh, L, lane, has_e, h);
     int since = st.since[w];
     int wflags = st.flags[w];
-    int n_acc = 0;
-
-    for (int sw = 0; sw < n_sweeps; ++sw) {
-        const uint32_t sweep = (uint32_t)(sweep0 + (unsigned long long)sw);
-        // Proposal of the lane's electron for this sweep (electron order is
-        // fixed, so its position is current until its own move); staged in
-        // LDS for the move that reads it.
-        if (has_e) rr_proposal<Real>(s, sh.misc[0], L.x, L.y, L.z, e, w, sweep, lane, &sh.prop[0][0], &sh.pph[0][0]);
-        __syncthreads();
-        int acc = 0;
+    // Proposal of the lane's electron for this sweep (electron order is fixed,
+    // so its position is current until its own move); staged in LDS for the
+    // move that reads it.  One sweep per launch keeps this straight-line code
+    // out of the move loop, so its transcendental constants die before it.
+    const uint32_t sweep = (uint32_t)sweep0;
+    if (has_e) rr_proposal<Real>(s, sh.misc[0], L.x, L.y, L.z, e, w, sweep, lane, &sh.prop[0][0], &sh.pph[0][0]);
+    __syncthreads();
+    int acc = 0;
 #pragma unroll 1
-        for (int p = 0; p < N; ++p) {
-            const int accepted = (ND == 0 || p < NU) ? K::template move<0>(cst, sh, L, lane, has_e, h, r, p)
-                                                     : K::template move<(ND > 0 ? 1 : 0)>(cst, sh, L, lane, has_e, h, r, p - NU);
-            acc += accepted;
-        }
-        n_acc += acc;
-        since += acc;
-        if (flags & VMC_SWEEP_ADAPT) {
-            const double f = rr_exp_f64(0.25 * ((double)acc / (double)N - 0.5));
-            if (lane == 0) {
-                const double sigma = sh.misc[0] * f;
-                sh.misc[0] = sigma > 0.5 * s.L ? 0.5 * s.L : sigma;
-            }
+    for (int p = 0; p < N; ++p) {
+        const int accepted = (ND == 0 || p < NU) ? K::template move<0>(cst, sh, L, lane, has_e, h, r, p)
+                                                 : K::template move<(ND > 0 ? 1 : 0)>(cst, sh, L, lane, has_e, h, r, p - NU);
+        acc += accepted;
+    }
+    const int n_acc = acc;
+    since += acc;
+    if (flags & VMC_SWEEP_ADAPT) {
+        const double f = exp(0.25 * ((double)acc / (double)N - 0.5));
+        if (lane == 0) {
+            const double sigma = sh.misc[0] * f;
+            sh.misc[0] = sigma > 0.5 * s.L ? 0.5 * s.L : sigma;
         }
     }
     Real *inv_out = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
@@ -705,7 +701,7 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
     __syncthreads();
     if (lane == 0) {
         sh.stat[6] += (double)n_acc;
-        sh.stat[7] += (double)n_sweeps * (double)N;
+        sh.stat[7] += (double)N;
 
         st.sigma[w] = sh.misc[0];
         st.since[w] = since;
@@ -720,15 +716,15 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
 // The sweep kernel, without and with a 4-waves-per-SIMD occupancy bound (the
 // bound caps VGPRs at 128 at the price of some spills; the autotuner picks).
 template <typename Real, int NU, int ND, int GPL>
-__global__ __launch_bounds__(kWave) void k_sweep_rr(DevSys s, DevState st, int n_sweeps,
+__global__ __launch_bounds__(kWave) void k_sweep_rr(DevSys s, DevState st,
                                                     unsigned long long sweep0, int flags) {
-    sweep_rr_body<Real, NU, ND, GPL>(s, st, n_sweeps, sweep0, flags);
+    sweep_rr_body<Real, NU, ND, GPL>(s, st, sweep0, flags);
 }
 
 template <typename Real, int NU, int ND, int GPL>
-__global__ __launch_bounds__(kWave, 4) void k_sweep_rr_o4(DevSys s, DevState st, int n_sweeps,
+__global__ __launch_bounds__(kWave, 4) void k_sweep_rr_o4(DevSys s, DevState st,
                                                           unsigned long long sweep0, int flags) {
-    sweep_rr_body<Real, NU, ND, GPL>(s, st, n_sweeps, sweep0, flags);
+    sweep_rr_body<Real, NU, ND, GPL>(s, st, sweep0, flags);
 }
 
 typedef double rr_v4d __attribute__((ext_vector_type(4)));
@@ -759,55 +755,70 @@ __device__ __forceinline__ void ns_gemm32(const double *A, const double *B, rr_v
 __device__ __forceinline__ int ns_row(int t, int q, int lane) { return 16 * (t >> 1) + 4 * q + (lane >> 4); }
 __device__ __forceinline__ int ns_col(int t, int lane) { return 16 * (t & 1) + (lane & 15); }
 
-// Shared memory of the post-sweep kernel: the sweep layout plus two FP64
-// 32 x 33 matrices for the Newton-Schulz refinement (the axis-power table of
-// the measurement aliases the first one).
-template <typename Real, int NU, int ND>
-struct RRPostShared {
-    RRShared<Real, NU, ND> base;
+// Shared memory of the re-inversion kernel: two FP64 32 x 33 matrices for
+// the Newton-Schulz refinement and the Gauss-Jordan pivot workspace.
+struct RRReinvShared {
     double E[32 * kNSStride];
     double Y[32 * kNSStride];
+    double gjcol[32];
+    int perm[32];
 };
 
-// Re-inversion of spin HP in FP64 on the matrix cores (Newton-Schulz):
-// with E = D^T (E[m][i] = phi_m(r_i), built from the positions in FP64) and
-// Y = invT, Z = E Y measures the drift |I - Z| (= |I - D^-1 D|), and
-// Y <- Y (2 I - Z) squares it; the FP32-drifted inverse of the sweep comes back
-// to FP64 accuracy in one step (two if the drift exceeds 1e-3).  A drift of
-// 0.5 or more (or a non-finite one) falls back to Gauss-Jordan from scratch.
-// Returns the drift found before refinement (negative if singular); a
-// Gauss-Jordan rebuild also refreshes log|det D| into `logdet`.
-template <typename Real, int NU, int ND, int GPL, int HP>
-__device__ double ns_reinvert(const DevSys &s, RRPostShared<Real, NU, ND> &ps,
-                              RRLane<Real, (NU > ND ? NU : ND)> &L, int lane, int h, int r,
-                              double &logdet) {
-    constexpr int NP = HP ? ND : NU;
-    constexpr int NMAX = NU > ND ? NU : ND;
+// E = D^T of one spin in FP64 into LDS (row stride `stride`): lane r of that
+// spin writes column r, E[m][r] = phi_m(r_r); with stride kNSStride the
+// 32 x 32 padding is zeroed too.
+template <int NP, int NMAX>
+__device__ __forceinline__ void ns_build_e(const DevSys &s, double *E, double x, double y,
+                                                     double z, bool mine, int r, int stride) {
     constexpr auto orb = rr::shell_orbitals<NMAX>();
-    // E: lane (HP, i) writes column i (its electron's D row) in FP64.
-    for (int e = lane; e < 32 * kNSStride; e += kWave) {
-        ps.E[e] = 0.0;
-        ps.Y[e] = 0.0;
-    }
-    __syncthreads();
-    if (h == HP && r < NP) {
+    if (stride == kNSStride)
+        for (int e = threadIdx.x; e < 32 * kNSStride; e += kWave) {
+            const int i = e / kNSStride, j = e - i * kNSStride;
+            if (i >= NP || j >= NP) E[e] = 0.0;
+        }
+    if (mine) {
         rr::Cx<double> a[3], a2[3];
-        a[0] = rr::axis_phasor<double>(L.x, s.inv_L);
-        a[1] = rr::axis_phasor<double>(L.y, s.inv_L);
-        a[2] = rr::axis_phasor<double>(L.z, s.inv_L);
+        a[0] = rr::axis_phasor<double>(x, s.inv_L);
+        a[1] = rr::axis_phasor<double>(y, s.inv_L);
+        a[2] = rr::axis_phasor<double>(z, s.inv_L);
 #pragma unroll
         for (int c = 0; c < 3; ++c) a2[c] = rr::cmul(a[c], a[c]);
 #pragma unroll
         for (int m = 0; m < NP; ++m) {
             const rr::Cx<double> ph = rr::orbital_phasor<double>(orb[m].x, orb[m].y, orb[m].z, a, a2);
-            ps.E[m * kNSStride + r] = orb[m].t ? ph.im : ph.re;
+            E[m * stride + r] = orb[m].t ? ph.im : ph.re;
         }
+    }
+}
+
+constexpr int kNSMaxSteps = 6;
+
+// Re-inversion of spin HP in FP64 on the matrix cores (Newton-Schulz):
+// with E = D^T (E[m][i] = phi_m(r_i), built from the positions in FP64) and
+// Y = invT, Z = E Y measures the drift |I - Z| (= |I - D^-1 D|), and
+// Y <- Y (2 I - Z) squares it; steps repeat (E rebuilt each time) until the
+// drift is at FP64 rounding -- one step for the usual FP32 drift of 1e-6..1e-4,
+// at most kNSMaxSteps from any drift below 0.5.  A drift of 0.5 or more (or a
+// non-finite one) falls back to Gauss-Jordan from scratch.
+// Returns the drift found before refinement (negative if singular); a
+// Gauss-Jordan rebuild also refreshes log|det D| into `logdet`.
+template <typename Real, int NU, int ND, int GPL, int HP>
+__device__ double ns_reinvert(const DevSys &s, RRReinvShared &ps,
+                              RRLane<Real, (NU > ND ? NU : ND)> &L, int lane, int h, int r,
+                              double &logdet) {
+    constexpr int NP = HP ? ND : NU;
+    constexpr int NMAX = NU > ND ? NU : ND;
+    // E: lane (HP, i) writes column i (its electron's D row) in FP64.
+    for (int e = lane; e < 32 * kNSStride; e += kWave) ps.Y[e] = 0.0;
+    ns_build_e<NP, NMAX>(s, ps.E, L.x, L.y, L.z, h == HP && r < NP, r, kNSStride);
+    if (h == HP && r < NP) {
 #pragma unroll
         for (int m = 0; m < NP; ++m) ps.Y[r * kNSStride + m] = (double)L.inv[m];
     }
     __syncthreads();
     double drift0 = -1.0;
-    for (int it = 0; it < 2; ++it) {
+#pragma unroll 1
+    for (int it = 0; it < kNSMaxSteps; ++it) {
         rr_v4d z[4];
         ns_gemm32(ps.E, ps.Y, z, lane);
         double dmax = 0.0;
@@ -833,7 +844,7 @@ __device__ double ns_reinvert(const DevSys &s, RRPostShared<Real, NU, ND> &ps,
             drift0 = INFINITY;
             break;
         }
-        if (it == 1 && dmax < 1e-13) break;
+        if (dmax < 1e-14) break;  // already at FP64 rounding
         __syncthreads();
         // W = 2 I - Z into E's storage, then Y <- Y W.
 #pragma unroll
@@ -852,45 +863,17 @@ __device__ double ns_reinvert(const DevSys &s, RRPostShared<Real, NU, ND> &ps,
 #pragma unroll
             for (int q = 0; q < 4; ++q) ps.Y[ns_row(t, q, lane) * kNSStride + ns_col(t, lane)] = y[t][q];
         __syncthreads();
-        if (dmax < 1e-3) break;
-        // second refinement: rebuild E (overwritten by W)
-        if (h == HP && r < NP) {
-            rr::Cx<double> a[3], a2[3];
-            a[0] = rr::axis_phasor<double>(L.x, s.inv_L);
-            a[1] = rr::axis_phasor<double>(L.y, s.inv_L);
-            a[2] = rr::axis_phasor<double>(L.z, s.inv_L);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) a2[c] = rr::cmul(a[c], a[c]);
-#pragma unroll
-            for (int m = 0; m < NP; ++m) {
-                const rr::Cx<double> ph = rr::orbital_phasor<double>(orb[m].x, orb[m].y, orb[m].z, a, a2);
-                ps.E[m * kNSStride + r] = orb[m].t ? ph.im : ph.re;
-            }
-        }
-        for (int e = lane; e < 32 * kNSStride; e += kWave) {
-            const int i = e / kNSStride, j = e - i * kNSStride;
-            if (i >= NP || j >= NP) ps.E[e] = 0.0;
-        }
+        if (dmax < 2e-8) break;  // the step left a drift of about dmax^2
+        // another step: rebuild E (overwritten by W)
+        ns_build_e<NP, NMAX>(s, ps.E, L.x, L.y, L.z, h == HP && r < NP, r, kNSStride);
         __syncthreads();
     }
     if (!(drift0 < 0.5)) {
         // Gauss-Jordan from scratch on A = D^T (row stride NP) in E's storage.
         __syncthreads();
-        if (h == HP && r < NP) {
-            rr::Cx<double> a[3], a2[3];
-            a[0] = rr::axis_phasor<double>(L.x, s.inv_L);
-            a[1] = rr::axis_phasor<double>(L.y, s.inv_L);
-            a[2] = rr::axis_phasor<double>(L.z, s.inv_L);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) a2[c] = rr::cmul(a[c], a[c]);
-#pragma unroll
-            for (int m = 0; m < NP; ++m) {
-                const rr::Cx<double> ph = rr::orbital_phasor<double>(orb[m].x, orb[m].y, orb[m].z, a, a2);
-                ps.E[m * NP + r] = orb[m].t ? ph.im : ph.re;
-            }
-        }
+        ns_build_e<NP, NMAX>(s, ps.E, L.x, L.y, L.z, h == HP && r < NP, r, NP);
         __syncthreads();
-        const double ld = rr_gauss_jordan(ps.E, ps.base.gjcol, ps.base.perm, NP, lane);
+        const double ld = rr_gauss_jordan(ps.E, ps.gjcol, ps.perm, NP, lane);
         if (!(ld > -INFINITY)) return -1.0;
         logdet = ld;
         if (h == HP && r < NP) {
@@ -908,24 +891,17 @@ __device__ double ns_reinvert(const DevSys &s, RRPostShared<Real, NU, ND> &ps,
     return drift0;
 }
 
-// After a sweep: the periodic FP64 re-inversion of walkers whose accepted
-// moves since the last one reached K (flag VMC_POST_REINVERT), then the local
-// energy (flag VMC_POST_MEASURE) with the same lane layout as the sweep,
-// feeding the FP64 accumulators and blocking (blocking_analysis.cpp:17-39).
-// Stat VMC_STAT_REINV counts re-inversions; the largest drift |I - D^-1 D|
-// met before a re-inversion is kept per walker in st.drift.
-constexpr int kPostMeasure = 1;
-constexpr int kPostReinvert = 2;
-
+// After a sweep: the periodic FP64 re-inversion of every walker whose
+// accepted moves since its last one reached K.  Stat VMC_STAT_REINV counts
+// re-inversions; the largest drift |I - D^-1 D| met before one is kept per
+// walker in st.drift.  Walkers not due leave at once.
 template <typename Real, int NU, int ND, int GPL>
-__global__ __launch_bounds__(kWave) void k_post_rr(DevSys s, DevState st, int block_size, int flags) {
-    using K = RRKernel<Real, NU, ND, GPL>;
-    constexpr int NMAX = K::NMAX;
+__global__ __launch_bounds__(kWave) void k_reinv_rr(DevSys s, DevState st) {
+    constexpr int NMAX = NU > ND ? NU : ND;
     constexpr int N = NU + ND;
-    __shared__ RRPostShared<Real, NU, ND> ps;
-    RRShared<Real, NU, ND> &sh = ps.base;
-    const RRConst<Real> cst(s);
     const int w = blockIdx.x;
+    if (s.reinv_k <= 0 || st.since[w] < s.reinv_k) return;
+    __shared__ RRReinvShared ps;
     const int lane = threadIdx.x;
     const int h = lane >> 5, r = lane & 31;
     const int nh = h ? ND : NU;
@@ -940,34 +916,56 @@ __global__ __launch_bounds__(kWave) void k_post_rr(DevSys s, DevState st, int bl
     L.x = has_e ? gp[e] : 0.0;
     L.y = has_e ? gp[N + e] : 0.0;
     L.z = has_e ? gp[2 * N + e] : 0.0;
+    double ld0 = st.logdet[2 * w + 0], ld1 = st.logdet[2 * w + 1];
+    double d = ns_reinvert<Real, NU, ND, GPL, 0>(s, ps, L, lane, h, r, ld0);
+    if constexpr (ND > 0) {
+        const double d1 = ns_reinvert<Real, NU, ND, GPL, 1>(s, ps, L, lane, h, r, ld1);
+        d = (d < 0.0 || d1 < 0.0) ? -1.0 : fmax(d, d1);
+    }
+    if (has_e) {
+#pragma unroll
+        for (int m = 0; m < NMAX; ++m)
+            if (m < nh) inv_hbm[s.inv_off[h] + r * nh + m] = L.inv[m];
+    }
+    if (lane == 0) {
+        st.since[w] = 0;
+        st.logdet[2 * w + 0] = ld0;
+        st.logdet[2 * w + 1] = ld1;
+        if (d < 0.0) st.flags[w] |= 1;
+        st.stats[(size_t)w * kStatCount + 15] += 1.0;
+        if (d > st.drift[w]) st.drift[w] = d;
+    }
+}
+
+// After a sweep: the local energy of every walker with the same lane layout
+// as the sweep, feeding the FP64 accumulators and the blocking state
+// (blocking_analysis.cpp:17-39).
+template <typename Real, int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave) void k_measure_rr(DevSys s, DevState st, int block_size) {
+    using K = RRKernel<Real, NU, ND, GPL>;
+    constexpr int NMAX = K::NMAX;
+    constexpr int N = NU + ND;
+    __shared__ RRShared<Real, NU, ND> sh;
+    const RRConst<Real> cst(s);
+    const int w = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int h = lane >> 5, r = lane & 31;
+    const int nh = h ? ND : NU;
+    const bool has_e = r < nh;
+    const int e = h ? NU + r : r;
+    RRLane<Real, NMAX> L;
+    const Real *inv_hbm = reinterpret_cast<const Real *>(st.inv) + (size_t)w * s.ninv;
+    const double *gp = st.pos + (size_t)w * 3 * N;
+#pragma unroll
+    for (int m = 0; m < NMAX; ++m)
+        L.inv[m] = (has_e && m < nh) ? inv_hbm[s.inv_off[h] + r * nh + m] : (Real)0;
+    L.x = has_e ? gp[e] : 0.0;
+    L.y = has_e ? gp[N + e] : 0.0;
+    L.z = has_e ? gp[2 * N + e] : 0.0;
     sh.pos[0][lane] = L.x;
     sh.pos[1][lane] = L.y;
     sh.pos[2][lane] = L.z;
     __syncthreads();
-    if ((flags & kPostReinvert) && s.reinv_k > 0 && st.since[w] >= s.reinv_k) {
-        double ld0 = st.logdet[2 * w + 0], ld1 = st.logdet[2 * w + 1];
-        double d = ns_reinvert<Real, NU, ND, GPL, 0>(s, ps, L, lane, h, r, ld0);
-        if constexpr (ND > 0) {
-            const double d1 = ns_reinvert<Real, NU, ND, GPL, 1>(s, ps, L, lane, h, r, ld1);
-            d = (d < 0.0 || d1 < 0.0) ? -1.0 : fmax(d, d1);
-        }
-        if (has_e) {
-#pragma unroll
-            for (int m = 0; m < NMAX; ++m)
-                if (m < nh) inv_hbm[s.inv_off[h] + r * nh + m] = L.inv[m];
-        }
-        if (lane == 0) {
-            st.since[w] = 0;
-            st.logdet[2 * w + 0] = ld0;
-            st.logdet[2 * w + 1] = ld1;
-            if (d < 0.0) st.flags[w] |= 1;
-            double *t = st.stats + (size_t)w * kStatCount;
-            t[15] += 1.0;
-            if (d > st.drift[w]) st.drift[w] = d;
-        }
-        __syncthreads();
-    }
-    if (!(flags & kPostMeasure)) return;
     const Measured m = K::measure(s, cst, sh, L, lane, has_e, h);
     if (lane == 0) {
         double *t = st.stats + (size_t)w * kStatCount;

@@ -157,6 +157,8 @@ def lib():
             "vmc_ensemble_read_stats": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_read_steps": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_read_drift": (C.c_int, [_VP, _dp, _VP]),
+            "vmc_ensemble_set_timing": (C.c_int, [_VP, C.c_int32]),
+            "vmc_ensemble_read_timing": (C.c_int, [_VP, _dp, C.POINTER(C.c_int64)]),
             "vmc_ensemble_info": (C.c_int, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
             "vmc_frozen_eval": (C.c_int, [sp, C.c_int32, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _dp,
                                           _VP]),
@@ -261,6 +263,17 @@ class Ensemble:
         s = np.zeros(self.W)
         _check(lib().vmc_ensemble_read_drift(self._h, _ptr(s), self.stream))
         return s
+
+    def set_timing(self, on=True):
+        """Record HIP events around every kernel launch on the ensemble's stream."""
+        _check(lib().vmc_ensemble_set_timing(self._h, 1 if on else 0))
+
+    def timing(self):
+        """{'sweep' | 'reinvert' | 'measure': (ms, launches)} since the last read."""
+        ms = np.zeros(3)
+        n = np.zeros(3, dtype=np.int64)
+        _check(lib().vmc_ensemble_read_timing(self._h, _ptr(ms), n.ctypes.data_as(C.POINTER(C.c_int64))))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(("sweep", "reinvert", "measure"))}
 
     def sweeps_done(self):
         w, s = C.c_int64(), C.c_uint64()
