@@ -201,7 +201,7 @@ def main():
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--mixed-only", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run one plain device copy of 1 GiB (HBM counter calibration)")
+                    help="also run plain device copies of 1 GiB (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the walkers are independent chains and one MI355X holds "
@@ -212,9 +212,13 @@ def main():
 
     V.device_check()
     if a.copy_calibration:
+        # Two plain 1 GiB copies for the HBM byte counters: the runtime's
+        # device-to-device copy and an elementwise kernel (per-lane vector
+        # loads and stores, like this library's kernels).
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
+        torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
     res = {}

@@ -193,6 +193,12 @@ int vmc_ensemble_set_timing(vmc_ensemble *e, int32_t enable);
 int vmc_ensemble_read_timing(vmc_ensemble *e, double *ms, int64_t *launches);
 /* Copy per-walker step sizes to host: [W]. */
 int vmc_ensemble_read_steps(vmc_ensemble *e, double *steps, void *stream);
+/* One walker's output frame: its VMC_STAT_COUNT statistics and its positions
+ * ([N][3], AoS) -- the data behind Simulation::measure's FrameData
+ * (src/simulation/simulation.cpp:226-235) for the walker the host streams
+ * out.  Either pointer may be NULL. */
+int vmc_ensemble_read_frame(vmc_ensemble *e, int64_t walker, double *stats, double *pos,
+                            void *stream);
 /* Global sweep counter (the Philox stream position) and walker count. */
 int vmc_ensemble_info(const vmc_ensemble *e, int64_t *n_walkers, uint64_t *sweeps_done);
 

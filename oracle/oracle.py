@@ -370,6 +370,8 @@ def ref():
                                          C.c_double, C.c_uint64, C.c_size_t, C.c_double,
                                          C.c_double, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _dp]
         R.ref_blocking.argtypes = [C.c_size_t, _dp, C.c_size_t, _dp, _dp]
+        R.ref_write_fixed.argtypes = [C.c_int, C.c_char_p, _ip]
+        R.ref_config.argtypes = [C.c_char_p, _dp]
         _ref = R
     return _ref
 
@@ -467,3 +469,35 @@ def bench_threads(sysd, seed, step, threads, sweeps, measure=True, move_kind=1):
     rate = lib().orc_bench_threads(C.byref(sysd), C.byref(cfg), threads, sweeps, int(measure),
                                    C.byref(el))
     return rate, el.value
+
+
+REF_CONFIG_FIELDS = ("num_threads", "num_particles", "warmup_sweeps", "measure_sweeps",
+                     "box_length", "block_size", "master_seed", "jastrow_a", "jastrow_b",
+                     "warmup_steps", "measure_steps", "step_size")
+
+
+def ref_write_fixed(fmt, path):
+    """The reference's writer of format fmt ('json', 'csv', 'bin') over the
+    facade self test's fixed records.  Returns None, or the exception kind."""
+    what = C.c_int32()
+    rc = ref().ref_write_fixed({"json": 0, "csv": 1, "bin": 2}[fmt], str(path).encode(),
+                               C.byref(what))
+    return None if rc == 0 else {1: "invalid_argument", 2: "logic_error"}[what.value]
+
+
+def ref_config(path):
+    """The reference's Config::from_file fields, or 'invalid_argument'."""
+    out = np.zeros(12)
+    if ref().ref_config(str(path).encode(), _ptr(out)):
+        return "invalid_argument"
+    return dict(zip(REF_CONFIG_FIELDS, out.tolist()))
+
+
+def ref_blocking(block_size, x):
+    """The reference's BlockingAnalysis over x: (mean, SE), or None if fewer
+    than two blocks completed."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    m, se = C.c_double(), C.c_double()
+    if ref().ref_blocking(block_size, _ptr(x), x.size, C.byref(m), C.byref(se)):
+        return None
+    return m.value, se.value

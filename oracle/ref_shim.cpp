@@ -46,6 +46,7 @@
 #include "slater_plane_wave/slater_plane_wave.hpp"
 #include "wavefunction/wavefunction.hpp"
 #include "blocking_analysis/blocking_analysis.hpp"
+#include "config/config.hpp"
 #undef private
 
 #include <cstdint>
@@ -284,6 +285,56 @@ int ref_blocking(std::size_t block_size, const double* x, std::size_t count, dou
     const auto [m, s]{b.mean_and_standard_error()};
     *mean = m;
     *se = s;
+    return 0;
+}
+
+// The reference's writers (output_writer.cpp) over the fixed record sequence
+// the facade's self test writes: fmt 0 JSON, 1 CSV, 2 BIN.  Returns 0, or 3
+// with the exception kind (1 invalid_argument, 2 logic_error) in *what.
+int ref_write_fixed(int fmt, const char* path, int* what) {
+    std::ofstream out{path, std::ios::binary | std::ios::trunc};
+    *what = 0;
+    try {
+        auto w{make_output_writer(fmt == 0 ? OutputFormat::JSON : fmt == 1 ? OutputFormat::CSV
+                                                                            : OutputFormat::BIN, out)};
+        w->write_init(InitData{.run_id = "vmc-seed-7\t\"q\"", .num_particles = 2, .box_length = 3.25,
+                               .warmup_steps = 20, .measure_steps = 4, .step_size = 0.325, .seed = 7,
+                               .block_size = 2});
+        w->write_frame(FrameData{.step = 1, .accepted = 1, .proposed = 1, .acceptance_rate = 1.0,
+                                 .local_energy = -0.1, .mean_energy = -0.1, .standard_error = std::nullopt,
+                                 .positions = {0.1, 0.2, 0.3, 1.0 / 3.0, 2.5, 3.0}});
+        w->write_frame(FrameData{.step = 2, .accepted = 1, .proposed = 2, .acceptance_rate = 0.5,
+                                 .local_energy = 0.7, .mean_energy = 0.3, .standard_error = 0.125,
+                                 .positions = {0.15, 0.2, 0.3, 1.0 / 3.0, 2.5, 3.0}});
+        w->write_done(DoneData{.total_accepted = 1, .total_proposed = 2, .final_acceptance_rate = 0.5,
+                               .final_mean_energy = 0.3, .final_standard_error = 0.125});
+    } catch (const std::invalid_argument&) {
+        *what = 1;
+        return 3;
+    } catch (const std::logic_error&) {
+        *what = 2;
+        return 3;
+    }
+    return 0;
+}
+
+// Config::from_file (config.hpp:32-60) of a file: the fields in a fixed
+// order (num_threads, num_particles, warmup_sweeps, measure_sweeps,
+// box_length, block_size, master_seed, jastrow_a, jastrow_b, warmup_steps,
+// measure_steps, step_size).  Returns 0, or 1 on std::invalid_argument.
+int ref_config(const char* path, double* out) {
+    try {
+        const Config c{Config::from_file(path)};
+        const double v[12]{static_cast<double>(c.num_threads), static_cast<double>(c.num_particles),
+                           static_cast<double>(c.warmup_sweeps), static_cast<double>(c.measure_sweeps),
+                           c.box_length, static_cast<double>(c.block_size),
+                           static_cast<double>(c.master_seed), c.jastrow_a, c.jastrow_b,
+                           static_cast<double>(c.warmup_steps), static_cast<double>(c.measure_steps),
+                           c.step_size};
+        std::copy(v, v + 12, out);
+    } catch (const std::invalid_argument&) {
+        return 1;
+    }
     return 0;
 }
 

@@ -773,6 +773,27 @@ int vmc_ensemble_read_steps(vmc_ensemble *e, double *steps, void *stream) {
     return VMC_OK;
 }
 
+int vmc_ensemble_read_frame(vmc_ensemble *e, int64_t walker, double *stats, double *pos,
+                            void *stream) {
+    if (!e || (!stats && !pos)) return fail(VMC_ERR_INVALID, "NULL argument");
+    if (walker < 0 || walker >= e->W) return fail(VMC_ERR_INVALID, "walker %lld out of range", (long long)walker);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int N = e->tables.dev.N;
+    const size_t w = static_cast<size_t>(walker);
+    std::vector<double> soa(pos ? 3 * static_cast<size_t>(N) : 0);
+    if (stats)
+        VMC_HIP_TRY(hipMemcpyAsync(stats, e->st.stats + w * kStatCount, sizeof(double) * kStatCount,
+                                   hipMemcpyDeviceToHost, s));
+    if (pos)
+        VMC_HIP_TRY(hipMemcpyAsync(soa.data(), e->st.pos + w * 3 * N, sizeof(double) * soa.size(),
+                                   hipMemcpyDeviceToHost, s));
+    VMC_HIP_TRY(hipStreamSynchronize(s));
+    if (pos)
+        for (int i = 0; i < N; ++i)
+            for (int c = 0; c < 3; ++c) pos[3 * i + c] = soa[static_cast<size_t>(c) * N + i];
+    return VMC_OK;
+}
+
 int vmc_ensemble_info(const vmc_ensemble *e, int64_t *n_walkers, uint64_t *sweeps_done) {
     if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
     if (n_walkers) *n_walkers = e->W;

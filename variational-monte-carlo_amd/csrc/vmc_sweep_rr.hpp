@@ -217,6 +217,20 @@ struct RRConst {
     __device__ double mi(double d) const {
         return d + L * (double)(d <= -half_L) - L * (double)(d > half_L);
     }
+    // Minimum-image separation a - b in the working precision: FP64 is the
+    // reference's masked form (jastrow_pade.cpp:18-24); FP32 rounds the FP64
+    // difference once and folds it with FP32 selects.
+    __device__ Real mi_r(double a, double b) const {
+        if constexpr (sizeof(Real) == 8) {
+            return (Real)mi(a - b);
+        } else {
+            Real d = (Real)(a - b);
+            const Real l = (Real)L, hl = (Real)half_L;
+            d = d <= -hl ? d + l : d;
+            d = d > hl ? d - l : d;
+            return d;
+        }
+    }
     // erfc(alpha r) / r with the r < 1e-12 guard and optional cutoff
     // (energy_tracking.cpp:101-139).
     __device__ Real ewald(Real r) const {
@@ -509,25 +523,26 @@ struct RRKernel {
         const Real apair = (h == HP) ? c.a_same : c.a_anti;
         Real un = (Real)0;
         if (has_e && lane != lp) {
-            const Real dx = (Real)c.mi(xn - L.x);
-            const Real dy = (Real)c.mi(yn - L.y);
-            const Real dz = (Real)c.mi(zn - L.z);
+            const Real dx = c.mi_r(xn, L.x);
+            const Real dy = c.mi_r(yn, L.y);
+            const Real dz = c.mi_r(zn, L.z);
             const Real rn = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
             un = apair * rn * wrcp<Real>((Real)1 + c.b * rn);
         }
         __syncthreads();
-        // s_k = sum_m phi_m invT[k][m] for every row k of spin HP.
+        // s_k = sum_m phi_m invT[k][m] for every row k of spin HP (computed
+        // on every lane without branching; only spin HP's rows are used).
         Real sk0 = (Real)0, sk1 = (Real)0;
-        if (h == HP && r < NP) {
 #pragma unroll
-            for (int m = 0; m < NP; m += 2) {
-                sk0 += sh.phi[m] * L.inv[m];
-                if (m + 1 < NP) sk1 += sh.phi[m + 1] * L.inv[m + 1];
-            }
+        for (int m = 0; m < NP; m += 2) {
+            sk0 += sh.phi[m] * L.inv[m];
+            if (m + 1 < NP) sk1 += sh.phi[m + 1] * L.inv[m + 1];
         }
         const Real sk = sk0 + sk1;
         const double R = (double)rr::readlane(sk, lp);
-        const double jn = rr::wave_total((double)un);
+        // New Jastrow row sum of the mover: summed in the working precision
+        // (its FP32 rounding only moves the accept threshold by ~1e-6).
+        const double jn = (double)rr::wave_total(un);
         const double dj = jn - rr::readlane(L.U, lp);
         const double ua = sh.prop[3][lp];
         // Metropolis test u < R^2 exp(2 dJ) (simulation.cpp:125-131).  The
@@ -537,15 +552,16 @@ struct RRKernel {
         const double r2 = R * R;
         const float e32 = __builtin_amdgcn_exp2f((float)(2.0 * dj * 1.4426950408889634));
         const double approx = r2 * (double)e32;
-        bool acc;
+        int acc;
         if (!(approx == approx)) {
-            acc = false;
+            acc = 0;
         } else if (fabs(ua - approx) > 1e-5 * approx || !(fabs(dj) < 40.0)) {
             acc = ua < approx;
         } else {
             acc = ua < r2 * rr_exp_f64(2.0 * dj);
         }
-        if (!acc) return 0;
+        // Every input above is wave-uniform; say so, so the branch is scalar.
+        if (!wave_uniform(acc)) return 0;
         // Accept (simulation.cpp:133-145).
         if (lane == lp) {
 #pragma unroll
@@ -553,21 +569,21 @@ struct RRKernel {
         }
         const double xo = rr::readlane(L.x, lp), yo = rr::readlane(L.y, lp), zo = rr::readlane(L.z, lp);
         __syncthreads();
-        if (h == HP && r < NP) {
-            const Real inv_ratio = (Real)(1.0 / R);
-            if (r == rp) {
+        {
+            // Sherman-Morrison as one FMA per element on every lane: rows
+            // k != p of spin HP lose (s_k / R) row_p; row p itself (row_p is
+            // its own registers) loses (1 - 1/R) of itself, i.e. becomes
+            // row_p / R; every other lane subtracts 0 x (a finite row).
+            const Real inv_ratio = wrcp<Real>((Real)R);
+            Real f = (r == rp) ? (Real)1 - inv_ratio : sk * inv_ratio;
+            f = (h == HP && r < NP) ? f : (Real)0;
 #pragma unroll
-                for (int m = 0; m < NP; ++m) L.inv[m] *= inv_ratio;
-            } else {
-                const Real f = sk * inv_ratio;
-#pragma unroll
-                for (int m = 0; m < NP; ++m) L.inv[m] -= sh.row[m] * f;
-            }
+            for (int m = 0; m < NP; ++m) L.inv[m] -= sh.row[m] * f;
         }
         if (has_e && lane != lp) {
-            const Real dx = (Real)c.mi(xo - L.x);
-            const Real dy = (Real)c.mi(yo - L.y);
-            const Real dz = (Real)c.mi(zo - L.z);
+            const Real dx = c.mi_r(xo, L.x);
+            const Real dy = c.mi_r(yo, L.y);
+            const Real dz = c.mi_r(zo, L.z);
             const Real ro = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
             L.U += (double)un - (double)(apair * ro * wrcp<Real>((Real)1 + c.b * ro));
         }
@@ -729,31 +745,40 @@ __global__ __launch_bounds__(kWave, 4) void k_sweep_rr_o4(DevSys s, DevState st,
 
 typedef double rr_v4d __attribute__((ext_vector_type(4)));
 
-// C (32 x 32) = A * B for FP64 row-major LDS matrices of row stride kNSStride,
-// on the matrix cores: 2 x 2 output tiles of v_mfma_f64_16x16x4f64, 8 k-steps.
-// Operand layout of that instruction: lane l supplies A[l & 15][l >> 4] and
-// B[l >> 4][l & 15]; output register q of lane l is C[4q + (l >> 4)][l & 15].
+// C = A * B for FP64 row-major LDS matrices of row stride kNSStride and
+// order 16 TT (TT = 1 or 2), on the matrix cores: TT x TT output tiles of
+// v_mfma_f64_16x16x4f64 over 4 TT k-steps.  Operand layout of that
+// instruction: lane l supplies A[l & 15][l >> 4] and B[l >> 4][l & 15];
+// output register q of lane l is C[4q + (l >> 4)][l & 15].
 constexpr int kNSStride = 33;
-__device__ __forceinline__ void ns_gemm32(const double *A, const double *B, rr_v4d acc[4],
-                                          int lane) {
+template <int TT>
+__device__ __forceinline__ void ns_gemm(const double *A, const double *B, rr_v4d acc[TT * TT],
+                                        int lane) {
     const int lo = lane & 15, hi = lane >> 4;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = rr_v4d{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < TT * TT; ++t) acc[t] = rr_v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
+    for (int kk = 0; kk < 4 * TT; ++kk) {
         const int k = 4 * kk + hi;
-        const double a0 = A[lo * kNSStride + k], a1 = A[(16 + lo) * kNSStride + k];
-        const double b0 = B[k * kNSStride + lo], b1 = B[k * kNSStride + 16 + lo];
-        acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[1], 0, 0, 0);
-        acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[2], 0, 0, 0);
-        acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[3], 0, 0, 0);
+        double a[TT], b[TT];
+#pragma unroll
+        for (int u = 0; u < TT; ++u) {
+            a[u] = A[(16 * u + lo) * kNSStride + k];
+            b[u] = B[k * kNSStride + 16 * u + lo];
+        }
+#pragma unroll
+        for (int u = 0; u < TT; ++u)
+#pragma unroll
+            for (int v = 0; v < TT; ++v)
+                acc[u * TT + v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[v], acc[u * TT + v], 0, 0, 0);
     }
 }
 
-// Row / column of accumulator q of tile t for this lane.
-__device__ __forceinline__ int ns_row(int t, int q, int lane) { return 16 * (t >> 1) + 4 * q + (lane >> 4); }
-__device__ __forceinline__ int ns_col(int t, int lane) { return 16 * (t & 1) + (lane & 15); }
+// Row / column of accumulator q of tile t (= u TT + v) for this lane.
+template <int TT>
+__device__ __forceinline__ int ns_row(int t, int q, int lane) { return 16 * (t / TT) + 4 * q + (lane >> 4); }
+template <int TT>
+__device__ __forceinline__ int ns_col(int t, int lane) { return 16 * (t % TT) + (lane & 15); }
 
 // Shared memory of the re-inversion kernel: two FP64 32 x 33 matrices for
 // the Newton-Schulz refinement and the Gauss-Jordan pivot workspace.
@@ -795,9 +820,10 @@ constexpr int kNSMaxSteps = 6;
 
 // Re-inversion of spin HP in FP64 on the matrix cores (Newton-Schulz):
 // with E = D^T (E[m][i] = phi_m(r_i), built from the positions in FP64) and
-// Y = invT, Z = E Y measures the drift |I - Z| (= |I - D^-1 D|), and
-// Y <- Y (2 I - Z) squares it; steps repeat (E rebuilt each time) until the
-// drift is at FP64 rounding -- one step for the usual FP32 drift of 1e-6..1e-4,
+// Y = invT, Z = E Y measures the drift d = |I - Z| (= |I - D^-1 D|), and
+// Y <- Y (2 I - Z) leaves a drift of about d^2.  Steps repeat (E rebuilt each
+// time) until d^2 is below the rounding of the stored inverse (1e-8 for an
+// FP32 inverse, 1e-16 for FP64): one step for the usual drift of 1e-6..1e-4,
 // at most kNSMaxSteps from any drift below 0.5.  A drift of 0.5 or more (or a
 // non-finite one) falls back to Gauss-Jordan from scratch.
 // Returns the drift found before refinement (negative if singular); a
@@ -816,18 +842,20 @@ __device__ double ns_reinvert(const DevSys &s, RRReinvShared &ps,
         for (int m = 0; m < NP; ++m) ps.Y[r * kNSStride + m] = (double)L.inv[m];
     }
     __syncthreads();
+    constexpr int TT = NP > 16 ? 2 : 1;
+    constexpr double kDone = sizeof(Real) == 4 ? 1e-4 : 1e-8;  // d below this: d^2 is rounding
     double drift0 = -1.0;
 #pragma unroll 1
     for (int it = 0; it < kNSMaxSteps; ++it) {
-        rr_v4d z[4];
-        ns_gemm32(ps.E, ps.Y, z, lane);
+        rr_v4d z[TT * TT];
+        ns_gemm<TT>(ps.E, ps.Y, z, lane);
         double dmax = 0.0;
         bool finite = true;
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < TT * TT; ++t)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int i = ns_row(t, q, lane), j = ns_col(t, lane);
+                const int i = ns_row<TT>(t, q, lane), j = ns_col<TT>(t, lane);
                 const double d = (i == j ? 1.0 : 0.0) - z[t][q];
                 if (i < NP && j < NP) {
                     dmax = fabs(d) > dmax ? fabs(d) : dmax;
@@ -844,26 +872,26 @@ __device__ double ns_reinvert(const DevSys &s, RRReinvShared &ps,
             drift0 = INFINITY;
             break;
         }
-        if (dmax < 1e-14) break;  // already at FP64 rounding
+        if (dmax < 1e-15) break;  // already at FP64 rounding
         __syncthreads();
         // W = 2 I - Z into E's storage, then Y <- Y W.
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < TT * TT; ++t)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int i = ns_row(t, q, lane), j = ns_col(t, lane);
+                const int i = ns_row<TT>(t, q, lane), j = ns_col<TT>(t, lane);
                 ps.E[i * kNSStride + j] = (i == j ? 2.0 : 0.0) - z[t][q];
             }
         __syncthreads();
-        rr_v4d y[4];
-        ns_gemm32(ps.Y, ps.E, y, lane);
+        rr_v4d y[TT * TT];
+        ns_gemm<TT>(ps.Y, ps.E, y, lane);
         __syncthreads();
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < TT * TT; ++t)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) ps.Y[ns_row(t, q, lane) * kNSStride + ns_col(t, lane)] = y[t][q];
+            for (int q = 0; q < 4; ++q) ps.Y[ns_row<TT>(t, q, lane) * kNSStride + ns_col<TT>(t, lane)] = y[t][q];
         __syncthreads();
-        if (dmax < 2e-8) break;  // the step left a drift of about dmax^2
+        if (dmax < kDone) break;
         // another step: rebuild E (overwritten by W)
         ns_build_e<NP, NMAX>(s, ps.E, L.x, L.y, L.z, h == HP && r < NP, r, kNSStride);
         __syncthreads();
