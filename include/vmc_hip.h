@@ -178,7 +178,7 @@ int vmc_ensemble_reset_stats(vmc_ensemble *e, void *stream);
 int vmc_ensemble_read_stats(vmc_ensemble *e, double *stats, void *stream);
 /* Copy per-walker inverse drift to host: [W], the largest max-norm of
  * I - D^-1 D met when a walker's mixed-precision inverse came due for its
- * FP64 re-inversion (register kernel; 0 until one happens).  No reference
+ * FP64 re-inversion (0 until one happens).  No reference
  * counterpart: the reference works in FP64 throughout and never re-inverts. */
 int vmc_ensemble_read_drift(vmc_ensemble *e, double *drift, void *stream);
 /* Per-kernel device time: while enabled, every sweep launch records HIP

@@ -134,3 +134,29 @@ def test_frozen_mixed_error_budget(vmc, oracle, name, factory):
     print(" ".join(f"{k}={float(v):.2e}" for k, v in errs.items()))
     bad = {k: float(v) for k, v in errs.items() if not v <= MIXED_BUDGET[k]}
     assert not bad, f"mixed precision beyond budget {MIXED_BUDGET}: {bad}"
+
+
+def test_n114_mixed_error_report(vmc, oracle):
+    """The mixed-precision configuration of N = 114 (57 + 57, |n|^2 <= 5
+    shells), r_s = 5, b = 0.8: ratio, log|det| and local energy against the
+    FP64 oracle on 256 seeded frozen configurations, to the mixed budget,
+    except the ratio: a 57-term FP32 dot against an FP32-rounded inverse,
+    held to 5e-5 of max(|R|, 1e-2)."""
+    factory = lambda V, p: V.System.heg(114, 5.0, 0.8, precision=p)
+    s, got, ref = run_case(vmc, oracle, factory, vmc.PREC_MIXED, 256, seed=114)
+    assert not np.any(got["SINGULAR"])
+    kin, vre, vrc, el = (gather(ref, k) for k in ("kinetic", "vreal", "vrecip", "e_local"))
+    escale = np.abs(kin) + np.abs(vre) + np.abs(vrc) + np.abs(el - kin - vre - vrc)
+    errs = dict(
+        ratio=np.max(np.abs(got["RATIO"] - gather(ref, "ratio")) /
+                     np.maximum(np.abs(gather(ref, "ratio")), 1e-2)),
+        logdet=max(np.max(np.abs(got[f"LOGDET{d}"] - [r["logdet"][d] for r in ref]) /
+                          np.maximum(np.abs([r["logdet"][d] for r in ref]), 1.0)) for d in (0, 1)),
+        e_local=np.max(np.abs(got["ELOCAL"] - el) / escale),
+        e_local_rel=np.median(np.abs(got["ELOCAL"] - el) / np.abs(el)),
+    )
+    print(" ".join(f"{k}={float(v):.2e}" for k, v in errs.items()))
+    budget = dict(ratio=5e-5, logdet=MIXED_BUDGET["logdet"],
+                  e_local=MIXED_BUDGET["e_local"], e_local_rel=1e-5)
+    bad = {k: float(v) for k, v in errs.items() if not v <= budget[k]}
+    assert not bad, f"N=114 mixed precision beyond budget {budget}: {bad}"

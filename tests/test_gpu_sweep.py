@@ -216,3 +216,27 @@ def test_kernel_timing_counts_every_launch(vmc):
     assert t["sweep"][1] == 5 and t["reinvert"][1] == 5 and t["measure"][1] == 3
     assert all(v[0] > 0 for v in t.values())
     assert all(v[1] == 0 for v in ens.timing().values())
+
+
+@pytest.mark.parametrize("K", [16, 64, 256])
+def test_n114_reinversion_interval_drift_and_energy(vmc, oracle, K):
+    """N = 114, r_s = 5, b = 0.8 in mixed precision with FP64 re-inversion
+    every K accepted moves (the generic kernel): the inverse drift met before
+    each re-inversion is recorded, stays small, and the local energy after
+    the sweeps matches the FP64 oracle at the walker's positions."""
+    s = vmc.System.heg(114, 5.0, 0.8, precision=1, reinvert_interval=K)
+    W = 16
+    ens = vmc.Ensemble(s, W, 1140 + K, step0=1.5)
+    ens.init_positions()
+    ens.sweep(12, measure=True, adapt=True)
+    st = ens.stats()
+    assert np.all(st[:, vmc.STAT["REINV"]] >= (1 if K == 256 else 4))
+    d = ens.drift()
+    assert np.all(d > 0) and np.all(d < 1e-3), d
+    pos = ens.positions()
+    osys = oracle_system(oracle, s)
+    for w in range(0, W, 4):
+        wk = oracle.Walker(osys, pos[w])
+        e, kin, _ = wk.local_energy()
+        scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
+        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale, (w, st[w, vmc.STAT["LAST_E"]], e)

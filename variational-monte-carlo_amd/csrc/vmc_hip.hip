@@ -124,6 +124,7 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevSys s, DevState st, int n_sw
     int since = st.since[w];
     int wflags = st.flags[w];
     double ld0 = st.logdet[2 * w + 0], ld1 = st.logdet[2 * w + 1];
+    double drift = st.drift[w];
 
     for (int sw = 0; sw < n_sweeps; ++sw) {
         const uint32_t sweep = (uint32_t)(sweep0 + (unsigned long long)sw);
@@ -172,7 +173,9 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevSys s, DevState st, int n_sw
                 if (s.reinv_k > 0 && ++since >= s.reinv_k) {
                     since = 0;
                     for (int dd = 0; dd < s.ndet; ++dd) {
-                        const double l = reinvert<Real>(s, dd, v, lane);
+                        double dr = 0.0;
+                        const double l = reinvert<Real>(s, dd, v, lane, &dr);
+                        drift = (dr > drift || dr != dr) ? dr : drift;
                         if (!(l > -INFINITY)) wflags |= 1;
                         else if (dd == 0) ld0 = l;
                         else ld1 = l;
@@ -228,6 +231,7 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevSys s, DevState st, int n_sw
         st.flags[w] = wflags;
         st.logdet[2 * w + 0] = ld0;
         st.logdet[2 * w + 1] = ld1;
+        st.drift[w] = drift;
     }
 }
 

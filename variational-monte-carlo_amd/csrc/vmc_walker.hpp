@@ -100,8 +100,12 @@ __device__ inline Real orbital(const DevSys &s, int d, int j, double ux, double 
 // (D^T)^-1 = invT.  Returns log|det D| = sum log|pivot|, or -inf when a
 // pivot falls below the reference's 1e-12 tolerance (matrix.hpp:35-39).
 // Replaces log_abs_det's LU + n solves (slater_plane_wave.cpp:202-318).
+// With `drift` non-null, the largest |I - A invT|, the drift of the inverse
+// being replaced, is computed first (FP64, before the Gauss-Jordan) and
+// returned there.
 template <typename Real>
-__device__ double reinvert(const DevSys &s, int d, const WalkerView<Real> &v, int lane) {
+__device__ double reinvert(const DevSys &s, int d, const WalkerView<Real> &v, int lane,
+                           double *drift = nullptr) {
     const int n = s.n[d];
     const int first = s.first[d];
     double *A = v.A;
@@ -116,6 +120,23 @@ __device__ double reinvert(const DevSys &s, int d, const WalkerView<Real> &v, in
         A[e] = o.w ? sn : cs;
     }
     wave_sync();
+    if (drift) {
+        const Real *Y = v.inv + s.inv_off[d];
+        double dmax = 0.0;
+        for (int e = lane; e < n * n; e += kWave) {
+            const int m = e / n, mp = e - (e / n) * n;
+            double z = 0.0;
+            for (int i = 0; i < n; ++i) z += A[m * n + i] * (double)Y[i * n + mp];
+            const double r = fabs((m == mp ? 1.0 : 0.0) - z);
+            dmax = (r > dmax || r != r) ? r : dmax;
+        }
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            const double o = __shfl_xor(dmax, off, kWave);
+            dmax = (o > dmax || o != o) ? o : dmax;
+        }
+        *drift = wave_uniform(dmax);
+    }
     double logdet = 0.0;
     bool singular = false;
     for (int c = 0; c < n; ++c) {
