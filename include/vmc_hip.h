@@ -218,6 +218,109 @@ int vmc_frozen_eval(const vmc_system *sys, int32_t n_configs, const double *pos,
                     const int32_t *moved, const double *newpos, double *scalars, double *grad_lap,
                     double *inv_before, double *inv_after, double *s_k, void *stream);
 
+/* ------------------------------------------------------------------ batch
+ * Systems of uneven size in one launch.  The reference runs one system per
+ * process and one chain per thread (src/main.cpp:50-68); a batch runs many
+ * systems' walker ensembles -- each an independent vmc_ensemble with its own
+ * seed -- as one persistent kernel per sweep: workgroups take global walker
+ * indices from a device queue until all systems are swept.  A walker's chain
+ * depends only on its system, seed, walker index and sweep index, so every
+ * system gives the same result in a batch as alone with the generic kernel.
+ * All systems of a batch share one working precision.
+ *
+ * inv_placement[i] (NULL = all 0): 0 inverse in LDS when it fits, 1 LDS
+ * (error if it does not fit), 2 HBM. */
+typedef struct vmc_batch vmc_batch;
+int vmc_batch_create(const vmc_system *systems, const int64_t *n_walkers, const uint64_t *seeds,
+                     const double *step0, const int32_t *inv_placement, int32_t n_systems,
+                     vmc_batch **out);
+int vmc_batch_destroy(vmc_batch *b);
+int vmc_batch_init_positions(vmc_batch *b, void *stream);
+/* n_sweeps launches of the persistent batch kernel (flags as for
+ * vmc_ensemble_sweep, block_size per system in measured sweeps). */
+int vmc_batch_sweep(vmc_batch *b, int32_t n_sweeps, int32_t flags, int32_t block_size, void *stream);
+/* Captures n_sweeps batch sweeps into a HIP graph (on a stream the batch
+ * owns); vmc_batch_replay launches it on `stream`.  The sweep index lives in
+ * device memory, so each replay continues the chains. */
+int vmc_batch_capture(vmc_batch *b, int32_t n_sweeps, int32_t flags, int32_t block_size);
+int vmc_batch_replay(vmc_batch *b, void *stream);
+/* Borrowed handle of system i for the vmc_ensemble_read_* / get_positions /
+ * reset_stats calls; sweeping or destroying it directly is refused. */
+int vmc_batch_member(vmc_batch *b, int32_t system, vmc_ensemble **out);
+int vmc_batch_info(const vmc_batch *b, int32_t *n_systems, int64_t *total_walkers, int32_t *grid,
+                   int64_t *lds_bytes);
+
+/* ---------------------------------------------------- placement, footprint */
+/* Generic kernel's inverse placement: 0 LDS when it fits (default), 1 LDS
+ * (error if it does not fit), 2 HBM.  No reference counterpart (a tuning
+ * knob of the device layout). */
+int vmc_ensemble_set_inverse_placement(vmc_ensemble *e, int32_t placement);
+/* Device bytes an ensemble of this system and walker count holds (state and
+ * tables), the admission measure of the request server. */
+int vmc_ensemble_footprint(const vmc_system *sys, int64_t n_walkers, int64_t *bytes);
+
+/* --------------------------------------------------------------- autotuner
+ * Picks, for one system and walker count, the kernel family (register
+ * resident or generic), its build variant, the inverse placement (LDS or
+ * HBM) and -- in mixed precision -- the longest FP64 re-inversion interval K
+ * in {256, 64, 16} whose measured inverse drift stays below drift_tol, by
+ * timing `sweeps` measured sweeps of each candidate with HIP events.  No
+ * reference counterpart (the reference has no device layout to tune). */
+typedef struct vmc_tuning {
+    int32_t kernel;            /* VMC_KERNEL_GENERIC or VMC_KERNEL_REGISTER */
+    int32_t rr_variant;        /* register kernel build variant */
+    int32_t inv_placement;     /* generic kernel: 1 LDS, 2 HBM */
+    int32_t reinvert_interval; /* chosen K (mixed); the system's own value in FP64 */
+    double moves_per_s;        /* electron-moves/s of the chosen settings */
+    double drift;              /* largest inverse drift met at the chosen K */
+} vmc_tuning;
+int vmc_autotune(const vmc_system *sys, int64_t n_walkers, double drift_tol, int32_t sweeps,
+                 vmc_tuning *out, void *stream);
+
+/* ------------------------------------------------------------------ server
+ * Continuous batching of energy evaluations (the runs the reference's
+ * Jastrow optimiser issues, src/optimizer/jastrow_optimizer.cpp:21-171, and
+ * main's chains, src/main.cpp:50-68).  A worker thread admits requests in
+ * arrival order while their device footprint fits hbm_budget_bytes and at
+ * most max_active run; each admitted request owns a stream and advances
+ * sweeps_per_turn sweeps per scheduler turn (at most two turns in flight),
+ * so requests that arrive while others run start at once and their kernels
+ * overlap.  Walker state stays on the device from admission to result.  A
+ * request's result depends only on the request: the same alone and in any
+ * mix. */
+typedef struct vmc_request {
+    vmc_system system;
+    int64_t n_walkers;
+    int32_t warmup_sweeps;    /* adaptive (simulation.cpp:159-196) */
+    int32_t measure_sweeps;   /* measured, E_L per sweep */
+    int32_t block_size;       /* measured sweeps per blocking block */
+    uint64_t seed;
+    double step0;
+} vmc_request;
+
+typedef struct vmc_result {
+    int64_t id;
+    int32_t status;           /* VMC_OK or the error that ended the request */
+    int64_t walkers;
+    double mean_energy;       /* walker aggregate as src/main.cpp:79-104 */
+    double standard_error;    /* < 0 when no walker completed two blocks */
+    double acceptance_rate;
+    double moves;             /* electron moves performed */
+    double queued_s;          /* submit -> admission */
+    double latency_s;         /* submit -> result */
+} vmc_result;
+
+typedef struct vmc_server vmc_server;
+int vmc_server_create(int64_t hbm_budget_bytes, int32_t max_active, int32_t sweeps_per_turn,
+                      vmc_server **out);
+/* Thread-safe.  VMC_ERR_MEMORY when the request alone exceeds the budget. */
+int vmc_server_submit(vmc_server *s, const vmc_request *req, int64_t *id);
+/* Up to `capacity` finished results; with wait != 0 blocks until one is
+ * ready or nothing is in flight. */
+int vmc_server_poll(vmc_server *s, vmc_result *out, int32_t capacity, int32_t *n_out, int32_t wait);
+/* Finishes every submitted request, then stops the worker. */
+int vmc_server_destroy(vmc_server *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,96 @@
+"""Host services over the ensemble API on the GPU: the autotuner and the
+continuous-batching request server."""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_picks_a_working_setting(vmc):
+    s = vmc.System.heg(54, 2.0, 1.0, precision=vmc.PREC_MIXED)
+    t = vmc.autotune(s, 256, drift_tol=1e-2, sweeps=4)
+    print("tuning", t, flush=True)
+    assert t["kernel"] in (vmc.KERNEL_GENERIC, vmc.KERNEL_REGISTER)
+    assert t["reinvert_interval"] in (16, 64, 256) and t["moves_per_s"] > 0
+    # The N = 54 system has a register-resident instantiation, which is faster.
+    assert t["kernel"] == vmc.KERNEL_REGISTER
+    s2 = vmc.System.heg(54, 2.0, 1.0, precision=vmc.PREC_MIXED, reinvert_interval=t["reinvert_interval"])
+    e = vmc.Ensemble(s2, 64, 3)
+    e.set_kernel(t["kernel"])
+    e.set_rr_variant(t["rr_variant"])
+    e.init_positions()
+    e.sweep(3, measure=True)
+    assert np.all(np.isfinite(e.stats()[:, vmc.STAT["LAST_E"]]))
+    e.close()
+    # A system without a register kernel (N = 66, 33 per spin) tunes the
+    # generic kernel's inverse placement.
+    t = vmc.autotune(vmc.System.heg(66, 2.0, 1.0, precision=vmc.PREC_MIXED), 64, drift_tol=1e-2, sweeps=2)
+    assert t["kernel"] == vmc.KERNEL_GENERIC and t["inv_placement"] in (1, 2)
+
+
+REQUESTS = [  # (N, r_s, b, walkers, warmup, measure, seed)
+    (14, 1.0, 1.0, 128, 6, 20, 11),
+    (38, 3.0, 1.5, 64, 4, 12, 12),
+    (54, 2.0, 1.0, 256, 5, 16, 13),
+    (66, 2.0, 0.8, 64, 3, 10, 14),
+    (14, 5.0, 0.6, 96, 6, 14, 15),
+    (54, 8.0, 1.9, 128, 4, 18, 16),
+]
+
+
+def submit(vmc, srv, spec, precision):
+    N, rs, b, W, warm, meas, seed = spec
+    s = vmc.System.heg(N, rs, b, precision=precision, reinvert_interval=64 if precision else 0)
+    return srv.submit(s, W, warm, meas, seed, step0=0.3 * rs, block_size=2)
+
+
+def collect(srv, n):
+    out = {}
+    while len(out) < n:
+        for r in srv.poll(wait=True):
+            out[r["id"]] = r
+    return out
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_server_results_same_alone_and_in_the_mix(vmc, precision):
+    alone = []
+    for spec in REQUESTS:
+        srv = vmc.Server(1 << 34, max_active=1)
+        i = submit(vmc, srv, spec, precision)
+        alone.append(collect(srv, 1)[i])
+        srv.close()
+    # The mix: arrivals spread over time, at most three running at once, so
+    # requests are admitted while others are on the device.
+    rng = np.random.default_rng(7)
+    srv = vmc.Server(1 << 34, max_active=3, sweeps_per_turn=2)
+    ids = []
+    for spec in REQUESTS:
+        ids.append(submit(vmc, srv, spec, precision))
+        time.sleep(float(rng.exponential(0.002)))
+    mix = collect(srv, len(REQUESTS))
+    srv.close()
+    for a, i in zip(alone, ids):
+        m = mix[i]
+        assert a["status"] == 0 and m["status"] == 0
+        for k in ("mean_energy", "standard_error", "acceptance_rate", "moves"):
+            assert a[k] == m[k], (k, a[k], m[k])
+    lat = sorted(r["latency_s"] for r in mix.values())
+    print(f"latencies p50 {lat[len(lat) // 2] * 1e3:.1f} ms max {lat[-1] * 1e3:.1f} ms", flush=True)
+
+
+def test_server_admission_by_footprint(vmc):
+    s = vmc.System.heg(54, 2.0, 1.0, precision=vmc.PREC_MIXED, reinvert_interval=64)
+    one = vmc.footprint(s, 256)
+    srv = vmc.Server(int(one * 1.5), max_active=8)  # room for one such request at a time
+    ids = [srv.submit(s, 256, 2, 6, seed) for seed in (1, 2, 3)]
+    res = collect(srv, 3)
+    assert all(res[i]["status"] == 0 for i in ids)
+    # The later requests waited for the earlier ones' memory.
+    q = [res[i]["queued_s"] for i in ids]
+    assert q[2] > q[0]
+    with pytest.raises(vmc.VmcError):
+        srv.submit(s, 1024, 2, 6, 9)  # larger than the whole budget
+    srv.close()

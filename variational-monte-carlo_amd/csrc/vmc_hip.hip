@@ -94,12 +94,10 @@ __global__ __launch_bounds__(kWave) void k_init(DevSys s, DevState st, int draw)
 // {p, walker, sweep}), ratio and Jastrow change, Metropolis test with a
 // Philox uniform (tag ACCP), Sherman-Morrison on accept; after each sweep the
 // optional step adaptation and local-energy measurement.
-template <typename Real, bool kInvLds>
-__global__ __launch_bounds__(kWave) void k_sweep(DevSys s, DevState st, int n_sweeps,
-                                                 unsigned long long sweep0, int flags,
-                                                 int block_size) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int w = blockIdx.x;
+template <typename Real>
+__device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kInvLds,
+                             int n_sweeps, unsigned long long sweep0, int flags, int block_size,
+                             unsigned char *smem) {
     const int lane = threadIdx.x;
     const int N = s.N;
     Real *inv_hbm = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
@@ -232,6 +230,63 @@ __global__ __launch_bounds__(kWave) void k_sweep(DevSys s, DevState st, int n_sw
         st.logdet[2 * w + 0] = ld0;
         st.logdet[2 * w + 1] = ld1;
         st.drift[w] = drift;
+    }
+    wave_sync();
+}
+
+template <typename Real, bool kInvLds>
+__global__ __launch_bounds__(kWave) void k_sweep(DevSys s, DevState st, int n_sweeps,
+                                                 unsigned long long sweep0, int flags,
+                                                 int block_size) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    sweep_walker<Real>(s, st, blockIdx.x, kInvLds, n_sweeps, sweep0, flags, block_size, smem);
+}
+
+// One system of a batch: its tables and state, whether its inverses live in
+// LDS, and where its walkers start in the batch's global walker numbering.
+struct BatchItem {
+    DevSys s;
+    DevState st;
+    int inv_lds;
+    int walker_begin;
+    int walker_end;
+};
+
+// One sweep of every walker of every system of a batch in one persistent
+// launch: each workgroup takes global walker indices from a device queue
+// until the batch is exhausted, so systems of uneven size and cost share the
+// machine.  The sweep index (the Philox counter) and the queue live in
+// device memory and k_batch_advance steps them, so a captured graph of
+// sweeps replays with fresh random numbers.  Walkers use their
+// system-local index and their system's seed: a system's chain is the same
+// alone and in a batch.
+template <typename Real>
+__global__ __launch_bounds__(kWave) void k_sweep_batch(const BatchItem *items, int n_items,
+                                                       int total, unsigned int *queue,
+                                                       const unsigned long long *sweep_ctr,
+                                                       int flags, int block_size) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int next;
+    const unsigned long long sweep0 = *sweep_ctr;
+    for (;;) {
+        if (threadIdx.x == 0) next = (int)atomicAdd(queue, 1u);
+        __syncthreads();
+        const int g = next;
+        __syncthreads();
+        if (g >= total) break;
+        int i = 0;
+        while (i + 1 < n_items && g >= items[i].walker_end) ++i;
+        const BatchItem &it = items[i];
+        sweep_walker<Real>(it.s, it.st, g - it.walker_begin, it.inv_lds != 0, 1, sweep0, flags,
+                           block_size, smem);
+    }
+}
+
+// After a batch sweep: next sweep index, queue back to zero.
+__global__ void k_batch_advance(unsigned int *queue, unsigned long long *sweep_ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *queue = 0u;
+        *sweep_ctr += 1ull;
     }
 }
 
@@ -489,6 +544,8 @@ struct vmc_ensemble {
     int kernel = VMC_KERNEL_AUTO;
     int rr_variant = 0;
     KernelTimer timer;
+    int inv_placement = 0;  // generic kernel: 0 LDS when it fits, 1 LDS, 2 HBM
+    bool in_batch = false;  // owned by a vmc_batch: swept and destroyed through it
 
     ~vmc_ensemble() {
         if (st.pos) (void)hipFree(st.pos);
@@ -534,7 +591,7 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
     const unsigned grid = static_cast<unsigned>(e->W);
     hipEvent_t ev = nullptr;
     const bool timed = e->timer.begin(stream, ev);
-    if (lds_in <= kMaxLdsBytes) {
+    if (lds_in <= kMaxLdsBytes && e->inv_placement != 2) {
         hipLaunchKernelGGL((k_sweep<Real, true>), dim3(grid), dim3(kWave), lds_in, stream, d, e->st,
                            n_sweeps, (unsigned long long)e->sweeps_done, flags, block_size);
     } else {
@@ -580,6 +637,22 @@ int vmc_system_sizes(const vmc_system *sys, int32_t *n_k, int32_t *n_g, int32_t 
     return VMC_OK;
 }
 
+int vmc_ensemble_footprint(const vmc_system *sys, int64_t n_walkers, int64_t *bytes) {
+    if (!bytes) return fail(VMC_ERR_INVALID, "bytes is NULL");
+    int32_t nk[2], ng = 0, ninv = 0;
+    int rc = vmc_system_sizes(sys, nk, &ng, &ninv);
+    if (rc) return rc;
+    if (n_walkers < 1) return fail(VMC_ERR_INVALID, "n_walkers must be >= 1");
+    const int64_t N = sys->n_up + sys->n_dn;
+    const int64_t nmax = std::max(sys->n_up, sys->n_dn);
+    const int64_t rb = sys->precision == VMC_PREC_MIXED ? 4 : 8;
+    const int64_t per_walker = 3 * N * 8 + ninv * rb + nmax * nmax * 8 + kStatCount * 8 +
+                               8 /* sigma */ + 16 /* logdet */ + 8 /* since, flags */ + 8 /* drift */;
+    const int64_t tables = 2 * nmax * 16 + static_cast<int64_t>(ng) * (16 + 8);
+    *bytes = n_walkers * per_walker + tables;
+    return VMC_OK;
+}
+
 int vmc_ensemble_create(const vmc_system *sys, int64_t n_walkers, uint64_t seed, double step0,
                         vmc_ensemble **out) {
     if (!out) return fail(VMC_ERR_INVALID, "out is NULL");
@@ -622,6 +695,7 @@ int vmc_ensemble_create(const vmc_system *sys, int64_t n_walkers, uint64_t seed,
 }
 
 int vmc_ensemble_destroy(vmc_ensemble *e) {
+    if (e && e->in_batch) return fail(VMC_ERR_STATE, "ensemble belongs to a batch");
     delete e;
     return VMC_OK;
 }
@@ -682,6 +756,7 @@ int vmc_ensemble_sweep(vmc_ensemble *e, int32_t n_sweeps, int32_t flags, int32_t
                        void *stream) {
     if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
     if (!e->positions_ready) return fail(VMC_ERR_STATE, "positions not initialised");
+    if (e->in_batch) return fail(VMC_ERR_STATE, "ensemble belongs to a batch; sweep the batch");
     if (n_sweeps < 0) return fail(VMC_ERR_INVALID, "n_sweeps must be >= 0");
     if (block_size < 1) return fail(VMC_ERR_INVALID, "block_size must be >= 1");
     if (n_sweeps == 0) return VMC_OK;
@@ -707,6 +782,15 @@ int vmc_ensemble_set_rr_variant(vmc_ensemble *e, int32_t variant) {
     if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
     if (variant != 0 && variant != 1) return fail(VMC_ERR_INVALID, "unknown variant %d", variant);
     e->rr_variant = variant;
+    return VMC_OK;
+}
+
+int vmc_ensemble_set_inverse_placement(vmc_ensemble *e, int32_t placement) {
+    if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
+    if (placement < 0 || placement > 2) return fail(VMC_ERR_INVALID, "unknown placement %d", placement);
+    if (placement == 1 && lds_for(e->tables.dev, e->tables.real_bytes, true) > kMaxLdsBytes)
+        return fail(VMC_ERR_INVALID, "inverse does not fit in LDS");
+    e->inv_placement = placement;
     return VMC_OK;
 }
 
@@ -865,6 +949,205 @@ int vmc_frozen_eval(const vmc_system *sys, int32_t n_configs, const double *pos,
     if (inv_after) VMC_HIP_TRY(hipMemcpyAsync(inv_after, d_ia, sizeof(double) * C * d.ninv, hipMemcpyDeviceToHost, s));
     if (s_k) VMC_HIP_TRY(hipMemcpyAsync(s_k, d_sk, sizeof(double) * C * 2 * d.ng, hipMemcpyDeviceToHost, s));
     VMC_HIP_TRY(hipStreamSynchronize(s));
+    return VMC_OK;
+}
+
+// ------------------------------------------------------------------ batch
+
+}  // extern "C"
+
+struct vmc_batch {
+    std::vector<vmc_ensemble *> members;
+    BatchItem *items = nullptr;
+    unsigned int *queue = nullptr;
+    unsigned long long *ctr = nullptr;
+    int total = 0;
+    int grid = 0;
+    size_t lds = 0;
+    int real_bytes = 8;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_sweeps = 0;
+
+    ~vmc_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        if (items) (void)hipFree(items);
+        if (queue) (void)hipFree(queue);
+        if (ctr) (void)hipFree(ctr);
+        for (auto *m : members) delete m;
+    }
+};
+
+namespace {
+
+template <typename Real>
+int batch_launch(vmc_batch *b, int n_sweeps, int flags, int block_size, hipStream_t stream) {
+    for (int i = 0; i < n_sweeps; ++i) {
+        hipLaunchKernelGGL((k_sweep_batch<Real>), dim3(static_cast<unsigned>(b->grid)), dim3(kWave),
+                           b->lds, stream, b->items, static_cast<int>(b->members.size()), b->total,
+                           b->queue, b->ctr, flags, block_size);
+        hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1), 0, stream, b->queue, b->ctr);
+    }
+    VMC_HIP_TRY(hipGetLastError());
+    return VMC_OK;
+}
+
+int batch_sweep_any(vmc_batch *b, int n_sweeps, int flags, int block_size, hipStream_t stream) {
+    return b->real_bytes == 8 ? batch_launch<double>(b, n_sweeps, flags, block_size, stream)
+                              : batch_launch<float>(b, n_sweeps, flags, block_size, stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int vmc_batch_create(const vmc_system *systems, const int64_t *n_walkers, const uint64_t *seeds,
+                     const double *step0, const int32_t *inv_placement, int32_t n_systems,
+                     vmc_batch **out) {
+    if (!out || !systems || !n_walkers || !seeds || !step0) return fail(VMC_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    if (n_systems < 1 || n_systems > 4096) return fail(VMC_ERR_INVALID, "n_systems out of range");
+    for (int i = 1; i < n_systems; ++i)
+        if ((systems[i].precision != VMC_PREC_FP64) != (systems[0].precision != VMC_PREC_FP64))
+            return fail(VMC_ERR_INVALID, "a batch runs one working precision");
+    auto *b = new vmc_batch();
+    std::vector<BatchItem> items(static_cast<size_t>(n_systems));
+    int64_t total = 0;
+    for (int i = 0; i < n_systems; ++i) {
+        vmc_ensemble *e = nullptr;
+        int rc = vmc_ensemble_create(&systems[i], n_walkers[i], seeds[i], step0[i], &e);
+        if (rc) {
+            delete b;
+            return rc;
+        }
+        e->in_batch = true;
+        b->members.push_back(e);
+        const DevSys &d = e->tables.dev;
+        b->real_bytes = e->tables.real_bytes;
+        const int place = inv_placement ? inv_placement[i] : 0;  // 0 auto, 1 LDS, 2 HBM
+        const bool fits = lds_for(d, e->tables.real_bytes, true) <= kMaxLdsBytes;
+        if (place == 1 && !fits) {
+            delete b;
+            return fail(VMC_ERR_INVALID, "system %d: inverse does not fit in LDS", i);
+        }
+        const bool inv_lds = place == 2 ? false : fits;
+        items[i].s = d;
+        items[i].st = e->st;
+        items[i].inv_lds = inv_lds ? 1 : 0;
+        items[i].walker_begin = static_cast<int>(total);
+        total += n_walkers[i];
+        items[i].walker_end = static_cast<int>(total);
+        b->lds = std::max(b->lds, lds_for(d, e->tables.real_bytes, inv_lds));
+    }
+    if (total > (int64_t(1) << 30)) {
+        delete b;
+        return fail(VMC_ERR_INVALID, "too many walkers in one batch");
+    }
+    b->total = static_cast<int>(total);
+    int rc;
+    if ((rc = dev_alloc(&b->items, items.size())) || (rc = dev_alloc(&b->queue, 1)) ||
+        (rc = dev_alloc(&b->ctr, 1))) {
+        delete b;
+        return rc;
+    }
+    if (hipMemcpy(b->items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(b->queue, 0, sizeof(unsigned int)) != hipSuccess ||
+        hipMemset(b->ctr, 0, sizeof(unsigned long long)) != hipSuccess) {
+        delete b;
+        return fail(VMC_ERR_DEVICE, "batch initialisation copy failed");
+    }
+    // Persistent grid: as many workgroups as fit on the device at once.
+    int dev = 0, cus = 0, per_cu = 0;
+    VMC_HIP_TRY(hipGetDevice(&dev));
+    VMC_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    hipError_t oc = b->real_bytes == 8
+                        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_batch<double>, kWave, b->lds)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_batch<float>, kWave, b->lds);
+    if (oc != hipSuccess || per_cu < 1) per_cu = 1;
+    b->grid = static_cast<int>(std::min<int64_t>(total, static_cast<int64_t>(cus) * per_cu));
+    *out = b;
+    return VMC_OK;
+}
+
+int vmc_batch_destroy(vmc_batch *b) {
+    delete b;
+    return VMC_OK;
+}
+
+int vmc_batch_init_positions(vmc_batch *b, void *stream) {
+    if (!b) return fail(VMC_ERR_INVALID, "batch is NULL");
+    for (size_t i = 0; i < b->members.size(); ++i) {
+        const int rc = vmc_ensemble_init_positions(b->members[i], stream);
+        if (rc) return rc;
+    }
+    return VMC_OK;
+}
+
+int vmc_batch_sweep(vmc_batch *b, int32_t n_sweeps, int32_t flags, int32_t block_size, void *stream) {
+    if (!b) return fail(VMC_ERR_INVALID, "batch is NULL");
+    if (n_sweeps < 0 || block_size < 1) return fail(VMC_ERR_INVALID, "bad sweep arguments");
+    for (auto *m : b->members)
+        if (!m->positions_ready) return fail(VMC_ERR_STATE, "positions not initialised");
+    const int rc = batch_sweep_any(b, n_sweeps, flags, block_size, static_cast<hipStream_t>(stream));
+    if (rc) return rc;
+    for (auto *m : b->members) m->sweeps_done += static_cast<uint64_t>(n_sweeps);
+    return VMC_OK;
+}
+
+int vmc_batch_capture(vmc_batch *b, int32_t n_sweeps, int32_t flags, int32_t block_size) {
+    if (!b) return fail(VMC_ERR_INVALID, "batch is NULL");
+    if (n_sweeps < 1 || block_size < 1) return fail(VMC_ERR_INVALID, "bad capture arguments");
+    if (b->exec) {
+        VMC_HIP_TRY(hipGraphExecDestroy(b->exec));
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        VMC_HIP_TRY(hipGraphDestroy(b->graph));
+        b->graph = nullptr;
+    }
+    if (!b->cap_stream) VMC_HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+    VMC_HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_sweep_any(b, n_sweeps, flags, block_size, b->cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(b->cap_stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    VMC_HIP_TRY(ec);
+    b->graph = g;
+    VMC_HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_sweeps = n_sweeps;
+    return VMC_OK;
+}
+
+int vmc_batch_replay(vmc_batch *b, void *stream) {
+    if (!b || !b->exec) return fail(VMC_ERR_STATE, "no captured batch graph");
+    for (auto *m : b->members)
+        if (!m->positions_ready) return fail(VMC_ERR_STATE, "positions not initialised");
+    VMC_HIP_TRY(hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream)));
+    for (auto *m : b->members) m->sweeps_done += static_cast<uint64_t>(b->graph_sweeps);
+    return VMC_OK;
+}
+
+int vmc_batch_member(vmc_batch *b, int32_t system, vmc_ensemble **out) {
+    if (!b || !out) return fail(VMC_ERR_INVALID, "NULL argument");
+    if (system < 0 || system >= static_cast<int32_t>(b->members.size()))
+        return fail(VMC_ERR_INVALID, "system %d out of range", system);
+    *out = b->members[static_cast<size_t>(system)];
+    return VMC_OK;
+}
+
+int vmc_batch_info(const vmc_batch *b, int32_t *n_systems, int64_t *total_walkers, int32_t *grid,
+                   int64_t *lds_bytes) {
+    if (!b) return fail(VMC_ERR_INVALID, "batch is NULL");
+    if (n_systems) *n_systems = static_cast<int32_t>(b->members.size());
+    if (total_walkers) *total_walkers = b->total;
+    if (grid) *grid = b->grid;
+    if (lds_bytes) *lds_bytes = static_cast<int64_t>(b->lds);
     return VMC_OK;
 }
 

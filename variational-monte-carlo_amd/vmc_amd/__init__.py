@@ -65,6 +65,27 @@ class SystemDesc(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("kernel", C.c_int32), ("rr_variant", C.c_int32), ("inv_placement", C.c_int32),
+                ("reinvert_interval", C.c_int32), ("moves_per_s", C.c_double), ("drift", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("system", SystemDesc), ("n_walkers", C.c_int64), ("warmup_sweeps", C.c_int32),
+                ("measure_sweeps", C.c_int32), ("block_size", C.c_int32), ("seed", C.c_uint64),
+                ("step0", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("walkers", C.c_int64),
+                ("mean_energy", C.c_double), ("standard_error", C.c_double),
+                ("acceptance_rate", C.c_double), ("moves", C.c_double), ("queued_s", C.c_double),
+                ("latency_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def box_length_from_rs(rs, n):
     return float(np.cbrt(4.0 * np.pi * n / 3.0) * rs)
 
@@ -137,6 +158,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with "
                                f"`make -C {PKG_DIR}` (hipcc, gfx950)")
+        # One HIP runtime per process: when PyTorch is present its runtime is
+        # loaded first, and the library binds to it (same soname), so torch
+        # streams and events and this library share devices and queues.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         sp = C.POINTER(SystemDesc)
         sig = {
@@ -157,6 +185,23 @@ def lib():
             "vmc_ensemble_read_stats": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_read_steps": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_read_drift": (C.c_int, [_VP, _dp, _VP]),
+            "vmc_ensemble_set_inverse_placement": (C.c_int, [_VP, C.c_int32]),
+            "vmc_ensemble_footprint": (C.c_int, [sp, C.c_int64, C.POINTER(C.c_int64)]),
+            "vmc_autotune": (C.c_int, [sp, C.c_int64, C.c_double, C.c_int32, C.POINTER(Tuning), _VP]),
+            "vmc_server_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(_VP)]),
+            "vmc_server_submit": (C.c_int, [_VP, C.POINTER(Request), C.POINTER(C.c_int64)]),
+            "vmc_server_poll": (C.c_int, [_VP, C.POINTER(Result), C.c_int32, _ip, C.c_int32]),
+            "vmc_server_destroy": (C.c_int, [_VP]),
+            "vmc_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
+                                           _dp, _ip, C.c_int32, C.POINTER(_VP)]),
+            "vmc_batch_destroy": (C.c_int, [_VP]),
+            "vmc_batch_init_positions": (C.c_int, [_VP, _VP]),
+            "vmc_batch_sweep": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _VP]),
+            "vmc_batch_capture": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32]),
+            "vmc_batch_replay": (C.c_int, [_VP, _VP]),
+            "vmc_batch_member": (C.c_int, [_VP, C.c_int32, C.POINTER(_VP)]),
+            "vmc_batch_info": (C.c_int, [_VP, _ip, C.POINTER(C.c_int64), _ip,
+                                         C.POINTER(C.c_int64)]),
             "vmc_ensemble_set_timing": (C.c_int, [_VP, C.c_int32]),
             "vmc_ensemble_read_timing": (C.c_int, [_VP, _dp, C.POINTER(C.c_int64)]),
             "vmc_ensemble_info": (C.c_int, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
@@ -193,6 +238,16 @@ def system_sizes(system: System):
 class Ensemble:
     """Device-resident walker ensemble (vmc_ensemble)."""
 
+    @classmethod
+    def _borrowed(cls, handle, system, n_walkers, seed, stream):
+        """View of a batch member: reads only, the batch owns and sweeps it."""
+        self = cls.__new__(cls)
+        self.system, self.W, self.N, self.seed, self.stream = system, int(n_walkers), system.N, seed, stream
+        self._desc = system.desc()
+        self._h = handle
+        self._owned = False
+        return self
+
     def __init__(self, system: System, n_walkers: int, seed: int, step0: float | None = None,
                  stream=None):
         self.system = system
@@ -205,11 +260,12 @@ class Ensemble:
         step = system.box_length / 10.0 if step0 is None else step0
         _check(lib().vmc_ensemble_create(C.byref(self._desc), self.W, seed, step,
                                          C.byref(self._h)))
+        self._owned = True
 
     def close(self):
-        if self._h:
+        if self._h and self._owned:
             lib().vmc_ensemble_destroy(self._h)
-            self._h = _VP()
+        self._h = _VP()
 
     def __del__(self):
         try:
@@ -335,3 +391,112 @@ def blocking_summary(stats):
     se_w = np.sqrt(var / nb[ok])
     se = float(math.sqrt(np.sum(se_w ** 2)) / stats.shape[0])
     return mean, se
+
+
+class Batch:
+    """Systems of uneven size swept by one persistent launch (vmc_batch)."""
+
+    def __init__(self, systems, walkers, seeds, step0=None, placement=None, stream=None):
+        n = len(systems)
+        self.systems = list(systems)
+        self.walkers = [int(w) for w in walkers]
+        self.seeds = [int(x) for x in seeds]
+        self.stream = stream
+        self._descs = (SystemDesc * n)(*[sy.desc() for sy in systems])
+        w = (C.c_int64 * n)(*self.walkers)
+        sd = (C.c_uint64 * n)(*self.seeds)
+        st = np.ascontiguousarray([sy.box_length / 10.0 for sy in systems] if step0 is None else step0,
+                                  dtype=np.float64)
+        pl = None if placement is None else np.ascontiguousarray(placement, dtype=np.int32)
+        self._h = _VP()
+        _check(lib().vmc_batch_create(C.cast(self._descs, C.c_void_p), w, sd, _ptr(st),
+                                      None if pl is None else _ptr(pl, _ip), n, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().vmc_batch_destroy(self._h)
+            self._h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_positions(self):
+        _check(lib().vmc_batch_init_positions(self._h, self.stream))
+
+    def sweep(self, n_sweeps=1, measure=False, adapt=False, block_size=1):
+        flags = (SWEEP_MEASURE if measure else 0) | (SWEEP_ADAPT if adapt else 0)
+        _check(lib().vmc_batch_sweep(self._h, n_sweeps, flags, block_size, self.stream))
+
+    def capture(self, n_sweeps, measure=False, adapt=False, block_size=1):
+        flags = (SWEEP_MEASURE if measure else 0) | (SWEEP_ADAPT if adapt else 0)
+        _check(lib().vmc_batch_capture(self._h, n_sweeps, flags, block_size))
+
+    def replay(self):
+        _check(lib().vmc_batch_replay(self._h, self.stream))
+
+    def member(self, i):
+        h = _VP()
+        _check(lib().vmc_batch_member(self._h, i, C.byref(h)))
+        return Ensemble._borrowed(h, self.systems[i], self.walkers[i], self.seeds[i], self.stream)
+
+    def info(self):
+        ns, gr = C.c_int32(), C.c_int32()
+        tw, lds = C.c_int64(), C.c_int64()
+        _check(lib().vmc_batch_info(self._h, C.byref(ns), C.byref(tw), C.byref(gr), C.byref(lds)))
+        return dict(n_systems=ns.value, total_walkers=tw.value, grid=gr.value, lds_bytes=lds.value)
+
+
+def footprint(system: System, n_walkers: int) -> int:
+    b = C.c_int64()
+    d = system.desc()
+    _check(lib().vmc_ensemble_footprint(C.byref(d), int(n_walkers), C.byref(b)))
+    return b.value
+
+
+def autotune(system: System, n_walkers: int, drift_tol=1e-3, sweeps=10, stream=None):
+    """Kernel family, variant, inverse placement and (mixed precision) the
+    re-inversion interval for this system, chosen by timing on the device."""
+    t = Tuning()
+    d = system.desc()
+    _check(lib().vmc_autotune(C.byref(d), int(n_walkers), float(drift_tol), int(sweeps), C.byref(t),
+                              stream))
+    return dict(kernel=t.kernel, rr_variant=t.rr_variant, inv_placement=t.inv_placement,
+                reinvert_interval=t.reinvert_interval, moves_per_s=t.moves_per_s, drift=t.drift)
+
+
+class Server:
+    """Continuous-batching request server (vmc_server)."""
+
+    def __init__(self, hbm_budget_bytes, max_active=64, sweeps_per_turn=1):
+        self._h = _VP()
+        _check(lib().vmc_server_create(int(hbm_budget_bytes), int(max_active), int(sweeps_per_turn),
+                                       C.byref(self._h)))
+
+    def submit(self, system: System, n_walkers, warmup_sweeps, measure_sweeps, seed, step0=None,
+               block_size=10):
+        r = Request(system.desc(), int(n_walkers), int(warmup_sweeps), int(measure_sweeps),
+                    int(block_size), int(seed),
+                    float(system.box_length / 10.0 if step0 is None else step0))
+        i = C.c_int64()
+        _check(lib().vmc_server_submit(self._h, C.byref(r), C.byref(i)))
+        return i.value
+
+    def poll(self, wait=False, capacity=256):
+        buf = (Result * capacity)()
+        n = C.c_int32()
+        _check(lib().vmc_server_poll(self._h, buf, capacity, C.byref(n), 1 if wait else 0))
+        return [buf[i].as_dict() for i in range(n.value)]
+
+    def close(self):
+        if self._h:
+            _check(lib().vmc_server_destroy(self._h))
+            self._h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
