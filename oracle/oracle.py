@@ -372,6 +372,8 @@ def ref():
         R.ref_blocking.argtypes = [C.c_size_t, _dp, C.c_size_t, _dp, _dp]
         R.ref_write_fixed.argtypes = [C.c_int, C.c_char_p, _ip]
         R.ref_config.argtypes = [C.c_char_p, _dp]
+        R.ref_optimizer.argtypes = [C.c_size_t, C.c_double, C.c_double, C.c_uint64, C.c_size_t,
+                                    _dp, _dp, _dp]
         _ref = R
     return _ref
 
@@ -501,3 +503,13 @@ def ref_blocking(block_size, x):
     if ref().ref_blocking(block_size, _ptr(x), x.size, C.byref(m), C.byref(se)):
         return None
     return m.value, se.value
+
+
+def ref_optimizer(n, L, a, seed, threads):
+    """The reference JastrowOptimizer on a small system: (b grid, energies,
+    the b optimize() returns)."""
+    b = np.zeros(2 * threads)
+    e = np.zeros(2 * threads)
+    opt = C.c_double()
+    ref().ref_optimizer(n, L, a, seed, threads, _ptr(b), _ptr(e), C.byref(opt))
+    return b, e, opt.value

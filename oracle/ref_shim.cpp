@@ -47,6 +47,7 @@
 #include "wavefunction/wavefunction.hpp"
 #include "blocking_analysis/blocking_analysis.hpp"
 #include "config/config.hpp"
+#include "optimizer/jastrow_optimizer.hpp"
 #undef private
 
 #include <cstdint>
@@ -335,6 +336,31 @@ int ref_config(const char* path, double* out) {
     } catch (const std::invalid_argument&) {
         return 1;
     }
+    return 0;
+}
+
+// JastrowOptimizer (jastrow_optimizer.cpp:21-171) on a small configuration:
+// the grid's b values and energies (its own evaluate(), deterministic for the
+// seed) and the b that optimize() returns.  b_out / e_out hold 2 * threads.
+int ref_optimizer(std::size_t n, double L, double a, uint64_t seed, std::size_t threads,
+                  double* b_out, double* e_out, double* optimal_b) {
+    Config cfg{};
+    cfg.num_particles = n;
+    cfg.box_length = L;
+    cfg.jastrow_a = a;
+    cfg.master_seed = seed;
+    cfg.num_threads = threads;
+    const double rs{JastrowOptimizer::compute_rs(cfg)};
+    const double b_min{1.0 / rs};
+    const double b_max{std::max(5.0, b_min + 0.5)};
+    const std::size_t pts{2U * threads};
+    const std::size_t sweeps{std::max<std::size_t>(50U, 500U / std::max<std::size_t>(n / 7U, 1U))};
+    for (std::size_t i = 0; i < pts; ++i) {
+        const double b{b_min + (b_max - b_min) * static_cast<double>(i) / static_cast<double>(pts - 1)};
+        b_out[i] = b;
+        e_out[i] = JastrowOptimizer::evaluate(cfg, b, sweeps, sweeps).energy;
+    }
+    *optimal_b = JastrowOptimizer::optimize(cfg, false).optimal_b;
     return 0;
 }
 

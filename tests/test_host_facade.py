@@ -131,3 +131,35 @@ def test_blocking_matches_reference_closed_forms(oracle):
         assert (f["mean"], f["se"]) == (m, se)
     r = selftest("blocking", 5, 1, 2, 3)
     assert r.returncode == 3 and "runtime_error" in r.stdout
+
+
+@pytest.mark.parametrize("n,L,seed,threads", [(7, 4.0, 5, 2), (19, 12.0, 9, 3), (27, 8.0, 3, 4)])
+def test_optimizer_grid_and_choice_match_reference(oracle, tmp_path, n, L, seed, threads):
+    """The facade's JastrowOptimizer scans the reference's b grid with the
+    reference's sweep counts, and its selection rule (lowest energy unless a
+    boundary artifact) picks what the reference's optimize() picks from the
+    reference's own energies at those points."""
+    if not oracle.ref_available():
+        pytest.skip("reference checker library not built (oracle/_ref)")
+    b_ref, e_ref, opt_ref = oracle.ref_optimizer(n, L, 0.25, seed, threads)
+    cfg = tmp_path / "c.cfg"
+    cfg.write_text(f"Num_Threads = {threads}\nNum_Particles = {n}\nBox_Length = {L}\nMaster_Seed = {seed}\n")
+    r = selftest("scan", cfg)
+    assert r.returncode == 0, r.stdout
+    grid = [float(x.split("=")[1]) for x in r.stdout.splitlines() if x.startswith("b=")]
+    np.testing.assert_allclose(grid, b_ref, rtol=1e-15, atol=0)
+    sweeps = [int(x.split("=")[1]) for x in r.stdout.splitlines() if x.startswith("sweeps=")][0]
+    assert sweeps == max(50, 500 // max(n // 7, 1))
+    args = [b_ref[0]] + [v for pair in zip(b_ref, e_ref) for v in pair]
+    r = selftest("select", *[repr(float(a)) for a in args])
+    chosen = float([x for x in r.stdout.splitlines() if x.startswith("b=")][0].split("=")[1])
+    assert chosen == opt_ref
+
+
+def test_optimizer_boundary_rule_cases():
+    """Boundary minimum far below a flat interior is an artifact; a boundary
+    minimum within twice the interior spread is kept."""
+    pick = lambda *a: float(selftest("select", *a).stdout.split("b=")[1].split()[0])
+    assert pick(0.5, 0.5, -1.0, 1.0, -0.5, 2.0, -0.51, 3.0, -0.49) == 2.0
+    assert pick(0.5, 0.5, -0.52, 1.0, -0.5, 2.0, -0.51, 3.0, -0.49) == 0.5
+    assert pick(0.5, 0.5, -0.4, 1.0, -0.5, 2.0, -0.51, 3.0, -0.49) == 2.0

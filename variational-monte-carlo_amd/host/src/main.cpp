@@ -1,9 +1,11 @@
 // main.cpp -- vmc_gpu: the reference's command-line run (src/main.cpp) on one
 // MI355X.  Reads config.cfg (or the path given), runs every walker of the
 // ensemble as an independent chain, streams walker 0 to output/vmc.bin and
-// prints the aggregated energy.  The reference's Jastrow b optimisation is
-// not run here: the b of the configuration is used as given.
+// prints the aggregated energy.  As in the reference, the Jastrow b is first
+// optimised (JastrowOptimizer, here a concurrent scan on the device);
+// --no-optimize keeps the configuration's b.
 #include "config/config.hpp"
+#include "optimizer/jastrow_optimizer.hpp"
 #include "output_writer/output_writer.hpp"
 #include "simulation/simulation.hpp"
 
@@ -13,11 +15,23 @@
 #include <iomanip>
 #include <iostream>
 #include <memory>
+#include <string>
 
 int main(int argc, char **argv) {
     try {
-        const Config config{Config::from_file(argc > 1 ? argv[1] : "config.cfg")};
-        std::cout << "\n<--- Variational Monte Carlo Simulation (MI355X) --->\n\n"
+        const char *path = "config.cfg";
+        bool optimize = true;
+        for (int i = 1; i < argc; ++i) {
+            if (std::string(argv[i]) == "--no-optimize") optimize = false;
+            else path = argv[i];
+        }
+        Config config{Config::from_file(path)};
+        std::cout << "\n<--- Variational Monte Carlo Simulation (MI355X) --->\n\n";
+        if (optimize) {
+            std::cout << "<--- Optimizing Jastrow b parameter --->\n";
+            config.jastrow_b = JastrowOptimizer::optimize(config, true).optimal_b;
+        }
+        std::cout
                   << "<--- Config Settings --->\n"
                   << "Number of walkers: " << config.walkers() << "\n"
                   << "Number of particles: " << config.num_particles << "\n"
@@ -27,7 +41,7 @@ int main(int argc, char **argv) {
                   << "Samples per block: " << config.block_size << "\n"
                   << "Master seed: " << config.master_seed << "\n"
                   << "Jastrow a: " << config.jastrow_a << "\n"
-                  << "Jastrow b: " << config.jastrow_b << "\n"
+                  << "Jastrow b: " << config.jastrow_b << (optimize ? " (optimized)" : "") << "\n"
                   << "Precision: " << (config.mixed_precision ? "mixed" : "fp64") << "\n"
                   << std::endl;
 

@@ -3,8 +3,11 @@
 //   vmc_host_selftest config <path>          parsed fields, or the exception
 //   vmc_host_selftest write <json|csv|bin> <out>   a fixed init/2 frames/done
 //   vmc_host_selftest blocking <bs> <x...>   mean and SE of the samples
+//   vmc_host_selftest scan <path>            the optimiser's b grid and sweeps
+//   vmc_host_selftest select <b_min> <b e>... the optimiser's choice
 #include "blocking_analysis/blocking_analysis.hpp"
 #include "config/config.hpp"
+#include "optimizer/jastrow_optimizer.hpp"
 #include "output_writer/output_writer.hpp"
 
 #include <cstdlib>
@@ -16,6 +19,7 @@
 #include <stdexcept>
 #include <string>
 #include <typeinfo>
+#include <vector>
 
 namespace {
 
@@ -71,6 +75,24 @@ int blocking_cmd(int argc, char **argv) {
     return 0;
 }
 
+int scan_cmd(const char *path) {
+    const Config c = Config::from_file(path);
+    std::cout << std::setprecision(std::numeric_limits<double>::max_digits10)
+              << "rs=" << JastrowOptimizer::compute_rs(c) << "\nsweeps=" << JastrowOptimizer::scan_sweeps(c) << "\n";
+    for (double b : JastrowOptimizer::grid(c)) std::cout << "b=" << b << "\n";
+    return 0;
+}
+
+int select_cmd(int argc, char **argv) {
+    std::vector<JastrowOptimizer::EvalResult> r;
+    for (int i = 3; i + 1 < argc; i += 2)
+        r.push_back({std::strtod(argv[i], nullptr), std::strtod(argv[i + 1], nullptr), 0.0});
+    const auto best = JastrowOptimizer::select(r, std::strtod(argv[2], nullptr));
+    std::cout << std::setprecision(std::numeric_limits<double>::max_digits10) << "b=" << best.b
+              << "\nenergy=" << best.energy << "\n";
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -78,6 +100,8 @@ int main(int argc, char **argv) {
         if (argc >= 3 && !std::strcmp(argv[1], "config")) return config_cmd(argv[2]);
         if (argc >= 4 && !std::strcmp(argv[1], "write")) return write_cmd(argv[2], argv[3]);
         if (argc >= 3 && !std::strcmp(argv[1], "blocking")) return blocking_cmd(argc, argv);
+        if (argc >= 3 && !std::strcmp(argv[1], "scan")) return scan_cmd(argv[2]);
+        if (argc >= 5 && !std::strcmp(argv[1], "select")) return select_cmd(argc, argv);
         std::cerr << "usage: vmc_host_selftest config <path> | write <fmt> <out> | blocking <bs> <x...>\n";
         return 2;
     } catch (const std::exception &e) {
