@@ -189,17 +189,85 @@ def reference_cpu(target_s):
                        f"{dt:.1f} s")
 
 
+def serving_run(V, precision, n_requests=40, scale=0.04, seed=5, rate_hz=8.0):
+    """Served request stream: Poisson arrivals of energy evaluations (N from
+    {14,38,54,66,114,162,246,294}, r_s log-uniform in [1,20], b uniform in
+    [0.5,2], walkers 128-4096 and sweeps 100-2000, both scaled by `scale` to
+    keep the run short) through the continuous-batching server; returns the
+    request latency percentiles and the aggregate electron-moves/s."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n_requests):
+        N = int(rng.choice([14, 38, 54, 66, 114, 162, 246, 294]))
+        rs = float(np.exp(rng.uniform(0.0, np.log(20.0))))
+        b = float(rng.uniform(0.5, 2.0))
+        W = max(16, int(rng.integers(128, 4097) * scale))
+        sw = max(10, int(rng.integers(100, 2001) * scale))
+        reqs.append((N, rs, b, W, sw, i + 1, float(rng.exponential(1.0 / rate_hz))))
+    srv = V.Server(64 << 30, max_active=16, sweeps_per_turn=2)
+    t0 = time.perf_counter()
+    for N, rs, b, W, sw, sd, gap in reqs:
+        s = V.System.heg(N, rs, b, precision=precision, reinvert_interval=256 if precision else 0)
+        srv.submit(s, W, sw // 4, sw, sd, step0=0.3 * rs, block_size=5)
+        time.sleep(gap)
+    res = []
+    while len(res) < len(reqs):
+        res += srv.poll(wait=True)
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = sorted(r["latency_s"] for r in res)
+    return dict(requests=len(res), failed=sum(r["status"] != 0 for r in res),
+                p50_ms=1e3 * pct(lat, 0.5), p99_ms=1e3 * pct(lat, 0.99),
+                moves_per_s=sum(r["moves"] for r in res) / wall, wall_s=wall,
+                offered_rate_hz=rate_hz)
+
+
+def batch_run(V, torch, sweeps=20, seed=2024):
+    """Uneven batch: 24 systems (N from {14,38,54,66,114,162}, r_s
+    log-uniform in [1,10], b uniform in [0.5,2], 64-1024 walkers), mixed
+    precision, one persistent launch per sweep replayed from a HIP graph."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    Ns = rng.choice([14, 38, 54, 66, 114, 162], size=24)
+    rs = np.exp(rng.uniform(0.0, np.log(10.0), 24))
+    bs = rng.uniform(0.5, 2.0, 24)
+    Ws = rng.integers(64, 1025, 24)
+    sy = [V.System.heg(int(n), float(r), float(b), precision=V.PREC_MIXED, reinvert_interval=256)
+          for n, r, b in zip(Ns, rs, bs)]
+    stream = torch.cuda.Stream()
+    b = V.Batch(sy, Ws, list(range(1, 25)), step0=[0.3 * r for r in rs], stream=stream.cuda_stream,
+                walker_threads=128)
+    b.init_positions()
+    b.sweep(3, adapt=True)
+    b.capture(10, measure=True)
+    b.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(sweeps // 10):
+        b.replay()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    b.close()
+    moves = float(sum(int(n) * int(w) for n, w in zip(Ns, Ws))) * (sweeps // 10) * 10
+    return dict(systems=24, walkers=int(Ws.sum()), moves_per_s=moves / dt,
+                ms_per_sweep=1e3 * dt / ((sweeps // 10) * 10))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--reinvert", type=int, default=64,
+    ap.add_argument("--reinvert", type=int, default=256,
                     help="accepted moves between FP64 re-inversions in the mixed path")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--mixed-only", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the served-stream and batch runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run plain device copies of 1 GiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -287,6 +355,16 @@ def main():
         line["mixed_over_fp64"] = res["mixed"]["moves_per_s"] / res["fp64"]["moves_per_s"]
         line["mixed_over_fp64_generic"] = (res["mixed"]["moves_per_s"] /
                                            res["fp64_generic"]["moves_per_s"])
+    if not a.no_serving:
+        sv = {"mixed": serving_run(V, V.PREC_MIXED)}
+        if not a.mixed_only:
+            sv["fp64"] = serving_run(V, V.PREC_FP64)
+        line["serving"] = sv
+        line["latency_ms"]["p50_solve"] = sv["mixed"]["p50_ms"]
+        line["latency_ms"]["p99_solve"] = sv["mixed"]["p99_ms"]
+        line["latency_ms"]["solve_what"] = ("request latency (submit to result) of a Poisson stream of "
+                                            "40 energy evaluations through the request server, mixed")
+        line["batch"] = batch_run(V, torch)
     line["physics"] = {k: {"e_per_electron": v["e_per_electron"],
                            "se_per_electron": v["se_per_electron"],
                            "acceptance": v["acceptance"],

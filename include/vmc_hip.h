@@ -229,11 +229,12 @@ int vmc_frozen_eval(const vmc_system *sys, int32_t n_configs, const double *pos,
  * All systems of a batch share one working precision.
  *
  * inv_placement[i] (NULL = all 0): 0 inverse in LDS when it fits, 1 LDS
- * (error if it does not fit), 2 HBM. */
+ * (error if it does not fit), 2 HBM.  walker_threads (64, 128 or 256) is the
+ * workgroup per walker of the persistent sweep (1, 2 or 4 wavefronts). */
 typedef struct vmc_batch vmc_batch;
 int vmc_batch_create(const vmc_system *systems, const int64_t *n_walkers, const uint64_t *seeds,
                      const double *step0, const int32_t *inv_placement, int32_t n_systems,
-                     vmc_batch **out);
+                     int32_t walker_threads, vmc_batch **out);
 int vmc_batch_destroy(vmc_batch *b);
 int vmc_batch_init_positions(vmc_batch *b, void *stream);
 /* n_sweeps launches of the persistent batch kernel (flags as for
@@ -255,19 +256,25 @@ int vmc_batch_info(const vmc_batch *b, int32_t *n_systems, int64_t *total_walker
  * (error if it does not fit), 2 HBM.  No reference counterpart (a tuning
  * knob of the device layout). */
 int vmc_ensemble_set_inverse_placement(vmc_ensemble *e, int32_t placement);
+/* Generic kernel's workgroup per walker: 64, 128 or 256 threads (1, 2 or 4
+ * wavefronts share one walker's orbital row, Jastrow pairs, Sherman-Morrison
+ * rows and energy sums; default 64).  No reference counterpart. */
+int vmc_ensemble_set_walker_threads(vmc_ensemble *e, int32_t threads);
 /* Device bytes an ensemble of this system and walker count holds (state and
  * tables), the admission measure of the request server. */
 int vmc_ensemble_footprint(const vmc_system *sys, int64_t n_walkers, int64_t *bytes);
 
 /* --------------------------------------------------------------- autotuner
  * Picks, for one system and walker count, the kernel family (register
- * resident or generic), its build variant, the inverse placement (LDS or
- * HBM) and -- in mixed precision -- the longest FP64 re-inversion interval K
+ * resident or generic), its build variant, the wavefronts per walker and the
+ * inverse placement (LDS or HBM) of the generic kernel and -- in mixed
+ * precision -- the longest FP64 re-inversion interval K
  * in {256, 64, 16} whose measured inverse drift stays below drift_tol, by
  * timing `sweeps` measured sweeps of each candidate with HIP events.  No
  * reference counterpart (the reference has no device layout to tune). */
 typedef struct vmc_tuning {
     int32_t kernel;            /* VMC_KERNEL_GENERIC or VMC_KERNEL_REGISTER */
+    int32_t walker_threads;    /* generic kernel: 64, 128 or 256 threads per walker */
     int32_t rr_variant;        /* register kernel build variant */
     int32_t inv_placement;     /* generic kernel: 1 LDS, 2 HBM */
     int32_t reinvert_interval; /* chosen K (mixed); the system's own value in FP64 */
@@ -276,6 +283,9 @@ typedef struct vmc_tuning {
 } vmc_tuning;
 int vmc_autotune(const vmc_system *sys, int64_t n_walkers, double drift_tol, int32_t sweeps,
                  vmc_tuning *out, void *stream);
+/* The walker workgroup the request server and the C++ facade use for a
+ * system: 64 threads below 64 electrons, 128 below 128, else 256. */
+int vmc_default_walker_threads(const vmc_system *sys);
 
 /* ------------------------------------------------------------------ server
  * Continuous batching of energy evaluations (the runs the reference's

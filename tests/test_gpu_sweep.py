@@ -240,3 +240,46 @@ def test_n114_reinversion_interval_drift_and_energy(vmc, oracle, K):
         e, kin, _ = wk.local_energy()
         scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
         assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale, (w, st[w, vmc.STAT["LAST_E"]], e)
+
+
+@pytest.mark.parametrize("threads", [128, 256])
+def test_generic_multi_wavefront_walkers_follow_the_oracle_twin(vmc, oracle, threads):
+    """The generic kernel with 2 or 4 wavefronts per walker (workgroup-wide
+    sums) follows the oracle's FP64 twin move for move, as the one-wavefront
+    kernel does; N = 66 (33 + 33, no register instantiation)."""
+    s = vmc.System.heg(66, 2.0, 1.0)
+    W, seed, sweeps, step = 3, 6606, 3, 0.6
+    ens = vmc.Ensemble(s, W, seed, step0=step)
+    ens.set_walker_threads(threads)
+    ens.init_positions()
+    osys = oracle_system(oracle, s)
+    walkers = [oracle.Walker(osys, oracle.init_positions(osys, seed, w)[0]) for w in range(W)]
+    for sweep in range(sweeps):
+        ens.sweep(1, measure=True)
+        st, pos = ens.stats(), ens.positions()
+        for w in range(W):
+            _, _, e = walkers[w].sweep(seed, w, sweep, step, move_kind=s.move_kind, adapt=False,
+                                       measure=True)
+            np.testing.assert_allclose(pos[w], walkers[w].positions(), rtol=0, atol=1e-9)
+            assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-9 * max(abs(e), 1e-3 * s.N)
+
+
+def test_generic_multi_wavefront_mixed_with_reinversion(vmc, oracle):
+    """N = 162 (81 + 81, inverse in HBM), 4 wavefronts per walker, mixed
+    precision with FP64 re-inversion every 64 accepted moves: drift recorded,
+    local energy within the mixed budget of the FP64 oracle."""
+    s = vmc.System.heg(162, 3.0, 1.0, precision=1, reinvert_interval=64)
+    ens = vmc.Ensemble(s, 4, 162, step0=0.9)
+    ens.set_walker_threads(256)
+    ens.init_positions()
+    ens.sweep(4, measure=True, adapt=True)
+    st = ens.stats()
+    assert np.all(st[:, vmc.STAT["REINV"]] >= 2)
+    assert np.all(ens.drift() > 0) and np.all(ens.drift() < 1e-3)
+    pos = ens.positions()
+    osys = oracle_system(oracle, s)
+    for w in range(4):
+        wk = oracle.Walker(osys, pos[w])
+        e, kin, _ = wk.local_energy()
+        scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
+        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale

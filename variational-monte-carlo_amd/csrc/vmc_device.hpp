@@ -79,6 +79,22 @@ __device__ inline double wave_uniform(double v) {
 // wavefront barrier.
 __device__ inline void wave_sync() { __syncthreads(); }
 
+// Sum over the whole workgroup (one walker may own several wavefronts): the
+// wave sums, then the per-wave totals in a fixed order, so every thread gets
+// the same value.
+__device__ inline double block_sum(double v) {
+    v = wave_sum(v);
+    if (blockDim.x == kWave) return v;
+    __shared__ double red[16];
+    const int wv = threadIdx.x / kWave;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) red[wv] = v;
+    __syncthreads();
+    double t = red[0];
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) t += red[i];
+    return t;
+}
+
 // ------------------------------------------------------------------ Philox
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) with the published
 // multipliers 0xD2511F53, 0xCD9E8D57 and Weyl keys 0x9E3779B9, 0xBB67AE85.

@@ -104,9 +104,9 @@ __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kI
     double *A = st.scratch + (size_t)w * s.nmax * s.nmax;
     WalkerView<Real> v = carve<Real>(smem, s, kInvLds, inv_hbm, A);
     if (kInvLds)
-        for (int e = lane; e < s.ninv; e += kWave) v.inv[e] = inv_hbm[e];
+        for (int e = lane; e < s.ninv; e += (int)blockDim.x) v.inv[e] = inv_hbm[e];
     double *gp = st.pos + (size_t)w * 3 * N;
-    for (int i = lane; i < N; i += kWave) {
+    for (int i = lane; i < N; i += (int)blockDim.x) {
         v.px[i] = gp[i];
         v.py[i] = gp[N + i];
         v.pz[i] = gp[2 * N + i];
@@ -126,7 +126,7 @@ __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kI
 
     for (int sw = 0; sw < n_sweeps; ++sw) {
         const uint32_t sweep = (uint32_t)(sweep0 + (unsigned long long)sw);
-        for (int q = lane; q < N; q += kWave) {
+        for (int q = lane; q < N; q += (int)blockDim.x) {
             const uint4 r = philox4x32_10(make_uint4((uint32_t)q, (uint32_t)w, sweep, kTagMove),
                                           s.key0, s.key1);
             const uint4 ra = philox4x32_10(make_uint4((uint32_t)q, (uint32_t)w, sweep, kTagAccept),
@@ -168,18 +168,7 @@ __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kI
                 ++acc;
                 const int d = spin_of(s, p);
                 if (d == 0) ld0 += log(fabs(pr.ratio)); else ld1 += log(fabs(pr.ratio));
-                if (s.reinv_k > 0 && ++since >= s.reinv_k) {
-                    since = 0;
-                    for (int dd = 0; dd < s.ndet; ++dd) {
-                        double dr = 0.0;
-                        const double l = reinvert<Real>(s, dd, v, lane, &dr);
-                        drift = (dr > drift || dr != dr) ? dr : drift;
-                        if (!(l > -INFINITY)) wflags |= 1;
-                        else if (dd == 0) ld0 = l;
-                        else ld1 = l;
-                    }
-                    stat[15] += 1.0;
-                }
+                ++since;  // the FP64 re-inversion follows the sweep (k_reinv_generic)
             }
         }
         stat[6] += (double)acc;
@@ -215,8 +204,8 @@ __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kI
         }
     }
     if (kInvLds)
-        for (int e = lane; e < s.ninv; e += kWave) inv_hbm[e] = v.inv[e];
-    for (int i = lane; i < N; i += kWave) {
+        for (int e = lane; e < s.ninv; e += (int)blockDim.x) inv_hbm[e] = v.inv[e];
+    for (int i = lane; i < N; i += (int)blockDim.x) {
         gp[i] = v.px[i];
         gp[N + i] = v.py[i];
         gp[2 * N + i] = v.pz[i];
@@ -234,8 +223,9 @@ __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kI
     wave_sync();
 }
 
+// One walker per workgroup of 64, 128 or 256 threads (1, 2 or 4 wavefronts).
 template <typename Real, bool kInvLds>
-__global__ __launch_bounds__(kWave) void k_sweep(DevSys s, DevState st, int n_sweeps,
+__global__ __launch_bounds__(256) void k_sweep(DevSys s, DevState st, int n_sweeps,
                                                  unsigned long long sweep0, int flags,
                                                  int block_size) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -261,7 +251,7 @@ struct BatchItem {
 // system-local index and their system's seed: a system's chain is the same
 // alone and in a batch.
 template <typename Real>
-__global__ __launch_bounds__(kWave) void k_sweep_batch(const BatchItem *items, int n_items,
+__global__ __launch_bounds__(256) void k_sweep_batch(const BatchItem *items, int n_items,
                                                        int total, unsigned int *queue,
                                                        const unsigned long long *sweep_ctr,
                                                        int flags, int block_size) {
@@ -288,6 +278,198 @@ __global__ void k_batch_advance(unsigned int *queue, unsigned long long *sweep_c
         *queue = 0u;
         *sweep_ctr += 1ull;
     }
+}
+
+// ------------------------------------------------ FP64 re-inversion (generic)
+// Walkers whose accepted moves since their last re-inversion reached K
+// rebuild each determinant's inverse in FP64 after the sweep, one 256-thread
+// workgroup per walker.  E = D^T (E[m][i] = phi_m(r_i)) is built in FP64 in
+// LDS (in the walker's HBM scratch when n^2 doubles do not fit); the drift
+// max|I - E invT| of the inverse being replaced is measured first, four
+// columns of invT at a time; Gauss-Jordan with partial pivoting then inverts
+// E in place (the reference's LU tolerance 1e-12, matrix.hpp:35-39) and the
+// result replaces invT in the working precision.
+constexpr int kReinvThreads = 256;
+constexpr int kReinvCols = 4;
+
+__host__ __device__ inline size_t reinv_lds_bytes(int nmax, bool a_in_lds) {
+    return (a_in_lds ? (size_t)nmax * nmax * 8 : 0) + (size_t)nmax * 8 /* gjcol */ +
+           (size_t)nmax * 4 /* perm */ + (size_t)kReinvCols * nmax * 8 /* columns */ + 64 * 8;
+}
+
+// Block-wide max of v (NaN wins); every thread gets the result.
+__device__ inline double block_max(double v, double *red) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, kWave);
+        v = (o > v || o != o) ? o : v;
+    }
+    const int wv = threadIdx.x / kWave;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) red[wv] = v;
+    __syncthreads();
+    double r = red[0];
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) r = (red[i] > r || red[i] != red[i]) ? red[i] : r;
+    return r;
+}
+
+// Block-wide argmax of (a, idx), ties to the lower index.
+__device__ inline void block_argmax(double &a, int &idx, double *red) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const double oa = __shfl_xor(a, off, kWave);
+        const int oi = __shfl_xor(idx, off, kWave);
+        if (oa > a || (oa == a && oi < idx)) { a = oa; idx = oi; }
+    }
+    const int wv = threadIdx.x / kWave;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        red[2 * wv] = a;
+        red[2 * wv + 1] = (double)idx;
+    }
+    __syncthreads();
+    a = red[0];
+    idx = (int)red[1];
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) {
+        const double oa = red[2 * i];
+        const int oi = (int)red[2 * i + 1];
+        if (oa > a || (oa == a && oi < idx)) { a = oa; idx = oi; }
+    }
+}
+
+template <typename Real>
+__device__ void reinv_walker(const DevSys &s, const DevState &st, int w, bool a_in_lds,
+                             unsigned char *smem) {
+    if (s.reinv_k <= 0 || st.since[w] < s.reinv_k) return;
+    const int t = threadIdx.x, nt = blockDim.x;
+    size_t off = 0;
+    double *A = a_in_lds ? reinterpret_cast<double *>(smem) : st.scratch + (size_t)w * s.nmax * s.nmax;
+    if (a_in_lds) off += (size_t)s.nmax * s.nmax * 8;
+    double *gjcol = reinterpret_cast<double *>(smem + off); off += (size_t)s.nmax * 8;
+    double *cols = reinterpret_cast<double *>(smem + off); off += (size_t)kReinvCols * s.nmax * 8;
+    double *red = reinterpret_cast<double *>(smem + off); off += 64 * 8;
+    int *perm = reinterpret_cast<int *>(smem + off);
+    const double *gp = st.pos + (size_t)w * 3 * s.N;
+    Real *inv = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
+    double drift = 0.0, ld[2] = {st.logdet[2 * w + 0], st.logdet[2 * w + 1]};
+    bool singular = false;
+    for (int d = 0; d < s.ndet; ++d) {
+        const int n = s.n[d], first = s.first[d];
+        Real *Y = inv + s.inv_off[d];
+        for (int e = t; e < n * n; e += nt) {
+            const int j = e / n, i = e - (e / n) * n;
+            const int4 o = s.orb[d * s.nmax + j];
+            double sn, cs;
+            sincos_2pi<double>(frac_phase(o.x, o.y, o.z, gp[first + i] * s.inv_L,
+                                          gp[s.N + first + i] * s.inv_L,
+                                          gp[2 * s.N + first + i] * s.inv_L), &sn, &cs);
+            A[e] = o.w ? sn : cs;
+        }
+        __syncthreads();
+        // Drift of the inverse being replaced, kReinvCols columns at a time.
+        double dmax = 0.0;
+        for (int c0 = 0; c0 < n; c0 += kReinvCols) {
+            const int nc = n - c0 < kReinvCols ? n - c0 : kReinvCols;
+            for (int e = t; e < nc * n; e += nt) {
+                const int c = e / n, i = e - (e / n) * n;
+                cols[c * n + i] = (double)Y[i * n + c0 + c];
+            }
+            __syncthreads();
+            for (int e = t; e < nc * n; e += nt) {
+                const int c = e / n, m = e - (e / n) * n;
+                double z = 0.0;
+                for (int i = 0; i < n; ++i) z += A[m * n + i] * cols[c * n + i];
+                const double r = fabs((m == c0 + c ? 1.0 : 0.0) - z);
+                dmax = (r > dmax || r != r) ? r : dmax;
+            }
+            __syncthreads();
+        }
+        dmax = block_max(dmax, red);
+        drift = (dmax > drift || dmax != dmax) ? dmax : drift;
+        // Gauss-Jordan in place: A becomes (D^T)^-1 = invT.
+        double logdet = 0.0;
+        for (int c = 0; c < n && !singular; ++c) {
+            double best = -1.0;
+            int bi = n;
+            for (int r = c + t; r < n; r += nt) {
+                const double a = fabs(A[r * n + c]);
+                if (a > best) { best = a; bi = r; }
+            }
+            block_argmax(best, bi, red);
+            if (!(best >= 1e-12)) {
+                singular = true;
+                break;
+            }
+            if (t == 0) perm[c] = bi;
+            if (bi != c)
+                for (int j = t; j < n; j += nt) {
+                    const double tmp = A[c * n + j];
+                    A[c * n + j] = A[bi * n + j];
+                    A[bi * n + j] = tmp;
+                }
+            __syncthreads();
+            const double piv = A[c * n + c];
+            logdet += log(fabs(piv));
+            const double pinv = 1.0 / piv;
+            for (int r = t; r < n; r += nt) gjcol[r] = A[r * n + c];
+            __syncthreads();
+            for (int j = t; j < n; j += nt) A[c * n + j] = (j == c ? 1.0 : A[c * n + j]) * pinv;
+            __syncthreads();
+            for (int e = t; e < n * n; e += nt) {
+                const int r = e / n, j = e - (e / n) * n;
+                if (r == c) continue;
+                A[e] = (j == c ? 0.0 : A[e]) - gjcol[r] * A[c * n + j];
+            }
+            __syncthreads();
+        }
+        if (singular) break;
+        for (int c = n - 1; c >= 0; --c) {
+            const int q = perm[c];
+            if (q != c)
+                for (int r = t; r < n; r += nt) {
+                    const double tmp = A[r * n + c];
+                    A[r * n + c] = A[r * n + q];
+                    A[r * n + q] = tmp;
+                }
+            __syncthreads();
+        }
+        for (int e = t; e < n * n; e += nt) Y[e] = (Real)A[e];
+        ld[d] = logdet;
+        __syncthreads();
+    }
+    if (t == 0) {
+        st.since[w] = 0;
+        st.stats[(size_t)w * kStatCount + 15] += 1.0;
+        if (singular) {
+            st.flags[w] |= 1;
+        } else {
+            st.logdet[2 * w + 0] = ld[0];
+            st.logdet[2 * w + 1] = ld[1];
+        }
+        if (drift > st.drift[w] || drift != drift) st.drift[w] = drift;
+    }
+    __syncthreads();
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kReinvThreads) void k_reinv_generic(DevSys s, DevState st, int a_in_lds) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    reinv_walker<Real>(s, st, blockIdx.x, a_in_lds != 0, smem);
+}
+
+// The batch's re-inversion: one workgroup per global walker, each finding
+// its system; walkers not due leave at once.
+template <typename Real>
+__global__ __launch_bounds__(kReinvThreads) void k_reinv_batch(const BatchItem *items, int n_items,
+                                                               size_t lds_budget) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int g = blockIdx.x;
+    int i = 0;
+    while (i + 1 < n_items && g >= items[i].walker_end) ++i;
+    const BatchItem &it = items[i];
+    if (it.s.reinv_k <= 0) return;
+    const bool a_in_lds = reinv_lds_bytes(it.s.nmax, true) <= lds_budget;
+    reinv_walker<Real>(it.s, it.st, g - it.walker_begin, a_in_lds, smem);
 }
 
 // Frozen-configuration evaluation (one configuration per wavefront).
@@ -545,6 +727,7 @@ struct vmc_ensemble {
     int rr_variant = 0;
     KernelTimer timer;
     int inv_placement = 0;  // generic kernel: 0 LDS when it fits, 1 LDS, 2 HBM
+    int walker_threads = 64;  // generic kernel: threads (1, 2 or 4 wavefronts) per walker
     bool in_batch = false;  // owned by a vmc_batch: swept and destroyed through it
 
     ~vmc_ensemble() {
@@ -589,18 +772,34 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
     }
     const size_t lds_in = lds_for(d, sizeof(Real), true);
     const unsigned grid = static_cast<unsigned>(e->W);
-    hipEvent_t ev = nullptr;
-    const bool timed = e->timer.begin(stream, ev);
-    if (lds_in <= kMaxLdsBytes && e->inv_placement != 2) {
-        hipLaunchKernelGGL((k_sweep<Real, true>), dim3(grid), dim3(kWave), lds_in, stream, d, e->st,
-                           n_sweeps, (unsigned long long)e->sweeps_done, flags, block_size);
-    } else {
-        const size_t lds_out = lds_for(d, sizeof(Real), false);
-        if (lds_out > kMaxLdsBytes) return fail(VMC_ERR_INVALID, "system too large for one walker per wavefront");
-        hipLaunchKernelGGL((k_sweep<Real, false>), dim3(grid), dim3(kWave), lds_out, stream, d,
-                           e->st, n_sweeps, (unsigned long long)e->sweeps_done, flags, block_size);
+    const bool inv_lds = lds_in <= kMaxLdsBytes && e->inv_placement != 2;
+    const size_t lds = inv_lds ? lds_in : lds_for(d, sizeof(Real), false);
+    if (lds > kMaxLdsBytes) return fail(VMC_ERR_INVALID, "system too large for one walker per wavefront");
+    // With periodic re-inversion, one sweep per launch, each followed by the
+    // FP64 re-inversion of the walkers that came due.
+    const bool reinv = d.reinv_k > 0;
+    const bool a_lds = reinv_lds_bytes(d.nmax, true) <= kMaxLdsBytes;
+    const size_t lds_r = reinv_lds_bytes(d.nmax, a_lds);
+    const int per_launch = reinv ? 1 : n_sweeps;
+    for (int i = 0; i < n_sweeps; i += per_launch) {
+        const int n = n_sweeps - i < per_launch ? n_sweeps - i : per_launch;
+        const unsigned long long s0 = (unsigned long long)e->sweeps_done + (unsigned long long)i;
+        hipEvent_t ev = nullptr;
+        bool timed = e->timer.begin(stream, ev);
+        if (inv_lds)
+            hipLaunchKernelGGL((k_sweep<Real, true>), dim3(grid), dim3(e->walker_threads), lds, stream, d, e->st, n,
+                               s0, flags, block_size);
+        else
+            hipLaunchKernelGGL((k_sweep<Real, false>), dim3(grid), dim3(e->walker_threads), lds, stream, d, e->st, n,
+                               s0, flags, block_size);
+        if (timed) e->timer.end(stream, ev, kTimeSweep);
+        if (reinv) {
+            timed = e->timer.begin(stream, ev);
+            hipLaunchKernelGGL((k_reinv_generic<Real>), dim3(grid), dim3(kReinvThreads), lds_r, stream, d,
+                               e->st, a_lds ? 1 : 0);
+            if (timed) e->timer.end(stream, ev, kTimeReinv);
+        }
     }
-    if (timed) e->timer.end(stream, ev, kTimeSweep);
     VMC_HIP_TRY(hipGetLastError());
     return VMC_OK;
 }
@@ -794,6 +993,14 @@ int vmc_ensemble_set_inverse_placement(vmc_ensemble *e, int32_t placement) {
     return VMC_OK;
 }
 
+int vmc_ensemble_set_walker_threads(vmc_ensemble *e, int32_t threads) {
+    if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
+    if (threads != 64 && threads != 128 && threads != 256)
+        return fail(VMC_ERR_INVALID, "threads per walker must be 64, 128 or 256");
+    e->walker_threads = threads;
+    return VMC_OK;
+}
+
 int vmc_ensemble_get_kernel(const vmc_ensemble *e, int32_t *kernel) {
     if (!e || !kernel) return fail(VMC_ERR_INVALID, "NULL argument");
     *kernel = (e->kernel != VMC_KERNEL_GENERIC && rr_supported(e->tables.dev)) ? VMC_KERNEL_REGISTER
@@ -964,6 +1171,9 @@ struct vmc_batch {
     int total = 0;
     int grid = 0;
     size_t lds = 0;
+    int threads = 64;       // threads per walker of the persistent sweep
+    bool reinv = false;     // some system re-inverts periodically
+    size_t reinv_lds = 0;   // dynamic LDS of the batch re-inversion kernel
     int real_bytes = 8;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -986,9 +1196,12 @@ namespace {
 template <typename Real>
 int batch_launch(vmc_batch *b, int n_sweeps, int flags, int block_size, hipStream_t stream) {
     for (int i = 0; i < n_sweeps; ++i) {
-        hipLaunchKernelGGL((k_sweep_batch<Real>), dim3(static_cast<unsigned>(b->grid)), dim3(kWave),
+        hipLaunchKernelGGL((k_sweep_batch<Real>), dim3(static_cast<unsigned>(b->grid)), dim3(b->threads),
                            b->lds, stream, b->items, static_cast<int>(b->members.size()), b->total,
                            b->queue, b->ctr, flags, block_size);
+        if (b->reinv)
+            hipLaunchKernelGGL((k_reinv_batch<Real>), dim3(static_cast<unsigned>(b->total)), dim3(kReinvThreads),
+                               b->reinv_lds, stream, b->items, static_cast<int>(b->members.size()), b->reinv_lds);
         hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1), 0, stream, b->queue, b->ctr);
     }
     VMC_HIP_TRY(hipGetLastError());
@@ -1006,14 +1219,17 @@ extern "C" {
 
 int vmc_batch_create(const vmc_system *systems, const int64_t *n_walkers, const uint64_t *seeds,
                      const double *step0, const int32_t *inv_placement, int32_t n_systems,
-                     vmc_batch **out) {
+                     int32_t walker_threads, vmc_batch **out) {
     if (!out || !systems || !n_walkers || !seeds || !step0) return fail(VMC_ERR_INVALID, "NULL argument");
+    if (walker_threads != 64 && walker_threads != 128 && walker_threads != 256)
+        return fail(VMC_ERR_INVALID, "walker_threads must be 64, 128 or 256");
     *out = nullptr;
     if (n_systems < 1 || n_systems > 4096) return fail(VMC_ERR_INVALID, "n_systems out of range");
     for (int i = 1; i < n_systems; ++i)
         if ((systems[i].precision != VMC_PREC_FP64) != (systems[0].precision != VMC_PREC_FP64))
             return fail(VMC_ERR_INVALID, "a batch runs one working precision");
     auto *b = new vmc_batch();
+    b->threads = walker_threads;
     std::vector<BatchItem> items(static_cast<size_t>(n_systems));
     int64_t total = 0;
     for (int i = 0; i < n_systems; ++i) {
@@ -1041,6 +1257,11 @@ int vmc_batch_create(const vmc_system *systems, const int64_t *n_walkers, const 
         total += n_walkers[i];
         items[i].walker_end = static_cast<int>(total);
         b->lds = std::max(b->lds, lds_for(d, e->tables.real_bytes, inv_lds));
+        if (d.reinv_k > 0) {
+            b->reinv = true;
+            const bool a_lds = reinv_lds_bytes(d.nmax, true) <= kMaxLdsBytes;
+            b->reinv_lds = std::max(b->reinv_lds, reinv_lds_bytes(d.nmax, a_lds));
+        }
     }
     if (total > (int64_t(1) << 30)) {
         delete b;
@@ -1064,8 +1285,8 @@ int vmc_batch_create(const vmc_system *systems, const int64_t *n_walkers, const 
     VMC_HIP_TRY(hipGetDevice(&dev));
     VMC_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     hipError_t oc = b->real_bytes == 8
-                        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_batch<double>, kWave, b->lds)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_batch<float>, kWave, b->lds);
+                        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_batch<double>, b->threads, b->lds)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_batch<float>, b->threads, b->lds);
     if (oc != hipSuccess || per_cu < 1) per_cu = 1;
     b->grid = static_cast<int>(std::min<int64_t>(total, static_cast<int64_t>(cus) * per_cu));
     *out = b;

@@ -52,8 +52,22 @@ void aggregate(const std::vector<double> &st, int64_t W, double *mean, double *s
     *acc = a / static_cast<double>(W);
 }
 
+}  // namespace
+
+extern "C" int vmc_default_walker_threads(const vmc_system *sys) {
+    // Wavefronts per walker of the generic kernel by size: one up to 63
+    // electrons, two up to 127, four above (a function of the system only, so
+    // a request's result does not depend on what else runs).
+    const int N = sys->n_up + sys->n_dn;
+    return N < 64 ? 64 : N < 128 ? 128 : 256;
+}
+
+namespace {
+
+int walker_threads_for(const vmc_system &s) { return vmc_default_walker_threads(&s); }
+
 struct Candidate {
-    int kernel, variant, placement;
+    int kernel, variant, placement, threads;
 };
 
 // Device time of `sweeps` measured sweeps of a fresh ensemble with the given
@@ -65,7 +79,8 @@ double time_candidate(const vmc_system &sys, int64_t W, const Candidate &c, int 
     double ms = -1.0;
     hipEvent_t a = nullptr, b = nullptr;
     if (!vmc_ensemble_set_kernel(e, c.kernel) && !vmc_ensemble_set_rr_variant(e, c.variant) &&
-        !vmc_ensemble_set_inverse_placement(e, c.placement) && !vmc_ensemble_init_positions(e, s) &&
+        !vmc_ensemble_set_inverse_placement(e, c.placement) &&
+        !vmc_ensemble_set_walker_threads(e, c.threads) && !vmc_ensemble_init_positions(e, s) &&
         !vmc_ensemble_sweep(e, 2, VMC_SWEEP_ADAPT, 1, s) && hipEventCreate(&a) == hipSuccess &&
         hipEventCreate(&b) == hipSuccess && hipEventRecord(a, s) == hipSuccess &&
         !vmc_ensemble_sweep(e, sweeps, VMC_SWEEP_MEASURE, 1, s) && hipEventRecord(b, s) == hipSuccess &&
@@ -99,11 +114,14 @@ int vmc_autotune(const vmc_system *sys, int64_t n_walkers, double drift_tol, int
         int rc = vmc_ensemble_create(sys, 1, 1, 0.1 * sys->box_length, &e);
         if (rc) return rc;
         if (!vmc_ensemble_set_kernel(e, VMC_KERNEL_REGISTER)) {
-            cands.push_back({VMC_KERNEL_REGISTER, 0, 0});
-            cands.push_back({VMC_KERNEL_REGISTER, 1, 0});
+            cands.push_back({VMC_KERNEL_REGISTER, 0, 0, 64});
+            cands.push_back({VMC_KERNEL_REGISTER, 1, 0, 64});
         }
-        if (!vmc_ensemble_set_inverse_placement(e, 1)) cands.push_back({VMC_KERNEL_GENERIC, 0, 1});
-        cands.push_back({VMC_KERNEL_GENERIC, 0, 2});
+        const bool lds_ok = !vmc_ensemble_set_inverse_placement(e, 1);
+        for (int th : {64, 128, 256}) {
+            if (lds_ok) cands.push_back({VMC_KERNEL_GENERIC, 0, 1, th});
+            cands.push_back({VMC_KERNEL_GENERIC, 0, 2, th});
+        }
         vmc_ensemble_destroy(e);
     }
     vmc_system t = *sys;
@@ -128,6 +146,7 @@ int vmc_autotune(const vmc_system *sys, int64_t n_walkers, double drift_tol, int
         if (rate > best) {
             best = rate;
             out->kernel = c.kernel;
+            out->walker_threads = c.threads;
             out->rr_variant = c.variant;
             out->inv_placement = c.placement;
         }
@@ -233,6 +252,7 @@ void vmc_server::run() {
             a.stream = free_streams.back();
             free_streams.pop_back();
             int rc = vmc_ensemble_create(&p.req.system, p.req.n_walkers, p.req.seed, p.req.step0, &a.e);
+            if (!rc) rc = vmc_ensemble_set_walker_threads(a.e, walker_threads_for(p.req.system));
             if (!rc) rc = vmc_ensemble_init_positions(a.e, a.stream);
             if (rc) {
                 vmc_result r{};

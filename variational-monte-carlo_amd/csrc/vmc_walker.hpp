@@ -201,7 +201,7 @@ __device__ double reinvert(const DevSys &s, int d, const WalkerView<Real> &v, in
 template <typename Real>
 __device__ inline void jastrow_rows(const DevSys &s, const WalkerView<Real> &v, int lane) {
     const Real b = (Real)s.b;
-    for (int i = lane; i < s.N; i += kWave) {
+    for (int i = lane; i < s.N; i += (int)blockDim.x) {
         double acc = 0.0;
         for (int j = 0; j < s.N; ++j) {
             if (j == i) continue;
@@ -230,7 +230,7 @@ __device__ Measured measure(const DevSys &s, const WalkerView<Real> &v, int lane
     const Real b = (Real)s.b;
     const Real ku = (Real)s.kunit;
     double kin = 0.0;
-    for (int i = lane; i < N; i += kWave) {
+    for (int i = lane; i < N; i += (int)blockDim.x) {
         double gx = 0, gy = 0, gz = 0, lp = 0;
         for (int j = 0; j < N; ++j) {
             if (j == i) continue;
@@ -279,18 +279,18 @@ __device__ Measured measure(const DevSys &s, const WalkerView<Real> &v, int lane
         }
     }
     Measured m;
-    m.kinetic = -0.5 * wave_sum(kin);
+    m.kinetic = -0.5 * block_sum(kin);
     double vr = 0.0;
-    for (int i = lane; i < N; i += kWave) {
+    for (int i = lane; i < N; i += (int)blockDim.x) {
         for (int j = i + 1; j < N; ++j) {
             const Real r = pair_dist<Real>(s, v.px[i], v.py[i], v.pz[i], v, j, nullptr, nullptr,
                                            nullptr);
             vr += (double)ewald_pair<Real>(r, s);
         }
     }
-    m.vreal = wave_sum(vr);
+    m.vreal = block_sum(vr);
     double vq = 0.0;
-    for (int g = lane; g < s.ng; g += kWave) {
+    for (int g = lane; g < s.ng; g += (int)blockDim.x) {
         const int4 G = s.gvec[g];
         double sre = 0.0, sim = 0.0;
         for (int i = 0; i < N; ++i) {
@@ -307,7 +307,7 @@ __device__ Measured measure(const DevSys &s, const WalkerView<Real> &v, int lane
             sk_out[2 * g + 1] = sim;
         }
     }
-    m.vrecip = s.pref * wave_sum(vq);
+    m.vrecip = s.pref * block_sum(vq);
     m.e_local = m.kinetic + m.vreal + m.vrecip + s.e_const;
     return m;
 }
@@ -331,14 +331,14 @@ __device__ inline Proposal propose(const DevSys &s, const WalkerView<Real> &v, i
     const Real *invs = v.inv + s.inv_off[d];
     const double ux = x * s.inv_L, uy = y * s.inv_L, uz = z * s.inv_L;
     double rp = 0.0;
-    for (int j = lane; j < n; j += kWave) {
+    for (int j = lane; j < n; j += (int)blockDim.x) {
         const Real phi = orbital<Real>(s, d, j, ux, uy, uz);
         v.row[j] = phi;
         rp += (double)phi * (double)invs[pl * n + j];
     }
     const Real b = (Real)s.b;
     double jp = 0.0;
-    for (int j = lane; j < s.N; j += kWave) {
+    for (int j = lane; j < s.N; j += (int)blockDim.x) {
         double u = 0.0;
         if (j != p) {
             const Real r = pair_dist<Real>(s, x, y, z, v, j, nullptr, nullptr, nullptr);
@@ -348,8 +348,8 @@ __device__ inline Proposal propose(const DevSys &s, const WalkerView<Real> &v, i
         jp += u;
     }
     Proposal pr;
-    pr.ratio = wave_sum(rp);
-    pr.jnew = wave_sum(jp);
+    pr.ratio = block_sum(rp);
+    pr.jnew = block_sum(jp);
     pr.dj = pr.jnew - v.U[p];
     wave_sync();
     return pr;
@@ -366,15 +366,15 @@ __device__ inline void accept_move(const DevSys &s, const WalkerView<Real> &v, i
     Real *invs = v.inv + s.inv_off[d];
     const Real b = (Real)s.b;
     const double ox = v.px[p], oy = v.py[p], oz = v.pz[p];
-    for (int j = lane; j < s.N; j += kWave) {
+    for (int j = lane; j < s.N; j += (int)blockDim.x) {
         if (j == p) continue;
         const Real r = pair_dist<Real>(s, ox, oy, oz, v, j, nullptr, nullptr, nullptr);
         v.U[j] += v.uj[j] - (double)pade_u<Real>(r, pair_a<Real>(s, p, j), b);
     }
-    for (int j = lane; j < n; j += kWave) v.col[j] = invs[pl * n + j];
+    for (int j = lane; j < n; j += (int)blockDim.x) v.col[j] = invs[pl * n + j];
     wave_sync();
     const Real inv_ratio = (Real)(1.0 / pr.ratio);
-    for (int k = lane; k < n; k += kWave) {
+    for (int k = lane; k < n; k += (int)blockDim.x) {
         if (k == pl) continue;
         double sk = 0.0;
         for (int m = 0; m < n; ++m) sk += (double)v.row[m] * (double)invs[k * n + m];
@@ -382,7 +382,7 @@ __device__ inline void accept_move(const DevSys &s, const WalkerView<Real> &v, i
         for (int j = 0; j < n; ++j) invs[k * n + j] -= v.col[j] * f;
     }
     wave_sync();
-    for (int j = lane; j < n; j += kWave) invs[pl * n + j] = v.col[j] * inv_ratio;
+    for (int j = lane; j < n; j += (int)blockDim.x) invs[pl * n + j] = v.col[j] * inv_ratio;
     if (lane == 0) {
         v.U[p] = pr.jnew;
         v.px[p] = x;

@@ -91,6 +91,8 @@ void Simulation::create_ensemble() {
     check(vmc_ensemble_create(&sys, static_cast<int64_t>(config_.walkers()), config_.master_seed,
                               config_.step_size, &ensemble_),
           "ensemble creation");
+    check(vmc_ensemble_set_walker_threads(ensemble_, vmc_default_walker_threads(&sys)),
+          "walker workgroup");
 }
 
 Simulation::MeasurementSummary Simulation::run() {

@@ -66,7 +66,8 @@ class SystemDesc(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("kernel", C.c_int32), ("rr_variant", C.c_int32), ("inv_placement", C.c_int32),
+    _fields_ = [("kernel", C.c_int32), ("walker_threads", C.c_int32), ("rr_variant", C.c_int32),
+                ("inv_placement", C.c_int32),
                 ("reinvert_interval", C.c_int32), ("moves_per_s", C.c_double), ("drift", C.c_double)]
 
 
@@ -193,7 +194,9 @@ def lib():
             "vmc_server_poll": (C.c_int, [_VP, C.POINTER(Result), C.c_int32, _ip, C.c_int32]),
             "vmc_server_destroy": (C.c_int, [_VP]),
             "vmc_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
-                                           _dp, _ip, C.c_int32, C.POINTER(_VP)]),
+                                           _dp, _ip, C.c_int32, C.c_int32, C.POINTER(_VP)]),
+            "vmc_ensemble_set_walker_threads": (C.c_int, [_VP, C.c_int32]),
+            "vmc_default_walker_threads": (C.c_int, [sp]),
             "vmc_batch_destroy": (C.c_int, [_VP]),
             "vmc_batch_init_positions": (C.c_int, [_VP, _VP]),
             "vmc_batch_sweep": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _VP]),
@@ -291,6 +294,14 @@ class Ensemble:
 
     def set_kernel(self, kernel):
         _check(lib().vmc_ensemble_set_kernel(self._h, kernel))
+
+    def set_walker_threads(self, threads):
+        """Generic kernel: 64, 128 or 256 threads (1, 2 or 4 wavefronts) per walker."""
+        _check(lib().vmc_ensemble_set_walker_threads(self._h, int(threads)))
+
+    def set_inverse_placement(self, placement):
+        """Generic kernel: 0 LDS when it fits, 1 LDS, 2 HBM."""
+        _check(lib().vmc_ensemble_set_inverse_placement(self._h, int(placement)))
 
     def kernel(self):
         k = C.c_int32()
@@ -396,7 +407,8 @@ def blocking_summary(stats):
 class Batch:
     """Systems of uneven size swept by one persistent launch (vmc_batch)."""
 
-    def __init__(self, systems, walkers, seeds, step0=None, placement=None, stream=None):
+    def __init__(self, systems, walkers, seeds, step0=None, placement=None, stream=None,
+                 walker_threads=64):
         n = len(systems)
         self.systems = list(systems)
         self.walkers = [int(w) for w in walkers]
@@ -410,7 +422,8 @@ class Batch:
         pl = None if placement is None else np.ascontiguousarray(placement, dtype=np.int32)
         self._h = _VP()
         _check(lib().vmc_batch_create(C.cast(self._descs, C.c_void_p), w, sd, _ptr(st),
-                                      None if pl is None else _ptr(pl, _ip), n, C.byref(self._h)))
+                                      None if pl is None else _ptr(pl, _ip), n, int(walker_threads),
+                                      C.byref(self._h)))
 
     def close(self):
         if self._h:
@@ -463,7 +476,8 @@ def autotune(system: System, n_walkers: int, drift_tol=1e-3, sweeps=10, stream=N
     d = system.desc()
     _check(lib().vmc_autotune(C.byref(d), int(n_walkers), float(drift_tol), int(sweeps), C.byref(t),
                               stream))
-    return dict(kernel=t.kernel, rr_variant=t.rr_variant, inv_placement=t.inv_placement,
+    return dict(kernel=t.kernel, walker_threads=t.walker_threads, rr_variant=t.rr_variant,
+                inv_placement=t.inv_placement,
                 reinvert_interval=t.reinvert_interval, moves_per_s=t.moves_per_s, drift=t.drift)
 
 
@@ -500,3 +514,8 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def default_walker_threads(system: System) -> int:
+    d = system.desc()
+    return lib().vmc_default_walker_threads(C.byref(d))
