@@ -23,6 +23,7 @@
 #include "vmc_walker.hpp"
 
 #include <array>
+#include <cstddef>
 
 namespace vmc {
 namespace rr {
@@ -204,6 +205,66 @@ struct RRShared {
     int perm[32];
 };
 
+// The LDS run prop | pph | tab of RRShared doubles as the stage through which
+// inverse blocks move between HBM and the lanes' registers.
+template <typename Real, int NU, int ND>
+__device__ __forceinline__ Real *rr_stage(RRShared<Real, NU, ND> &sh) {
+    using Sh = RRShared<Real, NU, ND>;
+    static_assert(offsetof(Sh, pph) == offsetof(Sh, prop) + sizeof(sh.prop) &&
+                      offsetof(Sh, tab) == offsetof(Sh, pph) + sizeof(sh.pph),
+                  "stage run must be contiguous");
+    constexpr int NMAX = NU > ND ? NU : ND;
+    static_assert(sizeof(sh.prop) + sizeof(sh.pph) + sizeof(sh.tab) >= sizeof(Real) * NMAX * NMAX,
+                  "stage too small for one inverse block");
+    return reinterpret_cast<Real *>(&sh.prop[0][0]);
+}
+
+// Inverse blocks of both spins between HBM ([NU x NU][ND x ND], row k =
+// invT[k][.]) and the lanes' registers (lane (h, r) holds row r of spin h):
+// each block moves with coalesced loads or stores of consecutive elements and
+// every lane reads or writes its own row in LDS (odd row stride, no bank
+// conflicts).
+template <typename Real, int NU, int ND, int NMAX>
+__device__ __forceinline__ void rr_inverse_in(const Real *blk, Real *stage, Real (&inv)[NMAX],
+                                              int lane, int h, int r) {
+#pragma unroll
+    for (int m = 0; m < NMAX; ++m) inv[m] = (Real)0;
+#pragma unroll
+    for (int hp = 0; hp < (ND > 0 ? 2 : 1); ++hp) {
+        constexpr int kU = NU;
+        const int np = hp ? ND : kU;
+        const Real *src = blk + (hp ? kU * kU : 0);
+        for (int e = lane; e < np * np; e += kWave) stage[e] = src[e];
+        __syncthreads();
+        if (h == hp && r < np) {
+#pragma unroll
+            for (int m = 0; m < NMAX; ++m)
+                if (m < np) inv[m] = stage[r * np + m];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename Real, int NU, int ND, int NMAX>
+__device__ __forceinline__ void rr_inverse_out(Real *blk, Real *stage, const Real (&inv)[NMAX],
+                                               int lane, int h, int r) {
+#pragma unroll
+    for (int hp = 0; hp < (ND > 0 ? 2 : 1); ++hp) {
+        constexpr int kU = NU;
+        const int np = hp ? ND : kU;
+        __syncthreads();
+        if (h == hp && r < np) {
+#pragma unroll
+            for (int m = 0; m < NMAX; ++m)
+                if (m < np) stage[r * np + m] = inv[m];
+        }
+        __syncthreads();
+        Real *dst = blk + (hp ? kU * kU : 0);
+        for (int e = lane; e < np * np; e += kWave) dst[e] = stage[e];
+    }
+    __syncthreads();
+}
+
 // Loop-invariant system scalars, hoisted once per kernel into registers.
 template <typename Real>
 struct RRConst {
@@ -341,26 +402,41 @@ struct RRKernel {
         lap = 0.0;
         vreal = 0.0;
         const Real m2b = (Real)-2 * c.b;
-        for (int j = 0; j < N; ++j) {
-            const int lj = lane_of(j);
-            const double xj = sh.pos[0][lj], yj = sh.pos[1][lj], zj = sh.pos[2][lj];
-            if (!has_e || lj == lane) continue;
-            const Real dx = (Real)c.mi(L.x - xj);
-            const Real dy = (Real)c.mi(L.y - yj);
-            const Real dz = (Real)c.mi(L.z - zj);
-            const Real r = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
-            const Real a = ((lj >> 5) == h) ? c.a_same : c.a_anti;
-            const Real den = wrcp<Real>((Real)1 + c.b * r);
-            U += (double)(a * r * den);
-            vreal += (double)c.ewald(r);
-            if (r < (Real)1e-12) continue;
-            const Real d1 = a * den * den;
-            const Real inv_r = wrcp<Real>(r);
-            const Real gf = d1 * inv_r;
-            g[0] += (double)(gf * dx);
-            g[1] += (double)(gf * dy);
-            g[2] += (double)(gf * dz);
-            lap += (double)(d1 * (m2b * den + (Real)2 * inv_r));
+        // Partners in chunks of kPairChunk: terms summed in the working
+        // precision within a chunk, chunk sums in FP64.
+        constexpr int kPairChunk = sizeof(Real) == 4 ? 8 : 1;
+        for (int j0 = 0; j0 < N; j0 += kPairChunk) {
+            Real u_c = 0, gx_c = 0, gy_c = 0, gz_c = 0, l_c = 0, v_c = 0;
+#pragma unroll
+            for (int jj = 0; jj < kPairChunk; ++jj) {
+                const int j = j0 + jj;
+                if (j >= N) break;
+                const int lj = lane_of(j);
+                const double xj = sh.pos[0][lj], yj = sh.pos[1][lj], zj = sh.pos[2][lj];
+                if (!has_e || lj == lane) continue;
+                const Real dx = c.mi_r(L.x, xj);
+                const Real dy = c.mi_r(L.y, yj);
+                const Real dz = c.mi_r(L.z, zj);
+                const Real r = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+                const Real a = ((lj >> 5) == h) ? c.a_same : c.a_anti;
+                const Real den = wrcp<Real>((Real)1 + c.b * r);
+                u_c += a * r * den;
+                v_c += c.ewald(r);
+                if (r < (Real)1e-12) continue;
+                const Real d1 = a * den * den;
+                const Real inv_r = wrcp<Real>(r);
+                const Real gf = d1 * inv_r;
+                gx_c += gf * dx;
+                gy_c += gf * dy;
+                gz_c += gf * dz;
+                l_c += d1 * (m2b * den + (Real)2 * inv_r);
+            }
+            U += (double)u_c;
+            vreal += (double)v_c;
+            g[0] += (double)gx_c;
+            g[1] += (double)gy_c;
+            g[2] += (double)gz_c;
+            lap += (double)l_c;
         }
         vreal *= 0.5;
     }
@@ -657,9 +733,7 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
     RRLane<Real, NMAX> L;
     const Real *inv_hbm = reinterpret_cast<const Real *>(st.inv) + (size_t)w * s.ninv;
     const double *gp = st.pos + (size_t)w * 3 * N;
-#pragma unroll
-    for (int m = 0; m < NMAX; ++m)
-        L.inv[m] = (has_e && m < nh) ? inv_hbm[s.inv_off[h] + r * nh + m] : (Real)0;
+    rr_inverse_in<Real, NU, ND, NMAX>(inv_hbm, rr_stage(sh), L.inv, lane, h, r);
     L.x = has_e ? gp[e] : 0.0;
     L.y = has_e ? gp[N + e] : 0.0;
     L.z = has_e ? gp[2 * N + e] : 0.0;
@@ -705,10 +779,8 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
         }
     }
     Real *inv_out = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
+    rr_inverse_out<Real, NU, ND, NMAX>(inv_out, rr_stage(sh), L.inv, lane, h, r);
     if (has_e) {
-#pragma unroll
-        for (int m = 0; m < NMAX; ++m)
-            if (m < nh) inv_out[s.inv_off[h] + r * nh + m] = L.inv[m];
         double *gq = st.pos + (size_t)w * 3 * N;
         gq[e] = L.x;
         gq[N + e] = L.y;
@@ -938,9 +1010,7 @@ __global__ __launch_bounds__(kWave) void k_reinv_rr(DevSys s, DevState st) {
     RRLane<Real, NMAX> L;
     Real *inv_hbm = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
     const double *gp = st.pos + (size_t)w * 3 * N;
-#pragma unroll
-    for (int m = 0; m < NMAX; ++m)
-        L.inv[m] = (has_e && m < nh) ? inv_hbm[s.inv_off[h] + r * nh + m] : (Real)0;
+    rr_inverse_in<Real, NU, ND, NMAX>(inv_hbm, reinterpret_cast<Real *>(ps.E), L.inv, lane, h, r);
     L.x = has_e ? gp[e] : 0.0;
     L.y = has_e ? gp[N + e] : 0.0;
     L.z = has_e ? gp[2 * N + e] : 0.0;
@@ -950,11 +1020,7 @@ __global__ __launch_bounds__(kWave) void k_reinv_rr(DevSys s, DevState st) {
         const double d1 = ns_reinvert<Real, NU, ND, GPL, 1>(s, ps, L, lane, h, r, ld1);
         d = (d < 0.0 || d1 < 0.0) ? -1.0 : fmax(d, d1);
     }
-    if (has_e) {
-#pragma unroll
-        for (int m = 0; m < NMAX; ++m)
-            if (m < nh) inv_hbm[s.inv_off[h] + r * nh + m] = L.inv[m];
-    }
+    rr_inverse_out<Real, NU, ND, NMAX>(inv_hbm, reinterpret_cast<Real *>(ps.E), L.inv, lane, h, r);
     if (lane == 0) {
         st.since[w] = 0;
         st.logdet[2 * w + 0] = ld0;
@@ -984,9 +1050,7 @@ __global__ __launch_bounds__(kWave) void k_measure_rr(DevSys s, DevState st, int
     RRLane<Real, NMAX> L;
     const Real *inv_hbm = reinterpret_cast<const Real *>(st.inv) + (size_t)w * s.ninv;
     const double *gp = st.pos + (size_t)w * 3 * N;
-#pragma unroll
-    for (int m = 0; m < NMAX; ++m)
-        L.inv[m] = (has_e && m < nh) ? inv_hbm[s.inv_off[h] + r * nh + m] : (Real)0;
+    rr_inverse_in<Real, NU, ND, NMAX>(inv_hbm, rr_stage(sh), L.inv, lane, h, r);
     L.x = has_e ? gp[e] : 0.0;
     L.y = has_e ? gp[N + e] : 0.0;
     L.z = has_e ? gp[2 * N + e] : 0.0;
