@@ -79,3 +79,32 @@ def test_cli_optimises_b_first_like_the_reference(oracle, tmp_path):
         grid = np.linspace(1 / rs, max(5.0, 1 / rs + 0.5), 2 * threads)
     assert np.min(np.abs(grid - b)) < 1e-3, (b, grid)
     assert "(optimized)" in out and np.isfinite(e)
+
+
+@pytest.mark.parametrize("n,L,a,b", [(7, 6.0, 0.25, 1.0), (19, 9.0, 0.3, 0.9), (27, 12.0, 0.25, 1.0)])
+def test_wavefunction_and_energy_tracker_match_the_reference_classes(oracle, tmp_path, n, L, a, b):
+    """The facade's WaveFunction / EnergyTracker (device evaluation) give the
+    reference classes' ln psi, grad / laplacian of ln psi and local energy on
+    the same configurations (reference built from its sources as a checker),
+    to 1e-10 relative."""
+    if not oracle.ref_available():
+        pytest.skip("reference checker library not built (oracle/_ref)")
+    selftest = os.path.join(ROOT, "variational-monte-carlo_amd", "bin", "vmc_host_selftest")
+    rng = np.random.default_rng(n)
+    for trial in range(3):
+        pos = rng.uniform(0.0, L, size=(n, 3))
+        f = tmp_path / f"pos{trial}.txt"
+        f.write_text("\n".join(" ".join(repr(float(v)) for v in row) for row in pos))
+        r = subprocess.run([selftest, "wavefunction", str(n), repr(L), repr(a), repr(b), str(f)],
+                           capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, r.stdout + r.stderr
+        vals = dict(x.split("=", 1) for x in r.stdout.splitlines() if x.startswith(("log_psi", "e_local")))
+        gl = np.array([[float(v) for v in x[3:].split()] for x in r.stdout.splitlines() if x.startswith("gl=")])
+        r_lp, r_grad, r_lap, r_e = oracle.ref_local_energy(pos, L, a, b)
+        lp, e = float(vals["log_psi"]), float(vals["e_local"])
+        assert abs(lp - r_lp) <= 1e-10 * max(1.0, abs(r_lp))
+        # E_L: a sum with cancellation, compared with the size of its parts.
+        parts = 0.5 * (np.abs(r_lap).sum() + (r_grad ** 2).sum()) + abs(r_e)
+        assert abs(e - r_e) <= 1e-10 * parts
+        np.testing.assert_allclose(gl[:, :3], r_grad, atol=1e-10 * max(1.0, np.abs(r_grad).max()), rtol=0)
+        np.testing.assert_allclose(gl[:, 3], r_lap, atol=1e-10 * max(1.0, np.abs(r_lap).max()), rtol=0)

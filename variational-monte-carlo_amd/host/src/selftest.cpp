@@ -5,9 +5,14 @@
 //   vmc_host_selftest blocking <bs> <x...>   mean and SE of the samples
 //   vmc_host_selftest scan <path>            the optimiser's b grid and sweeps
 //   vmc_host_selftest select <b_min> <b e>... the optimiser's choice
+//   vmc_host_selftest wavefunction <n> <L> <a> <b> <file of 3n positions>
+//                                            ln psi, E_L, grad/lap (needs a GPU)
 #include "blocking_analysis/blocking_analysis.hpp"
 #include "config/config.hpp"
+#include "energy_tracking/energy_tracking.hpp"
 #include "optimizer/jastrow_optimizer.hpp"
+#include "particles/particles.hpp"
+#include "wavefunction/wavefunction.hpp"
 #include "output_writer/output_writer.hpp"
 
 #include <cstdlib>
@@ -93,6 +98,29 @@ int select_cmd(int argc, char **argv) {
     return 0;
 }
 
+int wavefunction_cmd(char **argv) {
+    const std::size_t n = std::strtoull(argv[2], nullptr, 10);
+    const double L = std::strtod(argv[3], nullptr), a = std::strtod(argv[4], nullptr),
+                 b = std::strtod(argv[5], nullptr);
+    std::ifstream in{argv[6]};
+    Particles p{n};
+    for (std::size_t i = 0; i < n; ++i) in >> p.pos_x_get()[i] >> p.pos_y_get()[i] >> p.pos_z_get()[i];
+    WaveFunction wf{p, L, a, b};
+    const double lp = wf.evaluate_log_psi(p);
+    wf.evaluate_derivatives(p);
+    EnergyTracker et{L, static_cast<double>(n)};
+    et.initialize_structure_factors(p);
+    et.initialize_reciprocal_energy();
+    et.initialize_real_energy(p);
+    const double e = et.eval_total_energy(p);
+    std::cout << std::setprecision(std::numeric_limits<double>::max_digits10) << "log_psi=" << lp
+              << "\ne_local=" << e << "\n";
+    for (std::size_t i = 0; i < n; ++i)
+        std::cout << "gl=" << p.grad_log_psi_x_get()[i] << " " << p.grad_log_psi_y_get()[i] << " "
+                  << p.grad_log_psi_z_get()[i] << " " << p.lap_log_psi_get()[i] << "\n";
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -102,6 +130,7 @@ int main(int argc, char **argv) {
         if (argc >= 3 && !std::strcmp(argv[1], "blocking")) return blocking_cmd(argc, argv);
         if (argc >= 3 && !std::strcmp(argv[1], "scan")) return scan_cmd(argv[2]);
         if (argc >= 5 && !std::strcmp(argv[1], "select")) return select_cmd(argc, argv);
+        if (argc >= 7 && !std::strcmp(argv[1], "wavefunction")) return wavefunction_cmd(argv);
         std::cerr << "usage: vmc_host_selftest config <path> | write <fmt> <out> | blocking <bs> <x...>\n";
         return 2;
     } catch (const std::exception &e) {
