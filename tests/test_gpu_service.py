@@ -24,10 +24,14 @@ def test_autotune_picks_a_working_setting(vmc):
     e.sweep(3, measure=True)
     assert np.all(np.isfinite(e.stats()[:, vmc.STAT["LAST_E"]]))
     e.close()
-    # A system without a register kernel (N = 66, 33 per spin) tunes the
-    # generic kernel's inverse placement.
+    # N = 66 (33 per spin) has the multi-wavefront register kernel, which
+    # should win too; N = 186 (93 per spin) has no register instantiation,
+    # so the tuner picks among the generic kernel's placements and workgroups.
     t = vmc.autotune(vmc.System.heg(66, 2.0, 1.0, precision=vmc.PREC_MIXED), 64, drift_tol=1e-2, sweeps=2)
+    assert t["kernel"] == vmc.KERNEL_REGISTER
+    t = vmc.autotune(vmc.System.heg(186, 3.0, 1.0, precision=vmc.PREC_MIXED), 16, drift_tol=1e-2, sweeps=1)
     assert t["kernel"] == vmc.KERNEL_GENERIC and t["inv_placement"] in (1, 2)
+    assert t["walker_threads"] in (64, 128, 256)
 
 
 REQUESTS = [  # (N, r_s, b, walkers, warmup, measure, seed)

@@ -283,3 +283,62 @@ def test_generic_multi_wavefront_mixed_with_reinversion(vmc, oracle):
         e, kin, _ = wk.local_energy()
         scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
         assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale
+
+
+RRW_SYSTEMS = [
+    ("heg_N66_rs2", lambda V: V.System.heg(66, 2.0, 1.0)),
+    ("heg_N114_rs5", lambda V: V.System.heg(114, 5.0, 0.8)),
+    ("heg_N162_rs3", lambda V: V.System.heg(162, 3.0, 1.2)),
+    ("ref_N57", lambda V: V.System.reference(57, 12.0)),
+]
+
+
+@pytest.mark.parametrize("name,factory", RRW_SYSTEMS, ids=[s[0] for s in RRW_SYSTEMS])
+def test_multiwave_register_kernel_follows_oracle_twin(vmc, oracle, name, factory):
+    """Register-resident rows with one wavefront per spin (two for 81
+    electrons per spin) follow the oracle's FP64 twin move for move, like the
+    other kernels."""
+    s = factory(vmc)
+    W, seed, sweeps = 2, 31337, 2
+    step = 0.05 * s.box_length
+    ens = vmc.Ensemble(s, W, seed, step0=step)
+    ens.set_kernel(vmc.KERNEL_REGISTER)
+    assert ens.kernel() == vmc.KERNEL_REGISTER
+    ens.init_positions()
+    osys = oracle_system(oracle, s)
+    walkers = [oracle.Walker(osys, oracle.init_positions(osys, seed, w)[0]) for w in range(W)]
+    acc = np.zeros(W)
+    for sweep in range(sweeps):
+        ens.sweep(1, measure=True)
+        st, pos = ens.stats(), ens.positions()
+        for w in range(W):
+            a, _, e = walkers[w].sweep(seed, w, sweep, step, move_kind=s.move_kind, adapt=False,
+                                       measure=True)
+            acc[w] += a
+            np.testing.assert_allclose(pos[w], walkers[w].positions(), rtol=0, atol=1e-9)
+            assert st[w, vmc.STAT["ACCEPTED"]] == acc[w]
+            assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-9 * max(abs(e), 1e-3 * s.N)
+
+
+def test_multiwave_register_kernel_mixed_reinversion(vmc, oracle):
+    """N = 114 mixed precision on the multi-wavefront register kernel with
+    FP64 re-inversion every 64 accepted moves: drift recorded, local energy
+    within the mixed budget of the FP64 oracle, and statistically the same
+    chain as the generic kernel."""
+    s = vmc.System.heg(114, 5.0, 0.8, precision=1, reinvert_interval=64)
+    W = 32
+    ens = vmc.Ensemble(s, W, 57, step0=1.5)
+    ens.set_kernel(vmc.KERNEL_REGISTER)
+    ens.init_positions()
+    ens.sweep(8, measure=True, adapt=True)
+    st = ens.stats()
+    assert np.all(st[:, vmc.STAT["REINV"]] >= 3)
+    d = ens.drift()
+    assert np.all(d > 0) and np.all(d < 1e-3), d.max()
+    pos = ens.positions()
+    osys = oracle_system(oracle, s)
+    for w in range(0, W, 8):
+        wk = oracle.Walker(osys, pos[w])
+        e, kin, _ = wk.local_energy()
+        scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
+        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale

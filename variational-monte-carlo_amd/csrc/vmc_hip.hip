@@ -472,6 +472,18 @@ __global__ __launch_bounds__(kReinvThreads) void k_reinv_batch(const BatchItem *
     reinv_walker<Real>(it.s, it.st, g - it.walker_begin, a_in_lds, smem);
 }
 
+void reinv_generic_launch(const DevSys &d, const DevState &st, unsigned grid, hipStream_t stream,
+                          int real_bytes) {
+    const bool a_lds = reinv_lds_bytes(d.nmax, true) <= kMaxLdsBytes;
+    const size_t lds = reinv_lds_bytes(d.nmax, a_lds);
+    if (real_bytes == 8)
+        hipLaunchKernelGGL((k_reinv_generic<double>), dim3(grid), dim3(kReinvThreads), lds, stream, d, st,
+                           a_lds ? 1 : 0);
+    else
+        hipLaunchKernelGGL((k_reinv_generic<float>), dim3(grid), dim3(kReinvThreads), lds, stream, d, st,
+                           a_lds ? 1 : 0);
+}
+
 // Frozen-configuration evaluation (one configuration per wavefront).
 template <typename Real, bool kInvLds>
 __global__ __launch_bounds__(kWave) void k_frozen(DevSys s, int C, const double *pos,
@@ -756,6 +768,7 @@ int launch_init(vmc_ensemble *e, int draw, hipStream_t stream) {
 }
 
 bool rr_supported(const DevSys &d) { return rr_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng); }
+bool rrw_supported(const DevSys &d) { return rrw_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng); }
 
 template <typename Real>
 int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipStream_t stream) {
@@ -766,6 +779,17 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
         const bool ok = sizeof(Real) == 4
                             ? rr_launch_f32(d, e->st, grid, n_sweeps, s0, flags, block_size, e->rr_variant, stream, e->timer)
                             : rr_launch_f64(d, e->st, grid, n_sweeps, s0, flags, block_size, e->rr_variant, stream, e->timer);
+        if (!ok) return fail(VMC_ERR_INVALID, "no register-resident instantiation for this system");
+        VMC_HIP_TRY(hipGetLastError());
+        return VMC_OK;
+    }
+    if (e->kernel != VMC_KERNEL_GENERIC && rrw_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng)) {
+        const unsigned grid = static_cast<unsigned>(e->W);
+        const unsigned long long s0 = (unsigned long long)e->sweeps_done;
+        const ReinvLaunch rl{static_cast<int>(sizeof(Real))};
+        const bool ok = sizeof(Real) == 4
+                            ? rrw_launch_f32(d, e->st, grid, n_sweeps, s0, flags, block_size, stream, e->timer, rl)
+                            : rrw_launch_f64(d, e->st, grid, n_sweeps, s0, flags, block_size, stream, e->timer, rl);
         if (!ok) return fail(VMC_ERR_INVALID, "no register-resident instantiation for this system");
         VMC_HIP_TRY(hipGetLastError());
         return VMC_OK;
@@ -971,8 +995,8 @@ int vmc_ensemble_set_kernel(vmc_ensemble *e, int32_t kernel) {
     if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
     if (kernel != VMC_KERNEL_AUTO && kernel != VMC_KERNEL_GENERIC && kernel != VMC_KERNEL_REGISTER)
         return fail(VMC_ERR_INVALID, "unknown kernel %d", kernel);
-    if (kernel == VMC_KERNEL_REGISTER && !rr_supported(e->tables.dev))
-        return fail(VMC_ERR_INVALID, "register kernel needs 7, 19 or 27 electrons per spin");
+    if (kernel == VMC_KERNEL_REGISTER && !rr_supported(e->tables.dev) && !rrw_supported(e->tables.dev))
+        return fail(VMC_ERR_INVALID, "register kernel needs 7, 19, 27, 33, 57 or 81 electrons per spin");
     e->kernel = kernel;
     return VMC_OK;
 }
@@ -1003,8 +1027,9 @@ int vmc_ensemble_set_walker_threads(vmc_ensemble *e, int32_t threads) {
 
 int vmc_ensemble_get_kernel(const vmc_ensemble *e, int32_t *kernel) {
     if (!e || !kernel) return fail(VMC_ERR_INVALID, "NULL argument");
-    *kernel = (e->kernel != VMC_KERNEL_GENERIC && rr_supported(e->tables.dev)) ? VMC_KERNEL_REGISTER
-                                                                               : VMC_KERNEL_GENERIC;
+    *kernel = (e->kernel != VMC_KERNEL_GENERIC && (rr_supported(e->tables.dev) || rrw_supported(e->tables.dev)))
+                  ? VMC_KERNEL_REGISTER
+                  : VMC_KERNEL_GENERIC;
     return VMC_OK;
 }
 

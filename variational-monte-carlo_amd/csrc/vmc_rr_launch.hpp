@@ -58,6 +58,33 @@ inline bool rr_supported_sizes(int ndet, int n_up, int n_dn, int ng) {
     return n_ok && ng <= 12 * kWave;
 }
 
+// Spins of 33, 57 or 81 electrons (equal spins or polarised) have the
+// multi-wavefront register-resident instantiations (one wavefront per spin, two
+// for 81), structure factor over at most 768 G vectors.
+inline bool rrw_supported_sizes(int ndet, int n_up, int n_dn, int ng) {
+    const bool n_ok = n_up == 33 || n_up == 57 || n_up == 81;
+    if (ndet == 2) return n_ok && n_dn == n_up && ng <= 12 * kWave;
+    return n_ok && ng <= 12 * kWave;
+}
+
+// The FP64 re-inversion kernel of the generic path, callable from the
+// register-resident launchers (defined with the generic kernels).
+void reinv_generic_launch(const DevSys &d, const DevState &st, unsigned grid, hipStream_t stream,
+                          int real_bytes);
+struct ReinvLaunch {
+    int real_bytes;
+    void launch(const DevSys &d, const DevState &st, unsigned grid, hipStream_t s) const {
+        reinv_generic_launch(d, st, grid, s, real_bytes);
+    }
+};
+
+bool rrw_launch_f32(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,
+                    unsigned long long s0, int flags, int bs, hipStream_t stream, KernelTimer &timer,
+                    const ReinvLaunch &reinv);
+bool rrw_launch_f64(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,
+                    unsigned long long s0, int flags, int bs, hipStream_t stream, KernelTimer &timer,
+                    const ReinvLaunch &reinv);
+
 // Launches n_sweeps sweeps (and, with VMC_SWEEP_MEASURE, one measurement per
 // sweep).  variant 0: no occupancy bound; 1: at most 128 VGPRs.
 bool rr_launch_f32(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,

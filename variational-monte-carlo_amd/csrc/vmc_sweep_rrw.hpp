@@ -1,0 +1,490 @@
+// vmc_sweep_rrw.hpp -- register-resident sweep and local energy for spins of
+// 33 to 128 electrons: one wavefront (or two) per spin.
+//
+// The workgroup of a walker holds WPS wavefronts per determinant; thread
+// (wave v, lane l) owns row r = (v mod WPS) * 64 + l of spin h = v / WPS:
+// that electron's position, its Jastrow row sum U and its row of the spin's
+// transposed inverse in registers.  A move of electron p (spin hp) is three
+// phases separated by two workgroup barriers:
+//   A  every thread forms its new pair term against the proposal (DPP wave
+//      sum -> LDS); the rows of spin hp form the orbital row of the proposal;
+//   B  spin hp's rows form s_k = sum_m phi_m invT[k][m]; the mover publishes
+//      R = s_p, its U, old position and inverse row;
+//   C  every thread takes the same accept decision from the same LDS values;
+//      on accept spin hp's rows apply Sherman-Morrison (one FMA per element,
+//      the other spin's wavefronts skip it on a wave-uniform branch) and every
+//      thread updates its U.
+// Same chain as the one-wavefront kernel (the reference's
+// src/simulation/simulation.cpp:80-156 with per-sweep electron order and
+// Philox draws), same HBM layout, so ensembles switch kernels freely.
+#pragma once
+
+#include "vmc_sweep_rr.hpp"
+
+namespace vmc {
+
+template <typename Real, int NU, int ND, int WPS>
+struct RRWShared {
+    static constexpr int N = NU + ND;
+    static constexpr int NMAX = NU > ND ? NU : ND;
+    static constexpr int NWAVE = (ND > 0 ? 2 : 1) * WPS;
+    double pos[3][N];
+    double prop[4][N];   // this sweep's proposals (x, y, z) and accept uniforms
+    Real pph[6][N];      // axis phasors of the proposals
+    Real tab[kRRChunk * 3 * (kRRMaxPow + 1) * 2];
+    Real phi[NMAX];      // orbital row of the pending move
+    Real row[NMAX];      // inverse row of the mover
+    double red[NWAVE];   // per-wave partial sums
+    double red4[NWAVE][4];
+    double mv[8];        // mover: R, U, x, y, z
+    double stat[kStatCount];
+    double misc[4];      // sigma
+};
+
+template <typename Real, int NU, int ND, int WPS>
+struct RRW {
+    static constexpr int N = NU + ND;
+    static constexpr int NMAX = NU > ND ? NU : ND;
+    static constexpr int NWAVE = (ND > 0 ? 2 : 1) * WPS;
+    static constexpr int NT = NWAVE * kWave;
+    static constexpr auto orb = rr::shell_orbitals<NMAX>();
+    using Sh = RRWShared<Real, NU, ND, WPS>;
+    static_assert(ND == 0 || ND == NU, "polarised or equal spins");
+    static_assert(NMAX <= WPS * kWave, "rows per spin exceed the spin's wavefronts");
+    static_assert(rr::max_component<NMAX>() <= 2, "orbital phasors use powers up to 2");
+
+    // Orbital r of the compile-time table, packed (nx+4) | (ny+4)<<4 |
+    // (nz+4)<<8 | type<<12 (selected once per kernel).
+    __device__ static int packed_orbital(int r) {
+        int o = 0;
+#pragma unroll
+        for (int m = 0; m < NMAX; ++m)
+            if (r == m) o = (orb[m].x + 4) | ((orb[m].y + 4) << 4) | ((orb[m].z + 4) << 8) | (orb[m].t << 12);
+        return o;
+    }
+
+    struct Thread {
+        int v, lane, h, r, nh, e;
+        bool has_e;
+        __device__ Thread() {
+            const int t = threadIdx.x;
+            v = t >> 6;
+            lane = t & (kWave - 1);
+            h = ND > 0 ? v / WPS : 0;
+            r = (v % WPS) * kWave + lane;
+            nh = h ? ND : NU;
+            has_e = r < nh;
+            e = h ? NU + r : r;
+        }
+    };
+
+    // Workgroup sum of one value per thread (fixed order: every thread gets
+    // the same double).
+    __device__ static double block_total(double x, double *red) {
+        const double wsum = rr::wave_total(x);
+        const int v = threadIdx.x >> 6;
+        if ((threadIdx.x & (kWave - 1)) == 0) red[v] = wsum;
+        __syncthreads();
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < NWAVE; ++i) t += red[i];
+        __syncthreads();
+        return t;
+    }
+
+    __device__ static void load(const DevSys &s, const DevState &st, Sh &sh, const Thread &T,
+                                Real (&inv)[NMAX], double &x, double &y, double &z) {
+        const int w = blockIdx.x;
+        const Real *blk = reinterpret_cast<const Real *>(st.inv) + (size_t)w * s.ninv;
+        const double *gp = st.pos + (size_t)w * 3 * N;
+#pragma unroll
+        for (int m = 0; m < NMAX; ++m)
+            inv[m] = (T.has_e && m < T.nh) ? blk[(T.h ? NU * NU : 0) + T.r * T.nh + m] : (Real)0;
+        x = T.has_e ? gp[T.e] : 0.0;
+        y = T.has_e ? gp[N + T.e] : 0.0;
+        z = T.has_e ? gp[2 * N + T.e] : 0.0;
+        if (T.has_e) {
+            sh.pos[0][T.e] = x;
+            sh.pos[1][T.e] = y;
+            sh.pos[2][T.e] = z;
+        }
+    }
+
+    // Jastrow row sum of the thread's electron against every other.
+    __device__ static double jastrow_row(const RRConst<Real> &c, const Sh &sh, const Thread &T,
+                                         double x, double y, double z) {
+        double U = 0.0;
+        if (!T.has_e) return U;
+        for (int j = 0; j < N; ++j) {
+            if (j == T.e) continue;
+            const Real dx = c.mi_r(x, sh.pos[0][j]);
+            const Real dy = c.mi_r(y, sh.pos[1][j]);
+            const Real dz = c.mi_r(z, sh.pos[2][j]);
+            const Real rr_ = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+            const Real a = ((ND > 0 && j >= NU) ? 1 : 0) == T.h ? c.a_same : c.a_anti;
+            U += (double)(a * rr_ * wrcp<Real>((Real)1 + c.b * rr_));
+        }
+        return U;
+    }
+
+    // One move of electron p.  Returns 1 if accepted (uniform).
+    __device__ static int move(const RRConst<Real> &c, Sh &sh, const Thread &T, Real (&inv)[NMAX],
+                               double &x, double &y, double &z, double &U, int ob, int p) {
+        const int hp = (ND > 0 && p >= NU) ? 1 : 0;
+        const int rp = hp ? p - NU : p;
+        const int NP = hp ? ND : NU;
+        // Phase A.
+        const double xn = sh.prop[0][p], yn = sh.prop[1][p], zn = sh.prop[2][p];
+        if (T.h == hp && T.has_e) {
+            rr::Cx<Real> a[3], a2[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                a[k].re = sh.pph[2 * k][p];
+                a[k].im = sh.pph[2 * k + 1][p];
+                a2[k] = rr::cmul(a[k], a[k]);
+            }
+            const rr::Cx<Real> q = rr::orbital_phasor<Real>(orb_nx(ob), orb_ny(ob), orb_nz(ob), a, a2);
+            sh.phi[T.r] = orb_t(ob) ? q.im : q.re;
+        }
+        Real un = (Real)0;
+        if (T.has_e && T.e != p) {
+            const Real dx = c.mi_r(xn, x);
+            const Real dy = c.mi_r(yn, y);
+            const Real dz = c.mi_r(zn, z);
+            const Real rn = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+            const Real a = (T.h == hp) ? c.a_same : c.a_anti;
+            un = a * rn * wrcp<Real>((Real)1 + c.b * rn);
+        }
+        {
+            const Real ws = rr::wave_total(un);
+            if (T.lane == 0) sh.red[T.v] = (double)ws;
+        }
+        __syncthreads();
+        // Phase B.
+        Real sk = (Real)0;
+        if (T.h == hp) {
+            Real s0 = 0, s1 = 0;
+#pragma unroll
+            for (int m = 0; m < NMAX; m += 2) {
+                s0 += sh.phi[m] * inv[m];
+                if (m + 1 < NMAX) s1 += sh.phi[m + 1] * inv[m + 1];
+            }
+            sk = s0 + s1;
+            if (T.r == rp) {
+                sh.mv[0] = (double)sk;
+                sh.mv[1] = U;
+                sh.mv[2] = x;
+                sh.mv[3] = y;
+                sh.mv[4] = z;
+#pragma unroll
+                for (int m = 0; m < NMAX; ++m) sh.row[m] = inv[m];
+            }
+        }
+        double jn = 0.0;
+#pragma unroll
+        for (int i = 0; i < NWAVE; ++i) jn += sh.red[i];
+        __syncthreads();
+        // Phase C: the same decision in every thread (simulation.cpp:125-131),
+        // exact FP64 exponential within 1e-5 of the threshold.
+        const double R = sh.mv[0];
+        const double dj = jn - sh.mv[1];
+        const double ua = sh.prop[3][p];
+        const double r2 = R * R;
+        const float e32 = __builtin_amdgcn_exp2f((float)(2.0 * dj * 1.4426950408889634));
+        const double approx = r2 * (double)e32;
+        int acc;
+        if (!(approx == approx)) acc = 0;
+        else if (fabs(ua - approx) > 1e-5 * approx || !(fabs(dj) < 40.0)) acc = ua < approx;
+        else acc = ua < r2 * rr_exp_f64(2.0 * dj);
+        if (!wave_uniform(acc)) return 0;
+        if (T.h == hp) {
+            const Real inv_ratio = wrcp<Real>((Real)R);
+            Real f = (T.r == rp) ? (Real)1 - inv_ratio : sk * inv_ratio;
+            f = T.has_e ? f : (Real)0;
+#pragma unroll
+            for (int m = 0; m < NMAX; ++m) inv[m] -= sh.row[m] * f;
+        }
+        if (T.has_e && T.e != p) {
+            const Real dx = c.mi_r(sh.mv[2], x);
+            const Real dy = c.mi_r(sh.mv[3], y);
+            const Real dz = c.mi_r(sh.mv[4], z);
+            const Real ro = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+            const Real a = (T.h == hp) ? c.a_same : c.a_anti;
+            U += (double)un - (double)(a * ro * wrcp<Real>((Real)1 + c.b * ro));
+        }
+        if (T.has_e && T.e == p) {
+            U = jn;
+            x = xn;
+            y = yn;
+            z = zn;
+            sh.pos[0][p] = xn;
+            sh.pos[1][p] = yn;
+            sh.pos[2][p] = zn;
+        }
+        (void)NP;
+        return 1;
+    }
+};
+
+// Philox draws of one electron for this sweep into its proposal slots.
+template <typename Real>
+__device__ __forceinline__ void rrw_proposal(const DevSys &s, double sigma, double x0, double y0,
+                                             double z0, int e, int w, uint32_t sweep, int N,
+                                             double *prop, Real *pph) {
+    const uint4 rm = philox4x32_10(make_uint4((uint32_t)e, (uint32_t)w, sweep, kTagMove), s.key0, s.key1);
+    const uint4 ra = philox4x32_10(make_uint4((uint32_t)e, (uint32_t)w, sweep, kTagAccept), s.key0, s.key1);
+    double dx, dy, dz;
+    if (s.move_kind == 1) {
+        const double rho0 = sqrt(-2.0 * log(u01(rm.x)));
+        const double rho1 = sqrt(-2.0 * log(u01(rm.z)));
+        double s0, c0, s1, c1;
+        sincospi(2.0 * u01(rm.y), &s0, &c0);
+        sincospi(2.0 * u01(rm.w), &s1, &c1);
+        dx = rho0 * c0;
+        dy = rho0 * s0;
+        dz = rho1 * c1;
+    } else {
+        dx = 2.0 * u01(rm.x) - 1.0;
+        dy = 2.0 * u01(rm.y) - 1.0;
+        dz = 2.0 * u01(rm.z) - 1.0;
+    }
+    const double x = x0 + sigma * dx, y = y0 + sigma * dy, z = z0 + sigma * dz;
+    const double q[3] = {x - s.L * floor(x * s.inv_L), y - s.L * floor(y * s.inv_L),
+                         z - s.L * floor(z * s.inv_L)};
+    for (int k = 0; k < 3; ++k) {
+        prop[k * N + e] = q[k];
+        const rr::Cx<Real> ph = rr::axis_phasor<Real>(q[k], s.inv_L);
+        pph[(2 * k) * N + e] = ph.re;
+        pph[(2 * k + 1) * N + e] = ph.im;
+    }
+    prop[3 * N + e] = u01(ra.x);
+}
+
+template <typename Real, int NU, int ND, int WPS>
+__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_sweep_rrw(DevSys s, DevState st,
+                                                                          unsigned long long sweep0,
+                                                                          int flags) {
+    using K = RRW<Real, NU, ND, WPS>;
+    constexpr int N = K::N;
+    constexpr int NMAX = K::NMAX;
+    __shared__ typename K::Sh sh;
+    const RRConst<Real> cst(s);
+    const typename K::Thread T;
+    const int w = blockIdx.x;
+    Real inv[NMAX];
+    double x, y, z;
+    K::load(s, st, sh, T, inv, x, y, z);
+    if (threadIdx.x < kStatCount) sh.stat[threadIdx.x] = st.stats[(size_t)w * kStatCount + threadIdx.x];
+    if (threadIdx.x == 0) sh.misc[0] = st.sigma[w];
+    __syncthreads();
+    double U = K::jastrow_row(cst, sh, T, x, y, z);
+    const int ob = K::packed_orbital(T.r);
+    if (T.has_e)
+        rrw_proposal<Real>(s, sh.misc[0], x, y, z, T.e, w, (uint32_t)sweep0, N, &sh.prop[0][0], &sh.pph[0][0]);
+    __syncthreads();
+    int acc = 0;
+#pragma unroll 1
+    for (int p = 0; p < N; ++p) acc += K::move(cst, sh, T, inv, x, y, z, U, ob, p);
+    __syncthreads();
+    if (flags & VMC_SWEEP_ADAPT) {
+        const double f = exp(0.25 * ((double)acc / (double)N - 0.5));
+        if (threadIdx.x == 0) {
+            const double sigma = sh.misc[0] * f;
+            sh.misc[0] = sigma > 0.5 * s.L ? 0.5 * s.L : sigma;
+        }
+    }
+    Real *blk = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
+    if (T.has_e) {
+#pragma unroll
+        for (int m = 0; m < NMAX; ++m)
+            if (m < T.nh) blk[(T.h ? NU * NU : 0) + T.r * T.nh + m] = inv[m];
+        double *gq = st.pos + (size_t)w * 3 * N;
+        gq[T.e] = x;
+        gq[N + T.e] = y;
+        gq[2 * N + T.e] = z;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sh.stat[6] += (double)acc;
+        sh.stat[7] += (double)N;
+        st.sigma[w] = sh.misc[0];
+        st.since[w] += acc;
+    }
+    __syncthreads();
+    if (threadIdx.x < kStatCount) st.stats[(size_t)w * kStatCount + threadIdx.x] = sh.stat[threadIdx.x];
+}
+
+// Local energy after a sweep (energy_tracking.cpp:253-382 with the Jastrow
+// and Slater derivatives of jastrow_pade.cpp:57-138, slater_plane_wave.cpp:
+// 425-499): per thread its electron's pair terms and Slater terms; the
+// structure factor from per-electron axis-power tables staged in LDS, the
+// workgroup's threads owning G vectors.
+template <typename Real, int NU, int ND, int WPS>
+__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_measure_rrw(DevSys s, DevState st,
+                                                                            int block_size) {
+    using K = RRW<Real, NU, ND, WPS>;
+    constexpr int N = K::N;
+    constexpr int NMAX = K::NMAX;
+    constexpr int NT = K::NT;
+    constexpr int GPT = (kRRMaxGPerLane * kWave + NT - 1) / NT;  // G slots per thread
+    __shared__ typename K::Sh sh;
+    const RRConst<Real> c(s);
+    const typename K::Thread T;
+    const int w = blockIdx.x;
+    Real inv[NMAX];
+    double x, y, z;
+    K::load(s, st, sh, T, inv, x, y, z);
+    __syncthreads();
+    // Pair terms: Jastrow gradient / Laplacian and real-space Ewald.
+    double gx = 0, gy = 0, gz = 0, lap = 0, vreal = 0;
+    if (T.has_e) {
+        const Real m2b = (Real)-2 * c.b;
+        for (int j0 = 0; j0 < N; j0 += 8) {
+            Real gx_c = 0, gy_c = 0, gz_c = 0, l_c = 0, v_c = 0;
+#pragma unroll
+            for (int jj = 0; jj < 8; ++jj) {
+                const int j = j0 + jj;
+                if (j >= N || j == T.e) continue;
+                const Real dx = c.mi_r(x, sh.pos[0][j]);
+                const Real dy = c.mi_r(y, sh.pos[1][j]);
+                const Real dz = c.mi_r(z, sh.pos[2][j]);
+                const Real r = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+                const Real a = ((ND > 0 && j >= NU) ? 1 : 0) == T.h ? c.a_same : c.a_anti;
+                const Real den = wrcp<Real>((Real)1 + c.b * r);
+                v_c += c.ewald(r);
+                if (r < (Real)1e-12) continue;
+                const Real d1 = a * den * den;
+                const Real inv_r = wrcp<Real>(r);
+                const Real gf = d1 * inv_r;
+                gx_c += gf * dx;
+                gy_c += gf * dy;
+                gz_c += gf * dz;
+                l_c += d1 * (m2b * den + (Real)2 * inv_r);
+            }
+            gx += (double)gx_c;
+            gy += (double)gy_c;
+            gz += (double)gz_c;
+            lap += (double)l_c;
+            vreal += (double)v_c;
+        }
+        vreal *= 0.5;
+    }
+    // Slater gradient and Laplacian of ln|D_h| at the thread's electron.
+    double sx = 0, sy = 0, sz = 0, sl = 0;
+    if (T.has_e) {
+        rr::Cx<Real> a[3], a2[3];
+        a[0] = rr::axis_phasor<Real>(x, s.inv_L);
+        a[1] = rr::axis_phasor<Real>(y, s.inv_L);
+        a[2] = rr::axis_phasor<Real>(z, s.inv_L);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a2[k] = rr::cmul(a[k], a[k]);
+        const Real ku = (Real)s.kunit;
+#pragma unroll
+        for (int m = 0; m < NMAX; ++m) {
+            if (K::orb[m].t == 1) continue;
+            const rr::Cx<Real> ph = rr::orbital_phasor<Real>(K::orb[m].x, K::orb[m].y, K::orb[m].z, a, a2);
+            const Real k2 = ku * ku * (Real)(K::orb[m].x * K::orb[m].x + K::orb[m].y * K::orb[m].y + K::orb[m].z * K::orb[m].z);
+            const Real wc = inv[m];
+            Real gsc = -wc * ph.im;
+            Real lpc = -wc * ph.re;
+            if (m + 1 < NMAX && K::orb[m + 1].t == 1) {
+                const Real ws = inv[m + 1];
+                gsc += ws * ph.re;
+                lpc -= ws * ph.im;
+            }
+            sx += (double)(gsc * ku * (Real)K::orb[m].x);
+            sy += (double)(gsc * ku * (Real)K::orb[m].y);
+            sz += (double)(gsc * ku * (Real)K::orb[m].z);
+            sl += (double)(lpc * k2);
+        }
+    }
+    const double slap = sl - (sx * sx + sy * sy + sz * sz);
+    const double tx = sx + gx, ty = sy + gy, tz = sz + gz;
+    const double kin_t = T.has_e ? (slap + lap + tx * tx + ty * ty + tz * tz) : 0.0;
+    const double kinetic = -0.5 * K::block_total(kin_t, sh.red);
+    const double vr = K::block_total(T.has_e ? vreal : 0.0, sh.red);
+    // Structure factor S(G) = sum_j e^{i G.r_j}: threads own G vectors.
+    constexpr int P = kRRMaxPow + 1;
+    int gk[GPT];
+    Real sre[GPT], sim[GPT];
+#pragma unroll
+    for (int k = 0; k < GPT; ++k) {
+        const int g = (int)threadIdx.x + NT * k;
+        const int4 G = g < s.ng ? s.gvec[g] : make_int4(0, 0, 0, 0);
+        gk[k] = g < s.ng ? (G.x | ((G.y + 8) << 4) | ((G.z + 8) << 8)) : -1;
+        sre[k] = 0;
+        sim[k] = 0;
+    }
+    for (int c0 = 0; c0 < N; c0 += kRRChunk) {
+        const int cn = N - c0 < kRRChunk ? N - c0 : kRRChunk;
+        __syncthreads();
+        if (T.has_e && T.e >= c0 && T.e < c0 + cn) {
+            const int i = T.e - c0;
+            const double u[3] = {x, y, z};
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const rr::Cx<Real> a = rr::axis_phasor<Real>(u[ax], s.inv_L);
+                rr::Cx<Real> q{(Real)1, (Real)0};
+                Real *t = sh.tab + ((i * 3 + ax) * P) * 2;
+#pragma unroll
+                for (int m = 0; m < P; ++m) {
+                    t[2 * m] = q.re;
+                    t[2 * m + 1] = q.im;
+                    q = rr::cmul(q, a);
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = 0; i < cn; ++i) {
+            const Real *t = sh.tab + (i * 3 * P) * 2;
+#pragma unroll
+            for (int k = 0; k < GPT; ++k) {
+                if (gk[k] < 0) continue;
+                const int mx = gk[k] & 15, my = ((gk[k] >> 4) & 15) - 8, mz = ((gk[k] >> 8) & 15) - 8;
+                const int ay = my < 0 ? -my : my, az = mz < 0 ? -mz : mz;
+                const rr::Cx<Real> X{t[2 * mx], t[2 * mx + 1]};
+                rr::Cx<Real> Y{t[2 * (P + ay)], t[2 * (P + ay) + 1]};
+                rr::Cx<Real> Z{t[2 * (2 * P + az)], t[2 * (2 * P + az) + 1]};
+                if (my < 0) Y.im = -Y.im;
+                if (mz < 0) Z.im = -Z.im;
+                const rr::Cx<Real> ez = rr::cmul(rr::cmul(X, Y), Z);
+                sre[k] += ez.re;
+                sim[k] += ez.im;
+            }
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < GPT; ++k) {
+        const int g = (int)threadIdx.x + NT * k;
+        if (g < s.ng) acc += s.gw[g] * ((double)sre[k] * (double)sre[k] + (double)sim[k] * (double)sim[k]);
+    }
+    __syncthreads();
+    const double vrecip = s.pref * K::block_total(acc, sh.red);
+    if (threadIdx.x == 0) {
+        double *t = st.stats + (size_t)w * kStatCount;
+        const double en = kinetic + vr + vrecip + s.e_const;
+        const double pot = vr + vrecip + s.e_const;
+        t[0] += 1.0;
+        t[1] += en;
+        t[2] += en * en;
+        t[3] += kinetic;
+        t[4] += pot;
+        t[5] = en;
+        t[13] = kinetic;
+        t[14] = pot;
+        t[8] += en;
+        t[9] += 1.0;
+        if (t[9] >= (double)block_size) {
+            const double bm = t[8] / (double)block_size;
+            t[10] += 1.0;
+            const double delta = bm - t[11];
+            t[11] += delta / t[10];
+            t[12] += delta * (bm - t[11]);
+            t[8] = 0.0;
+            t[9] = 0.0;
+        }
+    }
+}
+
+}  // namespace vmc
