@@ -636,26 +636,30 @@ struct RRKernel {
         } else {
             acc = ua < r2 * rr_exp_f64(2.0 * dj);
         }
-        // Every input above is wave-uniform; say so, so the branch is scalar.
-        if (!wave_uniform(acc)) return 0;
-        // Accept (simulation.cpp:133-145).
+        // Every input above is wave-uniform; say so, so the branches are scalar.
+        const int accepted = wave_uniform(acc);
+        // Sherman-Morrison (simulation.cpp:133-145, slater_plane_wave.cpp:
+        // 367-423) as one FMA per element and lane: rows k != p of spin HP
+        // lose (s_k / R) row_p; row p itself (row_p is its own registers)
+        // loses (1 - 1/R) of itself, i.e. becomes row_p / R; every other lane
+        // subtracts 0 x (a finite row).  In FP32 the update runs on every move
+        // with f = 0 when rejected -- straight-line code keeps the rows in
+        // their registers; in FP64 (half-rate FMAs) only on accept.
+        if (sizeof(Real) == 8 && !accepted) return 0;
         if (lane == lp) {
 #pragma unroll
             for (int m = 0; m < NP; ++m) sh.row[m] = L.inv[m];
         }
-        const double xo = rr::readlane(L.x, lp), yo = rr::readlane(L.y, lp), zo = rr::readlane(L.z, lp);
         __syncthreads();
         {
-            // Sherman-Morrison as one FMA per element on every lane: rows
-            // k != p of spin HP lose (s_k / R) row_p; row p itself (row_p is
-            // its own registers) loses (1 - 1/R) of itself, i.e. becomes
-            // row_p / R; every other lane subtracts 0 x (a finite row).
             const Real inv_ratio = wrcp<Real>((Real)R);
             Real f = (r == rp) ? (Real)1 - inv_ratio : sk * inv_ratio;
-            f = (h == HP && r < NP) ? f : (Real)0;
+            f = (accepted && h == HP && r < NP) ? f : (Real)0;
 #pragma unroll
             for (int m = 0; m < NP; ++m) L.inv[m] -= sh.row[m] * f;
         }
+        if (!accepted) return 0;
+        const double xo = rr::readlane(L.x, lp), yo = rr::readlane(L.y, lp), zo = rr::readlane(L.z, lp);
         if (has_e && lane != lp) {
             const Real dx = c.mi_r(xo, L.x);
             const Real dy = c.mi_r(yo, L.y);
