@@ -342,3 +342,27 @@ def test_multiwave_register_kernel_mixed_reinversion(vmc, oracle):
         e, kin, _ = wk.local_energy()
         scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
         assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale
+
+
+@pytest.mark.parametrize("N", [246, 294])
+def test_multiwave_register_kernel_large_spins_mixed(vmc, oracle, N):
+    """123 and 147 electrons per spin (two and three wavefronts per spin, FP32
+    rows in registers): after sweeps with FP64 re-inversion the local energy
+    matches the FP64 oracle at the walker's positions to the mixed budget."""
+    s = vmc.System.heg(N, 4.0, 1.0, precision=1, reinvert_interval=128)
+    W = 4
+    ens = vmc.Ensemble(s, W, N, step0=1.2)
+    ens.set_kernel(vmc.KERNEL_REGISTER)
+    assert ens.kernel() == vmc.KERNEL_REGISTER
+    ens.init_positions()
+    ens.sweep(3, measure=True, adapt=True)
+    st = ens.stats()
+    assert np.all(st[:, vmc.STAT["REINV"]] >= 1)
+    assert np.all(ens.drift() < 1e-3)
+    pos = ens.positions()
+    osys = oracle_system(oracle, s)
+    for w in range(W):
+        wk = oracle.Walker(osys, pos[w])
+        e, kin, _ = wk.local_energy()
+        scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
+        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale, (w, st[w, vmc.STAT["LAST_E"]], e)

@@ -768,7 +768,9 @@ int launch_init(vmc_ensemble *e, int draw, hipStream_t stream) {
 }
 
 bool rr_supported(const DevSys &d) { return rr_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng); }
-bool rrw_supported(const DevSys &d) { return rrw_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng); }
+bool rrw_supported(const DevSys &d, int real_bytes) {
+    return rrw_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng, real_bytes);
+}
 
 template <typename Real>
 int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipStream_t stream) {
@@ -783,7 +785,7 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
         VMC_HIP_TRY(hipGetLastError());
         return VMC_OK;
     }
-    if (e->kernel != VMC_KERNEL_GENERIC && rrw_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng)) {
+    if (e->kernel != VMC_KERNEL_GENERIC && rrw_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng, (int)sizeof(Real))) {
         const unsigned grid = static_cast<unsigned>(e->W);
         const unsigned long long s0 = (unsigned long long)e->sweeps_done;
         const ReinvLaunch rl{static_cast<int>(sizeof(Real))};
@@ -995,8 +997,8 @@ int vmc_ensemble_set_kernel(vmc_ensemble *e, int32_t kernel) {
     if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
     if (kernel != VMC_KERNEL_AUTO && kernel != VMC_KERNEL_GENERIC && kernel != VMC_KERNEL_REGISTER)
         return fail(VMC_ERR_INVALID, "unknown kernel %d", kernel);
-    if (kernel == VMC_KERNEL_REGISTER && !rr_supported(e->tables.dev) && !rrw_supported(e->tables.dev))
-        return fail(VMC_ERR_INVALID, "register kernel needs 7, 19, 27, 33, 57 or 81 electrons per spin");
+    if (kernel == VMC_KERNEL_REGISTER && !rr_supported(e->tables.dev) && !rrw_supported(e->tables.dev, e->tables.real_bytes))
+        return fail(VMC_ERR_INVALID, "register kernel needs 7, 19, 27, 33, 57 or 81 electrons per spin (FP32 also 123, 147)");
     e->kernel = kernel;
     return VMC_OK;
 }
@@ -1027,7 +1029,7 @@ int vmc_ensemble_set_walker_threads(vmc_ensemble *e, int32_t threads) {
 
 int vmc_ensemble_get_kernel(const vmc_ensemble *e, int32_t *kernel) {
     if (!e || !kernel) return fail(VMC_ERR_INVALID, "NULL argument");
-    *kernel = (e->kernel != VMC_KERNEL_GENERIC && (rr_supported(e->tables.dev) || rrw_supported(e->tables.dev)))
+    *kernel = (e->kernel != VMC_KERNEL_GENERIC && (rr_supported(e->tables.dev) || rrw_supported(e->tables.dev, e->tables.real_bytes)))
                   ? VMC_KERNEL_REGISTER
                   : VMC_KERNEL_GENERIC;
     return VMC_OK;

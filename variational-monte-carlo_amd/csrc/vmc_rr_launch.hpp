@@ -60,9 +60,11 @@ inline bool rr_supported_sizes(int ndet, int n_up, int n_dn, int ng) {
 
 // Spins of 33, 57 or 81 electrons (equal spins or polarised) have the
 // multi-wavefront register-resident instantiations (one wavefront per spin, two
-// for 81), structure factor over at most 768 G vectors.
-inline bool rrw_supported_sizes(int ndet, int n_up, int n_dn, int ng) {
-    const bool n_ok = n_up == 33 || n_up == 57 || n_up == 81;
+// for 81), and in FP32 also 123 and 147 (two and three wavefronts); structure
+// factor over at most 768 G vectors.
+inline bool rrw_supported_sizes(int ndet, int n_up, int n_dn, int ng, int real_bytes) {
+    const bool n_ok = n_up == 33 || n_up == 57 || n_up == 81 ||
+                      (real_bytes == 4 && (n_up == 123 || n_up == 147));
     if (ndet == 2) return n_ok && n_dn == n_up && ng <= 12 * kWave;
     return n_ok && ng <= 12 * kWave;
 }

@@ -38,17 +38,25 @@ template <typename Real>
 bool rrw_launch(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,
                 unsigned long long s0, int flags, int bs, hipStream_t stream, KernelTimer &timer,
                 const ReinvLaunch &reinv) {
-    if (!rrw_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng)) return false;
+    if (!rrw_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng, (int)sizeof(Real))) return false;
 #define VMC_RRW(NU_, ND_, W_) \
     rrw_launch_g<Real, NU_, ND_, W_>(d, st, grid, n_sweeps, s0, flags, bs, stream, timer, reinv)
     if (d.ndet == 2) {
         if (d.n_up == 33) VMC_RRW(33, 33, 1);
         else if (d.n_up == 57) VMC_RRW(57, 57, 1);
-        else VMC_RRW(81, 81, 2);
+        else if (d.n_up == 81) VMC_RRW(81, 81, 2);
+        else if constexpr (sizeof(Real) == 4) {
+            if (d.n_up == 123) VMC_RRW(123, 123, 2);
+            else VMC_RRW(147, 147, 3);
+        }
     } else {
         if (d.n_up == 33) VMC_RRW(33, 0, 1);
         else if (d.n_up == 57) VMC_RRW(57, 0, 1);
-        else VMC_RRW(81, 0, 2);
+        else if (d.n_up == 81) VMC_RRW(81, 0, 2);
+        else if constexpr (sizeof(Real) == 4) {
+            if (d.n_up == 123) VMC_RRW(123, 0, 2);
+            else VMC_RRW(147, 0, 3);
+        }
     }
 #undef VMC_RRW
     return true;

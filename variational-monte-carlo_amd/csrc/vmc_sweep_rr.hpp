@@ -143,6 +143,17 @@ __device__ __forceinline__ Cx<Real> cpow2(int n, Cx<Real> a1, Cx<Real> a2) {
     return r;
 }
 
+// a^n for |n| <= 3 given a, a^2 and a^3 (conjugate for negative n).
+template <typename Real>
+__device__ __forceinline__ Cx<Real> cpow3(int n, Cx<Real> a1, Cx<Real> a2, Cx<Real> a3) {
+    const int m = n < 0 ? -n : n;
+    Cx<Real> r;
+    r.re = m == 0 ? (Real)1 : (m == 1 ? a1.re : (m == 2 ? a2.re : a3.re));
+    r.im = m == 0 ? (Real)0 : (m == 1 ? a1.im : (m == 2 ? a2.im : a3.im));
+    r.im = n < 0 ? -r.im : r.im;
+    return r;
+}
+
 // Axis phasor e^{2 pi i u} of a coordinate in [0, L).
 template <typename Real>
 __device__ __forceinline__ Cx<Real> axis_phasor(double x, double inv_L) {
@@ -175,6 +186,14 @@ template <typename Real>
 __device__ __forceinline__ Cx<Real> orbital_phasor(int nx, int ny, int nz, const Cx<Real> a[3],
                                                    const Cx<Real> a2[3]) {
     return cmul(cmul(cpow2(nx, a[0], a2[0]), cpow2(ny, a[1], a2[1])), cpow2(nz, a[2], a2[2]));
+}
+
+// Orbital phasor for components up to 3 (a3[c] = a[c]^3).
+template <typename Real>
+__device__ __forceinline__ Cx<Real> orbital_phasor3(int nx, int ny, int nz, const Cx<Real> a[3],
+                                                    const Cx<Real> a2[3], const Cx<Real> a3[3]) {
+    return cmul(cmul(cpow3(nx, a[0], a2[0], a3[0]), cpow3(ny, a[1], a2[1], a3[1])),
+                cpow3(nz, a[2], a2[2], a3[2]));
 }
 
 template <typename Real>

@@ -1,5 +1,5 @@
 // vmc_sweep_rrw.hpp -- register-resident sweep and local energy for spins of
-// 33 to 128 electrons: one wavefront (or two) per spin.
+// 33 to 192 electrons: one to three wavefronts per spin.
 //
 // The workgroup of a walker holds WPS wavefronts per determinant; thread
 // (wave v, lane l) owns row r = (v mod WPS) * 64 + l of spin h = v / WPS:
@@ -51,7 +51,23 @@ struct RRW {
     using Sh = RRWShared<Real, NU, ND, WPS>;
     static_assert(ND == 0 || ND == NU, "polarised or equal spins");
     static_assert(NMAX <= WPS * kWave, "rows per spin exceed the spin's wavefronts");
-    static_assert(rr::max_component<NMAX>() <= 2, "orbital phasors use powers up to 2");
+    static constexpr int kMaxC = rr::max_component<NMAX>();
+    static_assert(kMaxC <= 3, "orbital phasors use powers up to 3");
+
+    // e^{2 pi i n.u} from the axis phasors a and their powers.
+    __device__ static rr::Cx<Real> phasor(int nx, int ny, int nz, const rr::Cx<Real> a[3]) {
+        rr::Cx<Real> a2[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a2[k] = rr::cmul(a[k], a[k]);
+        if constexpr (kMaxC <= 2) {
+            return rr::orbital_phasor<Real>(nx, ny, nz, a, a2);
+        } else {
+            rr::Cx<Real> a3[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) a3[k] = rr::cmul(a2[k], a[k]);
+            return rr::orbital_phasor3<Real>(nx, ny, nz, a, a2, a3);
+        }
+    }
 
     // Orbital r of the compile-time table, packed (nx+4) | (ny+4)<<4 |
     // (nz+4)<<8 | type<<12 (selected once per kernel).
@@ -136,14 +152,13 @@ struct RRW {
         // Phase A.
         const double xn = sh.prop[0][p], yn = sh.prop[1][p], zn = sh.prop[2][p];
         if (T.h == hp && T.has_e) {
-            rr::Cx<Real> a[3], a2[3];
+            rr::Cx<Real> a[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 a[k].re = sh.pph[2 * k][p];
                 a[k].im = sh.pph[2 * k + 1][p];
-                a2[k] = rr::cmul(a[k], a[k]);
             }
-            const rr::Cx<Real> q = rr::orbital_phasor<Real>(orb_nx(ob), orb_ny(ob), orb_nz(ob), a, a2);
+            const rr::Cx<Real> q = phasor(orb_nx(ob), orb_ny(ob), orb_nz(ob), a);
             sh.phi[T.r] = orb_t(ob) ? q.im : q.re;
         }
         Real un = (Real)0;
@@ -372,17 +387,15 @@ __global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_measure_rrw(Dev
     // Slater gradient and Laplacian of ln|D_h| at the thread's electron.
     double sx = 0, sy = 0, sz = 0, sl = 0;
     if (T.has_e) {
-        rr::Cx<Real> a[3], a2[3];
+        rr::Cx<Real> a[3];
         a[0] = rr::axis_phasor<Real>(x, s.inv_L);
         a[1] = rr::axis_phasor<Real>(y, s.inv_L);
         a[2] = rr::axis_phasor<Real>(z, s.inv_L);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) a2[k] = rr::cmul(a[k], a[k]);
         const Real ku = (Real)s.kunit;
 #pragma unroll
         for (int m = 0; m < NMAX; ++m) {
             if (K::orb[m].t == 1) continue;
-            const rr::Cx<Real> ph = rr::orbital_phasor<Real>(K::orb[m].x, K::orb[m].y, K::orb[m].z, a, a2);
+            const rr::Cx<Real> ph = K::phasor(K::orb[m].x, K::orb[m].y, K::orb[m].z, a);
             const Real k2 = ku * ku * (Real)(K::orb[m].x * K::orb[m].x + K::orb[m].y * K::orb[m].y + K::orb[m].z * K::orb[m].z);
             const Real wc = inv[m];
             Real gsc = -wc * ph.im;
