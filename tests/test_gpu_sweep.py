@@ -7,6 +7,8 @@ rounding until an accept test lands within rounding of its threshold, which
 over a few sweeps does not happen for these seeds: positions, accept counts
 and local energies are compared move for move.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -366,3 +368,46 @@ def test_multiwave_register_kernel_large_spins_mixed(vmc, oracle, N):
         e, kin, _ = wk.local_energy()
         scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
         assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale, (w, st[w, vmc.STAT["LAST_E"]], e)
+
+
+NS_SYSTEMS = [
+    ("heg_N66_rs2_register", lambda V: V.System.heg(66, 2.0, 1.0, precision=1, reinvert_interval=16), 2),
+    ("heg_N114_rs5_register", lambda V: V.System.heg(114, 5.0, 0.8, precision=1, reinvert_interval=16), 2),
+    ("heg_N162_rs3_register", lambda V: V.System.heg(162, 3.0, 1.2, precision=1, reinvert_interval=16), 2),
+    ("heg_N114_rs5_generic", lambda V: V.System.heg(114, 5.0, 0.8, precision=1, reinvert_interval=16), 1),
+    ("ref_N57_generic", lambda V: dataclasses.replace(V.System.reference(57, 12.0, precision=1),
+                                                      reinvert_interval=16), 1),
+]
+
+
+@pytest.mark.parametrize("name,factory,kernel", NS_SYSTEMS, ids=[s[0] for s in NS_SYSTEMS])
+def test_matrix_core_reinversion_reaches_fp32_rounding(vmc, oracle, name, factory, kernel):
+    """FP32 inverses of 33 to 81 electrons per spin are re-inverted by
+    Newton-Schulz steps on the FP64 matrix cores.  With K = 16 every walker
+    re-inverts after every sweep, and the local energy is measured right
+    after that, so its error against the FP64 oracle shows the quality of the
+    re-inverted inverse: within 1e-6 of the sum of the magnitudes of its
+    parts (the FP32 working set's own rounding is about 1e-7 there), ten
+    times inside the mixed-precision budget.  A wrong or unconverged inverse
+    (drift 1e-4 or more) misses this bound."""
+    s = factory(vmc)
+    W = 8
+    ens = vmc.Ensemble(s, W, 4242, step0=0.3 * s.box_length / 6)
+    ens.set_kernel(kernel)
+    assert ens.kernel() == kernel
+    ens.init_positions()
+    ens.sweep(3, measure=True, adapt=True)
+    st = ens.stats()
+    assert np.all(st[:, vmc.STAT["REINV"]] == 3), st[:, vmc.STAT["REINV"]]
+    d = ens.drift()
+    assert np.all(d > 0) and np.all(d < 1e-3), d
+    pos = ens.positions()
+    osys = oracle_system(oracle, s)
+    worst = 0.0
+    for w in range(W):
+        wk = oracle.Walker(osys, pos[w])
+        e, kin, _ = wk.local_energy()
+        scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
+        worst = max(worst, abs(st[w, vmc.STAT["LAST_E"]] - e) / scale)
+    print(f"{name}: E_L error / parts max {worst:.2e}, drift max {d.max():.2e}")
+    assert worst <= 1e-6, worst

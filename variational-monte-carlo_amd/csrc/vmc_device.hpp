@@ -95,6 +95,99 @@ __device__ inline double block_sum(double v) {
     return t;
 }
 
+// Block-wide max of v (NaN wins); every thread gets the result.
+__device__ inline double block_max(double v, double *red) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, kWave);
+        v = (o > v || o != o) ? o : v;
+    }
+    const int wv = threadIdx.x / kWave;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) red[wv] = v;
+    __syncthreads();
+    double r = red[0];
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) r = (red[i] > r || red[i] != red[i]) ? red[i] : r;
+    return r;
+}
+
+// Block-wide argmax of (a, idx), ties to the lower index.
+__device__ inline void block_argmax(double &a, int &idx, double *red) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const double oa = __shfl_xor(a, off, kWave);
+        const int oi = __shfl_xor(idx, off, kWave);
+        if (oa > a || (oa == a && oi < idx)) { a = oa; idx = oi; }
+    }
+    const int wv = threadIdx.x / kWave;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        red[2 * wv] = a;
+        red[2 * wv + 1] = (double)idx;
+    }
+    __syncthreads();
+    a = red[0];
+    idx = (int)red[1];
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) {
+        const double oa = red[2 * i];
+        const int oi = (int)red[2 * i + 1];
+        if (oa > a || (oa == a && oi < idx)) { a = oa; idx = oi; }
+    }
+}
+
+// Gauss-Jordan inversion with partial pivoting of the n x n row-major FP64
+// matrix A (LDS or global memory) in place, by the whole workgroup; the
+// reference's LU tolerance, a pivot below 1e-12 is singular
+// (src/utilities/matrix.hpp:35-39).  gjcol holds n doubles and perm n ints of
+// workspace.  Returns log|det A|, or -inf for a singular matrix (A is then
+// left part-way).
+__device__ inline double gj_invert(double *A, int n, double *gjcol, int *perm, double *red) {
+    const int t = threadIdx.x, nt = blockDim.x;
+    double logdet = 0.0;
+    for (int c = 0; c < n; ++c) {
+        double best = -1.0;
+        int bi = n;
+        for (int r = c + t; r < n; r += nt) {
+            const double a = fabs(A[r * n + c]);
+            if (a > best) { best = a; bi = r; }
+        }
+        block_argmax(best, bi, red);
+        if (!(best >= 1e-12)) return -INFINITY;
+        if (t == 0) perm[c] = bi;
+        if (bi != c)
+            for (int j = t; j < n; j += nt) {
+                const double tmp = A[c * n + j];
+                A[c * n + j] = A[bi * n + j];
+                A[bi * n + j] = tmp;
+            }
+        __syncthreads();
+        const double piv = A[c * n + c];
+        logdet += log(fabs(piv));
+        const double pinv = 1.0 / piv;
+        for (int r = t; r < n; r += nt) gjcol[r] = A[r * n + c];
+        __syncthreads();
+        for (int j = t; j < n; j += nt) A[c * n + j] = (j == c ? 1.0 : A[c * n + j]) * pinv;
+        __syncthreads();
+        for (int e = t; e < n * n; e += nt) {
+            const int r = e / n, j = e - (e / n) * n;
+            if (r == c) continue;
+            A[e] = (j == c ? 0.0 : A[e]) - gjcol[r] * A[c * n + j];
+        }
+        __syncthreads();
+    }
+    for (int c = n - 1; c >= 0; --c) {
+        const int q = perm[c];
+        if (q != c)
+            for (int r = t; r < n; r += nt) {
+                const double tmp = A[r * n + c];
+                A[r * n + c] = A[r * n + q];
+                A[r * n + q] = tmp;
+            }
+        __syncthreads();
+    }
+    return logdet;
+}
+
 // ------------------------------------------------------------------ Philox
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) with the published
 // multipliers 0xD2511F53, 0xCD9E8D57 and Weyl keys 0x9E3779B9, 0xBB67AE85.

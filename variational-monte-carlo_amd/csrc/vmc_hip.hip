@@ -297,46 +297,6 @@ __host__ __device__ inline size_t reinv_lds_bytes(int nmax, bool a_in_lds) {
            (size_t)nmax * 4 /* perm */ + (size_t)kReinvCols * nmax * 8 /* columns */ + 64 * 8;
 }
 
-// Block-wide max of v (NaN wins); every thread gets the result.
-__device__ inline double block_max(double v, double *red) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        const double o = __shfl_xor(v, off, kWave);
-        v = (o > v || o != o) ? o : v;
-    }
-    const int wv = threadIdx.x / kWave;
-    __syncthreads();
-    if ((threadIdx.x & (kWave - 1)) == 0) red[wv] = v;
-    __syncthreads();
-    double r = red[0];
-    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) r = (red[i] > r || red[i] != red[i]) ? red[i] : r;
-    return r;
-}
-
-// Block-wide argmax of (a, idx), ties to the lower index.
-__device__ inline void block_argmax(double &a, int &idx, double *red) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        const double oa = __shfl_xor(a, off, kWave);
-        const int oi = __shfl_xor(idx, off, kWave);
-        if (oa > a || (oa == a && oi < idx)) { a = oa; idx = oi; }
-    }
-    const int wv = threadIdx.x / kWave;
-    __syncthreads();
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        red[2 * wv] = a;
-        red[2 * wv + 1] = (double)idx;
-    }
-    __syncthreads();
-    a = red[0];
-    idx = (int)red[1];
-    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) {
-        const double oa = red[2 * i];
-        const int oi = (int)red[2 * i + 1];
-        if (oa > a || (oa == a && oi < idx)) { a = oa; idx = oi; }
-    }
-}
-
 template <typename Real>
 __device__ void reinv_walker(const DevSys &s, const DevState &st, int w, bool a_in_lds,
                              unsigned char *smem) {
@@ -387,51 +347,10 @@ __device__ void reinv_walker(const DevSys &s, const DevState &st, int w, bool a_
         dmax = block_max(dmax, red);
         drift = (dmax > drift || dmax != dmax) ? dmax : drift;
         // Gauss-Jordan in place: A becomes (D^T)^-1 = invT.
-        double logdet = 0.0;
-        for (int c = 0; c < n && !singular; ++c) {
-            double best = -1.0;
-            int bi = n;
-            for (int r = c + t; r < n; r += nt) {
-                const double a = fabs(A[r * n + c]);
-                if (a > best) { best = a; bi = r; }
-            }
-            block_argmax(best, bi, red);
-            if (!(best >= 1e-12)) {
-                singular = true;
-                break;
-            }
-            if (t == 0) perm[c] = bi;
-            if (bi != c)
-                for (int j = t; j < n; j += nt) {
-                    const double tmp = A[c * n + j];
-                    A[c * n + j] = A[bi * n + j];
-                    A[bi * n + j] = tmp;
-                }
-            __syncthreads();
-            const double piv = A[c * n + c];
-            logdet += log(fabs(piv));
-            const double pinv = 1.0 / piv;
-            for (int r = t; r < n; r += nt) gjcol[r] = A[r * n + c];
-            __syncthreads();
-            for (int j = t; j < n; j += nt) A[c * n + j] = (j == c ? 1.0 : A[c * n + j]) * pinv;
-            __syncthreads();
-            for (int e = t; e < n * n; e += nt) {
-                const int r = e / n, j = e - (e / n) * n;
-                if (r == c) continue;
-                A[e] = (j == c ? 0.0 : A[e]) - gjcol[r] * A[c * n + j];
-            }
-            __syncthreads();
-        }
-        if (singular) break;
-        for (int c = n - 1; c >= 0; --c) {
-            const int q = perm[c];
-            if (q != c)
-                for (int r = t; r < n; r += nt) {
-                    const double tmp = A[r * n + c];
-                    A[r * n + c] = A[r * n + q];
-                    A[r * n + q] = tmp;
-                }
-            __syncthreads();
+        const double logdet = gj_invert(A, n, gjcol, perm, red);
+        if (!(logdet > -INFINITY)) {
+            singular = true;
+            break;
         }
         for (int e = t; e < n * n; e += nt) Y[e] = (Real)A[e];
         ld[d] = logdet;
@@ -474,6 +393,10 @@ __global__ __launch_bounds__(kReinvThreads) void k_reinv_batch(const BatchItem *
 
 void reinv_generic_launch(const DevSys &d, const DevState &st, unsigned grid, hipStream_t stream,
                           int real_bytes) {
+    if (reinv_ns_supported(d, real_bytes)) {
+        reinv_ns_launch(d, st, grid, stream);
+        return;
+    }
     const bool a_lds = reinv_lds_bytes(d.nmax, true) <= kMaxLdsBytes;
     const size_t lds = reinv_lds_bytes(d.nmax, a_lds);
     if (real_bytes == 8)
@@ -804,8 +727,6 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
     // With periodic re-inversion, one sweep per launch, each followed by the
     // FP64 re-inversion of the walkers that came due.
     const bool reinv = d.reinv_k > 0;
-    const bool a_lds = reinv_lds_bytes(d.nmax, true) <= kMaxLdsBytes;
-    const size_t lds_r = reinv_lds_bytes(d.nmax, a_lds);
     const int per_launch = reinv ? 1 : n_sweeps;
     for (int i = 0; i < n_sweeps; i += per_launch) {
         const int n = n_sweeps - i < per_launch ? n_sweeps - i : per_launch;
@@ -821,8 +742,7 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
         if (timed) e->timer.end(stream, ev, kTimeSweep);
         if (reinv) {
             timed = e->timer.begin(stream, ev);
-            hipLaunchKernelGGL((k_reinv_generic<Real>), dim3(grid), dim3(kReinvThreads), lds_r, stream, d,
-                               e->st, a_lds ? 1 : 0);
+            reinv_generic_launch(d, e->st, grid, stream, static_cast<int>(sizeof(Real)));
             if (timed) e->timer.end(stream, ev, kTimeReinv);
         }
     }

@@ -1,0 +1,269 @@
+// vmc_reinv_ns.hip -- periodic FP64 re-inversion of FP32 inverses on the
+// matrix cores, for determinants of up to 96 electrons (the multi-wavefront
+// register kernels' 33, 57 and 81 per spin, and the generic kernel's sizes up
+// to 96).
+//
+// One 256-thread workgroup per due walker (accepted moves since its last
+// re-inversion >= K); walkers not due leave at once.  Per determinant of order
+// n, padded to P = 16 NT with an identity block:
+//   E = D^T in FP64 (E[m][i] = phi_m(r_i), axis phasors of the FP64
+//       positions), Y = the FP32 inverse invT (= E^-1 up to its drift);
+//   repeat: Z = E Y on v_mfma_f64_16x16x4f64; the drift d = max|I - Z| over
+//       the n x n block (recorded on the first pass); R = I - Z;
+//       Y <- Y + Y R (FP64 products, stored back in FP32);
+//   until d < 1e-4, after which the step just taken leaves d^2 < 1e-8, below
+//   the rounding of the FP32 inverse.  A drift of 0.5 or more, a non-finite
+//   one, or no convergence within kNSSteps falls back to Gauss-Jordan from
+//   scratch, which also refreshes log|det D|.
+// The identity padding keeps the padded block exact: Z, R and Y stay the
+// identity there and zero across, so the n x n block evolves as if unpadded.
+// The reference never re-inverts: it keeps its FP64 inverse by
+// Sherman-Morrison updates alone (src/slater_plane_wave/slater_plane_wave.cpp:367-423).
+#include "vmc_device.hpp"
+#include "vmc_rr_launch.hpp"
+#include "vmc_sweep_rr.hpp"
+
+namespace vmc {
+namespace {
+
+typedef double ns_v4d __attribute__((ext_vector_type(4)));
+
+constexpr int kNSThreads = 256;
+constexpr int kNSWaves = kNSThreads / kWave;
+constexpr int kNSSteps = 6;
+constexpr double kNSDone = 1e-4;
+
+template <int NT>
+struct NSShared {
+    static constexpr int P = 16 * NT;   // padded order
+    static constexpr int PE = P + 1;    // row pitch of E in doubles
+    static constexpr int PF = P + 1;    // row pitch of Y and R in floats
+    double E[P * PE];
+    float Y[P * PF];
+    float R[P * PF];                    // also Gauss-Jordan's gjcol and perm
+    double red[2 * kNSWaves];
+};
+
+// Output tiles of wave wv: with NT even, the wave's (NT/2) x (NT/2) block of
+// the 2 x 2 block grid (each A and B fragment loaded once per k-step for NT/2
+// tiles); with NT odd, tiles wv, wv + 4, ... in row-major order.
+template <int NT>
+struct NSTiles {
+    static constexpr bool kBlock = NT % 2 == 0;
+    static constexpr int kHalf = NT / 2;
+    static constexpr int kCount = kBlock ? kHalf * kHalf : (NT * NT + kNSWaves - 1) / kNSWaves;
+    __device__ static bool valid(int i, int wv) { return kBlock || wv + kNSWaves * i < NT * NT; }
+    __device__ static int row(int i, int wv) {
+        return kBlock ? (wv >> 1) * kHalf + i / kHalf : (wv + kNSWaves * i) / NT;
+    }
+    __device__ static int col(int i, int wv) {
+        return kBlock ? (wv & 1) * kHalf + i % kHalf : (wv + kNSWaves * i) % NT;
+    }
+};
+
+// acc[i] = (A B) on tile i of this wave; A and B are row-major LDS matrices of
+// order P with pitches PA and PB.  Operand layout of v_mfma_f64_16x16x4f64:
+// lane l supplies A[l & 15][l >> 4] and B[l >> 4][l & 15]; output register q
+// of lane l is C[4q + (l >> 4)][l & 15].
+template <int NT, typename TA, int PA, typename TB, int PB>
+__device__ __forceinline__ void ns_mm(const TA *A, const TB *B, ns_v4d acc[NSTiles<NT>::kCount], int wv,
+                                      int lane) {
+    using T = NSTiles<NT>;
+    const int lo = lane & 15, hi = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < T::kCount; ++i) acc[i] = ns_v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int kk = 0; kk < 4 * NT; ++kk) {
+        const int k = 4 * kk + hi;
+        if constexpr (T::kBlock) {
+            double a[T::kHalf], b[T::kHalf];
+#pragma unroll
+            for (int u = 0; u < T::kHalf; ++u) {
+                a[u] = (double)A[(16 * ((wv >> 1) * T::kHalf + u) + lo) * PA + k];
+                b[u] = (double)B[k * PB + 16 * ((wv & 1) * T::kHalf + u) + lo];
+            }
+#pragma unroll
+            for (int u = 0; u < T::kHalf; ++u)
+#pragma unroll
+                for (int v = 0; v < T::kHalf; ++v)
+                    acc[u * T::kHalf + v] =
+                        __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[v], acc[u * T::kHalf + v], 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int i = 0; i < T::kCount; ++i) {
+                if (T::valid(i, wv)) {
+                    const double a = (double)A[(16 * T::row(i, wv) + lo) * PA + k];
+                    const double b = (double)B[k * PB + 16 * T::col(i, wv) + lo];
+                    acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
+                }
+            }
+        }
+    }
+}
+
+// E = D^T of determinant d in FP64 (row pitch `pitch`): four threads per
+// electron i, each writing every fourth orbital of column i from the axis
+// phasors of the electron's FP64 position.  With `pad`, rows and columns n..P
+// are set to the identity.
+template <int NT>
+__device__ void ns_build(const DevSys &s, const double *gp, int d, double *E, int pitch, bool pad) {
+    constexpr int P = 16 * NT;
+    const int n = s.n[d], first = s.first[d];
+    if (pad)
+        for (int e = threadIdx.x; e < P * P; e += kNSThreads) {
+            const int i = e / P, j = e - (e / P) * P;
+            if (i >= n || j >= n) E[i * pitch + j] = i == j ? 1.0 : 0.0;
+        }
+    for (int e = threadIdx.x; e < 4 * n; e += kNSThreads) {
+        const int c = e / n, i = e - c * n;
+        rr::Cx<double> a[3], a2[3], a3[3];
+        a[0] = rr::axis_phasor<double>(gp[first + i], s.inv_L);
+        a[1] = rr::axis_phasor<double>(gp[s.N + first + i], s.inv_L);
+        a[2] = rr::axis_phasor<double>(gp[2 * s.N + first + i], s.inv_L);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            a2[q] = rr::cmul(a[q], a[q]);
+            a3[q] = rr::cmul(a2[q], a[q]);
+        }
+        for (int m = c; m < n; m += 4) {
+            const int4 o = s.orb[d * s.nmax + m];
+            const rr::Cx<double> ph = rr::orbital_phasor3<double>(o.x, o.y, o.z, a, a2, a3);
+            E[m * pitch + i] = o.w ? ph.im : ph.re;
+        }
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(kNSThreads) void k_reinv_ns(DevSys s, DevState st) {
+    using S = NSShared<NT>;
+    using T = NSTiles<NT>;
+    constexpr int P = S::P, PE = S::PE, PF = S::PF;
+    const int w = blockIdx.x;
+    if (s.reinv_k <= 0 || st.since[w] < s.reinv_k) return;
+    __shared__ S sh;
+    const int t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
+    const double *gp = st.pos + (size_t)w * 3 * s.N;
+    float *inv = reinterpret_cast<float *>(st.inv) + (size_t)w * s.ninv;
+    double drift = 0.0, ld[2] = {st.logdet[2 * w + 0], st.logdet[2 * w + 1]};
+    bool singular = false;
+    for (int d = 0; d < s.ndet; ++d) {
+        const int n = s.n[d];
+        float *Yg = inv + s.inv_off[d];
+        ns_build<NT>(s, gp, d, sh.E, PE, true);
+        for (int e = t; e < P * P; e += kNSThreads) {
+            const int i = e / P, j = e - (e / P) * P;
+            sh.Y[i * PF + j] = (i < n && j < n) ? Yg[i * n + j] : (i == j ? 1.0f : 0.0f);
+        }
+        __syncthreads();
+        double d0 = 0.0;
+        bool ok = false;
+#pragma unroll 1
+        for (int it = 0; it < kNSSteps; ++it) {
+            ns_v4d z[T::kCount];
+            ns_mm<NT, double, PE, float, PF>(sh.E, sh.Y, z, wv, lane);
+            double dm = 0.0;
+#pragma unroll
+            for (int i = 0; i < T::kCount; ++i) {
+                if (!T::valid(i, wv)) continue;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = 16 * T::row(i, wv) + 4 * q + (lane >> 4);
+                    const int c = 16 * T::col(i, wv) + (lane & 15);
+                    const double dv = (r == c ? 1.0 : 0.0) - z[i][q];
+                    if (r < n && c < n) dm = (fabs(dv) > dm || dv != dv) ? fabs(dv) : dm;
+                    sh.R[r * PF + c] = (float)dv;
+                }
+            }
+            dm = block_max(dm, sh.red);  // its barriers also publish R
+            if (it == 0) d0 = dm;
+            if (!(dm < 0.5)) break;
+            if (dm < 1e-15) {
+                ok = true;
+                break;
+            }
+            ns_v4d y[T::kCount];
+            ns_mm<NT, float, PF, float, PF>(sh.Y, sh.R, y, wv, lane);
+            float yn[T::kCount][4];
+#pragma unroll
+            for (int i = 0; i < T::kCount; ++i) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = 16 * T::row(i, wv) + 4 * q + (lane >> 4);
+                    const int c = 16 * T::col(i, wv) + (lane & 15);
+                    yn[i][q] = T::valid(i, wv) ? (float)((double)sh.Y[r * PF + c] + y[i][q]) : 0.0f;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < T::kCount; ++i) {
+                if (!T::valid(i, wv)) continue;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    sh.Y[(16 * T::row(i, wv) + 4 * q + (lane >> 4)) * PF + 16 * T::col(i, wv) + (lane & 15)] =
+                        yn[i][q];
+            }
+            __syncthreads();
+            if (dm < kNSDone) {
+                ok = true;
+                break;
+            }
+        }
+        drift = (d0 > drift || d0 != d0) ? d0 : drift;
+        if (ok) {
+            for (int e = t; e < n * n; e += kNSThreads) {
+                const int i = e / n, j = e - (e / n) * n;
+                Yg[e] = sh.Y[i * PF + j];
+            }
+        } else {
+            // Gauss-Jordan from scratch on E (row pitch n) in FP64.
+            __syncthreads();
+            ns_build<NT>(s, gp, d, sh.E, n, false);
+            __syncthreads();
+            double *gjcol = reinterpret_cast<double *>(sh.R);
+            int *perm = reinterpret_cast<int *>(gjcol + P);
+            const double l = gj_invert(sh.E, n, gjcol, perm, sh.red);
+            if (!(l > -INFINITY)) {
+                singular = true;
+                break;
+            }
+            ld[d] = l;
+            for (int e = t; e < n * n; e += kNSThreads) Yg[e] = (float)sh.E[e];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        st.since[w] = 0;
+        st.stats[(size_t)w * kStatCount + 15] += 1.0;
+        if (singular) {
+            st.flags[w] |= 1;
+        } else {
+            st.logdet[2 * w + 0] = ld[0];
+            st.logdet[2 * w + 1] = ld[1];
+        }
+        if (drift > st.drift[w] || drift != drift) st.drift[w] = drift;
+    }
+}
+
+template <int NT>
+void launch_ns(const DevSys &d, const DevState &st, unsigned grid, hipStream_t stream) {
+    hipLaunchKernelGGL((k_reinv_ns<NT>), dim3(grid), dim3(kNSThreads), 0, stream, d, st);
+}
+
+}  // namespace
+
+bool reinv_ns_supported(const DevSys &d, int real_bytes) {
+    return real_bytes == 4 && d.nmax <= 96;
+}
+
+void reinv_ns_launch(const DevSys &d, const DevState &st, unsigned grid, hipStream_t stream) {
+    switch ((d.nmax + 15) / 16) {
+        case 1: launch_ns<1>(d, st, grid, stream); break;
+        case 2: launch_ns<2>(d, st, grid, stream); break;
+        case 3: launch_ns<3>(d, st, grid, stream); break;
+        case 4: launch_ns<4>(d, st, grid, stream); break;
+        case 5: launch_ns<5>(d, st, grid, stream); break;
+        default: launch_ns<6>(d, st, grid, stream); break;
+    }
+}
+
+}  // namespace vmc

@@ -69,8 +69,14 @@ inline bool rrw_supported_sizes(int ndet, int n_up, int n_dn, int ng, int real_b
     return n_ok && ng <= 12 * kWave;
 }
 
-// The FP64 re-inversion kernel of the generic path, callable from the
-// register-resident launchers (defined with the generic kernels).
+// Newton-Schulz re-inversion on the matrix cores for FP32 inverses of
+// determinants up to 96 electrons (vmc_reinv_ns.hip).
+bool reinv_ns_supported(const DevSys &d, int real_bytes);
+void reinv_ns_launch(const DevSys &d, const DevState &st, unsigned grid, hipStream_t stream);
+
+// The FP64 re-inversion after a sweep of the multi-wavefront register and the
+// generic kernels: Newton-Schulz where supported, else Gauss-Jordan (defined
+// with the generic kernels).
 void reinv_generic_launch(const DevSys &d, const DevState &st, unsigned grid, hipStream_t stream,
                           int real_bytes);
 struct ReinvLaunch {
