@@ -219,37 +219,55 @@ int vmc_frozen_eval(const vmc_system *sys, int32_t n_configs, const double *pos,
                     double *inv_before, double *inv_after, double *s_k, void *stream);
 
 /* ------------------------------------------------------------------ batch
- * Systems of uneven size in one launch.  The reference runs one system per
- * process and one chain per thread (src/main.cpp:50-68); a batch runs many
- * systems' walker ensembles -- each an independent vmc_ensemble with its own
- * seed -- as one persistent kernel per sweep: workgroups take global walker
- * indices from a device queue until all systems are swept.  A walker's chain
- * depends only on its system, seed, walker index and sweep index, so every
- * system gives the same result in a batch as alone with the generic kernel.
- * All systems of a batch share one working precision.
+ * Systems of uneven size, r_s, Jastrow parameters and walker counts swept
+ * together.  The reference runs one system per process and one chain per
+ * thread (src/main.cpp:50-68); a batch runs many systems' walker ensembles --
+ * each an independent vmc_ensemble with its own seed -- with one launch per
+ * kernel per sweep for each group of systems: the register-resident systems
+ * of one size form a group (one workgroup per walker across all of them),
+ * and the systems on the generic kernel share one persistent launch whose
+ * workgroups take walker indices from a device queue.  Groups run on streams
+ * of their own, forked from and joined back into the caller's stream around
+ * each sweep (parallel branches of a captured graph).  The sweep index lives
+ * in device memory.  A walker's chain depends only on its system, seed,
+ * walker index, sweep index and kernel, so every system gives the same
+ * result in a batch as alone with the same kernel.  All systems of a batch
+ * share one working precision.
  *
- * inv_placement[i] (NULL = all 0): 0 inverse in LDS when it fits, 1 LDS
- * (error if it does not fit), 2 HBM.  walker_threads (64, 128 or 256) is the
- * workgroup per walker of the persistent sweep (1, 2 or 4 wavefronts). */
+ * inv_placement[i] (NULL = all 0): generic kernel's inverse, 0 in LDS when
+ * it fits, 1 LDS (error if it does not fit), 2 HBM.  walker_threads (64, 128
+ * or 256) is the generic persistent sweep's workgroup per walker (1, 2 or 4
+ * wavefronts). */
 typedef struct vmc_batch vmc_batch;
 int vmc_batch_create(const vmc_system *systems, const int64_t *n_walkers, const uint64_t *seeds,
                      const double *step0, const int32_t *inv_placement, int32_t n_systems,
                      int32_t walker_threads, vmc_batch **out);
 int vmc_batch_destroy(vmc_batch *b);
 int vmc_batch_init_positions(vmc_batch *b, void *stream);
-/* n_sweeps launches of the persistent batch kernel (flags as for
- * vmc_ensemble_sweep, block_size per system in measured sweeps). */
+/* n_sweeps batch sweeps (flags as for vmc_ensemble_sweep, block_size per
+ * system in measured sweeps). */
 int vmc_batch_sweep(vmc_batch *b, int32_t n_sweeps, int32_t flags, int32_t block_size, void *stream);
 /* Captures n_sweeps batch sweeps into a HIP graph (on a stream the batch
- * owns); vmc_batch_replay launches it on `stream`.  The sweep index lives in
- * device memory, so each replay continues the chains. */
+ * owns); vmc_batch_replay launches it on `stream`.  Each replay continues
+ * the chains.  Changing kernels or overlap drops the graph. */
 int vmc_batch_capture(vmc_batch *b, int32_t n_sweeps, int32_t flags, int32_t block_size);
 int vmc_batch_replay(vmc_batch *b, void *stream);
 /* Borrowed handle of system i for the vmc_ensemble_read_* / get_positions /
  * reset_stats calls; sweeping or destroying it directly is refused. */
 int vmc_batch_member(vmc_batch *b, int32_t system, vmc_ensemble **out);
+/* grid: workgroups of one sweep's sweep launches over all groups; lds_bytes:
+ * the generic group's dynamic LDS per workgroup. */
 int vmc_batch_info(const vmc_batch *b, int32_t *n_systems, int64_t *total_walkers, int32_t *grid,
                    int64_t *lds_bytes);
+/* Kernel of system `system` (-1: every system) as for vmc_ensemble_set_kernel;
+ * regroups the batch (synchronises the device).  No reference counterpart. */
+int vmc_batch_set_kernel(vmc_batch *b, int32_t system, int32_t kernel);
+/* overlap != 0 (default): groups on their own streams; 0: all on the
+ * caller's stream, one after another.  No reference counterpart. */
+int vmc_batch_set_overlap(vmc_batch *b, int32_t overlap);
+/* Groups: their count, family[g] (0 generic, 1 register, 2 multi-wavefront
+ * register) and group_of[i] for each system, up to `cap` entries each. */
+int vmc_batch_groups(const vmc_batch *b, int32_t *n_groups, int32_t *family, int32_t *group_of, int32_t cap);
 
 /* ---------------------------------------------------- placement, footprint */
 /* Generic kernel's inverse placement: 0 LDS when it fits (default), 1 LDS

@@ -1,10 +1,12 @@
-"""Uneven batches: many systems' walker ensembles swept by one persistent
-launch per sweep, optionally captured into a HIP graph.
+"""Uneven batches: many systems' walker ensembles swept together, optionally
+captured into a HIP graph.
 
-A walker's chain depends only on its system, seed, walker index and sweep
-index, and the batch kernel runs the generic kernel's per-walker routine, so
-every system of a batch must come out exactly as when it runs alone with the
-generic kernel -- positions, accept counts, step sizes and local energies.
+Register-resident systems of one size form a group swept by one launch per
+kernel; the systems on the generic kernel share one persistent launch.  A
+walker's chain depends only on its system, seed, walker index, sweep index
+and kernel, so every system of a batch must come out exactly as when it runs
+alone with the same kernel -- positions, accept counts, step sizes and local
+energies -- whatever the grouping, the stream overlap or graph replay.
 """
 import numpy as np
 import pytest
@@ -16,22 +18,13 @@ SPECS = [  # (N, r_s, b, walkers, seed)
     (38, 3.0, 1.5, 9, 202),
     (66, 2.0, 0.8, 5, 303),
     (54, 5.0, 0.5, 12, 404),
+    (186, 4.0, 1.0, 3, 505),   # 93 per spin: generic kernel only
+    (54, 2.0, 1.2, 7, 606),
 ]
 
 
 def systems(V, precision=0, K=0):
     return [V.System.heg(N, rs, b, precision=precision, reinvert_interval=K) for N, rs, b, _, _ in SPECS]
-
-
-def run_alone(V, sy, W, seed, sweeps, measure, adapt):
-    e = V.Ensemble(sy, W, seed)
-    e.set_kernel(V.KERNEL_GENERIC)
-    e.init_positions()
-    for _ in range(sweeps):
-        e.sweep(1, measure=measure, adapt=adapt, block_size=2)
-    out = (e.positions(), e.stats(), e.steps())
-    e.close()
-    return out
 
 
 def snapshot(b):
@@ -49,49 +42,71 @@ def assert_same(a, b):
         np.testing.assert_array_equal(t1, t2)
 
 
+def alone(V, s, W, seed, kernel):
+    e = V.Ensemble(s, W, seed)
+    e.set_kernel(kernel)
+    e.init_positions()
+    e.sweep(2, adapt=True, block_size=2)
+    e.sweep(3, measure=True, block_size=2)
+    out = (e.positions(), e.stats(), e.steps())
+    e.close()
+    return out
+
+
 @pytest.mark.parametrize("precision,K", [(0, 0), (1, 32)], ids=["fp64", "mixed_K32"])
-def test_batch_equals_each_system_alone(vmc, precision, K):
+@pytest.mark.parametrize("kernel", [0, 1], ids=["auto", "generic"])
+def test_batch_equals_each_system_alone(vmc, precision, K, kernel):
     sy = systems(vmc, precision, K)
     W = [w for *_, w, _ in SPECS]
     seeds = [s for *_, s in SPECS]
     b = vmc.Batch(sy, W, seeds)
+    b.set_kernel(kernel)
     info = b.info()
-    print("batch", info, flush=True)
+    fam, grp = b.groups()
+    print("batch", info, fam, grp, flush=True)
     assert info["total_walkers"] == sum(W) and 1 <= info["grid"] <= sum(W)
+    if kernel == 0:
+        # 7, 19, 27 per spin: register; 33: multi-wavefront; 93: generic.
+        # The two N = 54 systems share a group.
+        assert sorted(fam) == [0, 1, 1, 1, 2]
+        assert grp[3] == grp[5] and len(set(grp)) == 5
+        assert fam[grp[4]] == 0 and fam[grp[2]] == 2 and fam[grp[0]] == 1
+    else:
+        assert fam == [0] and set(grp) == {0}
     b.init_positions()
     b.sweep(2, adapt=True, block_size=2)
     b.sweep(3, measure=True, block_size=2)
     got = snapshot(b)
     b.close()
     for i, (spec, s) in enumerate(zip(SPECS, sy)):
-        e = vmc.Ensemble(s, spec[3], spec[4])
-        e.set_kernel(vmc.KERNEL_GENERIC)
-        e.init_positions()
-        e.sweep(2, adapt=True, block_size=2)
-        e.sweep(3, measure=True, block_size=2)
-        alone = (e.positions(), e.stats(), e.steps())
-        e.close()
-        assert_same([got[i]], [alone])
-        assert np.all(alone[1][:, vmc.STAT["NMEAS"]] == 3)
+        ref = alone(vmc, s, spec[3], spec[4], kernel)
+        assert_same([got[i]], [ref])
+        assert np.all(ref[1][:, vmc.STAT["NMEAS"]] == 3)
 
 
-def test_batch_inverse_placement_hbm_is_identical(vmc):
-    sy = systems(vmc)
+def test_batch_overlap_and_placement_do_not_change_results(vmc):
+    """Groups on their own streams or all on one, and the generic group's
+    inverses in LDS or HBM: identical chains."""
+    sy = systems(vmc, precision=1, K=64)
     W = [w for *_, w, _ in SPECS]
     seeds = [s for *_, s in SPECS]
     outs = []
-    for placement in (None, [2] * len(SPECS)):
+    for overlap, placement in ((True, None), (False, None), (True, [2] * len(SPECS))):
         b = vmc.Batch(sy, W, seeds, placement=placement)
+        b.set_overlap(overlap)
         b.init_positions()
         b.sweep(3, measure=True, adapt=True)
         outs.append(snapshot(b))
         b.close()
     assert_same(outs[0], outs[1])
+    assert_same(outs[0], outs[2])
 
 
-def test_batch_graph_replays_continue_the_chains(vmc):
-    """A captured graph of 2 sweeps, replayed 3 times with eager reads
-    between the replays, equals 6 eager batch sweeps after every replay."""
+@pytest.mark.parametrize("overlap", [True, False], ids=["overlap", "one_stream"])
+def test_batch_graph_replays_continue_the_chains(vmc, overlap):
+    """A captured graph of 2 sweeps (groups as parallel branches with
+    overlap), replayed 3 times with eager reads between the replays, equals
+    6 eager batch sweeps after every replay."""
     import torch
 
     sy = systems(vmc, precision=1, K=64)
@@ -100,6 +115,7 @@ def test_batch_graph_replays_continue_the_chains(vmc):
     stream = torch.cuda.Stream()
     eager = vmc.Batch(sy, W, seeds, stream=stream.cuda_stream)
     graph = vmc.Batch(sy, W, seeds, stream=stream.cuda_stream)
+    graph.set_overlap(overlap)
     eager.init_positions()
     graph.init_positions()
     graph.capture(2, measure=True, block_size=2)
@@ -112,6 +128,20 @@ def test_batch_graph_replays_continue_the_chains(vmc):
         assert graph.member(i).sweeps_done() == 6
     eager.close()
     graph.close()
+
+
+def test_batch_regroup_drops_the_graph(vmc):
+    sy = systems(vmc, precision=1, K=64)[:2]
+    b = vmc.Batch(sy, [4, 4], [1, 2])
+    b.init_positions()
+    b.capture(1)
+    b.replay()
+    b.set_kernel(vmc.KERNEL_GENERIC, system=1)
+    fam, grp = b.groups()
+    assert sorted(fam) == [0, 1] and fam[grp[1]] == 0
+    with pytest.raises(vmc.VmcError):
+        b.replay()
+    b.close()
 
 
 def test_batch_members_refuse_direct_sweeps(vmc):

@@ -57,6 +57,24 @@ struct DevState {
     double *drift;    // [W] largest |I - D^-1 D| met before a re-inversion
 };
 
+// One system of a batch group: its tables and state, whether its inverses
+// live in LDS (generic kernel), and its walkers' range in the group's walker
+// numbering.
+struct BatchItem {
+    DevSys s;
+    DevState st;
+    int inv_lds;
+    int walker_begin;
+    int walker_end;
+};
+
+// Index of the item whose walker range holds g (items in walker order).
+__device__ inline int batch_find(const BatchItem *items, int n_items, int g) {
+    int i = 0;
+    while (i + 1 < n_items && g >= items[i].walker_end) ++i;
+    return i;
+}
+
 // -------------------------------------------------------------- reductions
 __device__ inline double wave_sum(double v) {
 #pragma unroll

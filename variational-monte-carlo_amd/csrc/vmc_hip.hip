@@ -232,16 +232,6 @@ __global__ __launch_bounds__(256) void k_sweep(DevSys s, DevState st, int n_swee
     sweep_walker<Real>(s, st, blockIdx.x, kInvLds, n_sweeps, sweep0, flags, block_size, smem);
 }
 
-// One system of a batch: its tables and state, whether its inverses live in
-// LDS, and where its walkers start in the batch's global walker numbering.
-struct BatchItem {
-    DevSys s;
-    DevState st;
-    int inv_lds;
-    int walker_begin;
-    int walker_end;
-};
-
 // One sweep of every walker of every system of a batch in one persistent
 // launch: each workgroup takes global walker indices from a device queue
 // until the batch is exhausted, so systems of uneven size and cost share the
@@ -264,9 +254,7 @@ __global__ __launch_bounds__(256) void k_sweep_batch(const BatchItem *items, int
         const int g = next;
         __syncthreads();
         if (g >= total) break;
-        int i = 0;
-        while (i + 1 < n_items && g >= items[i].walker_end) ++i;
-        const BatchItem &it = items[i];
+        const BatchItem &it = items[batch_find(items, n_items, g)];
         sweep_walker<Real>(it.s, it.st, g - it.walker_begin, it.inv_lds != 0, 1, sweep0, flags,
                            block_size, smem);
     }
@@ -275,7 +263,7 @@ __global__ __launch_bounds__(256) void k_sweep_batch(const BatchItem *items, int
 // After a batch sweep: next sweep index, queue back to zero.
 __global__ void k_batch_advance(unsigned int *queue, unsigned long long *sweep_ctr) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        *queue = 0u;
+        if (queue) *queue = 0u;
         *sweep_ctr += 1ull;
     }
 }
@@ -383,9 +371,7 @@ __global__ __launch_bounds__(kReinvThreads) void k_reinv_batch(const BatchItem *
                                                                size_t lds_budget) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int g = blockIdx.x;
-    int i = 0;
-    while (i + 1 < n_items && g >= items[i].walker_end) ++i;
-    const BatchItem &it = items[i];
+    const BatchItem &it = items[batch_find(items, n_items, g)];
     if (it.s.reinv_k <= 0) return;
     const bool a_in_lds = reinv_lds_bytes(it.s.nmax, true) <= lds_budget;
     reinv_walker<Real>(it.s, it.st, g - it.walker_begin, a_in_lds, smem);
@@ -405,6 +391,22 @@ void reinv_generic_launch(const DevSys &d, const DevState &st, unsigned grid, hi
     else
         hipLaunchKernelGGL((k_reinv_generic<float>), dim3(grid), dim3(kReinvThreads), lds, stream, d, st,
                            a_lds ? 1 : 0);
+}
+
+void reinv_batch_launch(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
+                        hipStream_t stream, int real_bytes) {
+    if (reinv_ns_supported(shape, real_bytes)) {
+        reinv_ns_batch_launch(shape, items, n_items, grid, stream);
+        return;
+    }
+    const bool a_lds = reinv_lds_bytes(shape.nmax, true) <= kMaxLdsBytes;
+    const size_t lds = reinv_lds_bytes(shape.nmax, a_lds);
+    if (real_bytes == 8)
+        hipLaunchKernelGGL((k_reinv_batch<double>), dim3(grid), dim3(kReinvThreads), lds, stream, items, n_items,
+                           lds);
+    else
+        hipLaunchKernelGGL((k_reinv_batch<float>), dim3(grid), dim3(kReinvThreads), lds, stream, items, n_items,
+                           lds);
 }
 
 // Frozen-configuration evaluation (one configuration per wavefront).
@@ -1110,29 +1112,76 @@ int vmc_frozen_eval(const vmc_system *sys, int32_t n_configs, const double *pos,
 
 }  // extern "C"
 
+// A batch group: systems swept by one kernel family and, for the register
+// families, of one size, so that one launch per kernel covers them all.
+struct BatchGroup {
+    int family = 0;                  // 0 generic (persistent), 1 register, 2 multi-wavefront register
+    DevSys shape{};                  // a member's tables (sizes shared by a register group)
+    std::vector<int> members;        // indices into vmc_batch::members
+    BatchItem *items = nullptr;      // device copy; walker ranges numbered within the group
+    int total = 0;                   // walkers of the group
+    bool reinv = false;              // some member re-inverts periodically
+    unsigned int *queue = nullptr;   // generic: the persistent kernel's walker queue
+    int grid = 0;                    // workgroups of the group's sweep launch
+    size_t lds = 0;                  // generic: dynamic LDS of the sweep
+    double cost = 0.0;               // sum of W N^2 (launch order, largest first)
+    hipStream_t lane = nullptr;      // own stream when groups overlap (not the first group)
+    hipEvent_t join = nullptr;
+    // generic: the re-inverting members by determinant order, each subset
+    // re-inverted by the kernel an ensemble of that order uses alone
+    struct Reinv {
+        DevSys shape;
+        BatchItem *items;
+        int n_items, total;
+    };
+    std::vector<Reinv> reinv_sets;
+
+    void release() {
+        for (auto &r : reinv_sets)
+            if (r.items) (void)hipFree(r.items);
+        reinv_sets.clear();
+        if (items) (void)hipFree(items);
+        if (queue) (void)hipFree(queue);
+        if (lane) (void)hipStreamDestroy(lane);
+        if (join) (void)hipEventDestroy(join);
+        items = nullptr;
+        queue = nullptr;
+        lane = nullptr;
+        join = nullptr;
+    }
+};
+
 struct vmc_batch {
     std::vector<vmc_ensemble *> members;
-    BatchItem *items = nullptr;
-    unsigned int *queue = nullptr;
+    std::vector<int> placement;      // per member: generic kernel's inverse placement (0 auto, 1 LDS, 2 HBM)
+    std::vector<BatchGroup> groups;
     unsigned long long *ctr = nullptr;
-    int total = 0;
-    int grid = 0;
-    size_t lds = 0;
-    int threads = 64;       // threads per walker of the persistent sweep
-    bool reinv = false;     // some system re-inverts periodically
-    size_t reinv_lds = 0;   // dynamic LDS of the batch re-inversion kernel
+    int64_t total = 0;
+    int threads = 64;       // threads per walker of the generic persistent sweep
     int real_bytes = 8;
+    bool overlap = true;    // groups on their own streams, joined after each sweep
+    hipEvent_t fork = nullptr;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_sweeps = 0;
 
-    ~vmc_batch() {
+    void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_sweeps = 0;
+    }
+    void drop_groups() {
+        for (auto &g : groups) g.release();
+        groups.clear();
+    }
+    ~vmc_batch() {
+        drop_graph();
+        drop_groups();
+        if (fork) (void)hipEventDestroy(fork);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
-        if (items) (void)hipFree(items);
-        if (queue) (void)hipFree(queue);
         if (ctr) (void)hipFree(ctr);
         for (auto *m : members) delete m;
     }
@@ -1140,16 +1189,164 @@ struct vmc_batch {
 
 namespace {
 
+// Kernel family of a batch member, by the rule the ensemble's own sweep
+// uses: register-resident when an instantiation exists and the member does
+// not ask for the generic kernel.
+int batch_family(const vmc_ensemble *e) {
+    const DevSys &d = e->tables.dev;
+    if (e->kernel == VMC_KERNEL_GENERIC) return 0;
+    if (rr_supported(d)) return 1;
+    if (rrw_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng, e->tables.real_bytes)) return 2;
+    return 0;
+}
+
+// (Re)builds the batch's groups from its members' kernels.  The caller has
+// synchronised the device (old group buffers may be in use otherwise).
+int batch_build_groups(vmc_batch *b) {
+    b->drop_graph();
+    b->drop_groups();
+    for (size_t i = 0; i < b->members.size(); ++i) {
+        const vmc_ensemble *e = b->members[i];
+        const DevSys &d = e->tables.dev;
+        const int fam = batch_family(e);
+        BatchGroup *g = nullptr;
+        for (auto &h : b->groups)
+            if (h.family == fam && (fam == 0 || (h.shape.ndet == d.ndet && h.shape.n_up == d.n_up &&
+                                                 h.shape.n_dn == d.n_dn && h.shape.nmax == d.nmax)))
+                g = &h;
+        if (!g) {
+            b->groups.emplace_back();
+            g = &b->groups.back();
+            g->family = fam;
+            g->shape = d;
+        }
+        g->members.push_back(static_cast<int>(i));
+        g->cost += static_cast<double>(e->W) * d.N * d.N;
+    }
+    std::stable_sort(b->groups.begin(), b->groups.end(),
+                     [](const BatchGroup &x, const BatchGroup &y) { return x.cost > y.cost; });
+    int dev = 0, cus = 0;
+    VMC_HIP_TRY(hipGetDevice(&dev));
+    VMC_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        BatchGroup &g = b->groups[gi];
+        std::vector<BatchItem> items(g.members.size());
+        int64_t total = 0;
+        for (size_t k = 0; k < g.members.size(); ++k) {
+            const int i = g.members[k];
+            const vmc_ensemble *e = b->members[static_cast<size_t>(i)];
+            const DevSys &d = e->tables.dev;
+            items[k].s = d;
+            items[k].st = e->st;
+            items[k].walker_begin = static_cast<int>(total);
+            total += e->W;
+            items[k].walker_end = static_cast<int>(total);
+            items[k].inv_lds = 0;
+            if (d.reinv_k > 0) g.reinv = true;
+            if (g.family == 0) {
+                const bool fits = lds_for(d, e->tables.real_bytes, true) <= kMaxLdsBytes;
+                const bool inv_lds = b->placement[static_cast<size_t>(i)] == 2 ? false : fits;
+                items[k].inv_lds = inv_lds ? 1 : 0;
+                g.lds = std::max(g.lds, lds_for(d, e->tables.real_bytes, inv_lds));
+            }
+        }
+        g.total = static_cast<int>(total);
+        int rc;
+        if ((rc = dev_alloc(&g.items, items.size()))) return rc;
+        VMC_HIP_TRY(hipMemcpy(g.items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice));
+        if (g.family == 0) {
+            std::vector<int> orders;
+            for (const auto &it : items)
+                if (it.s.reinv_k > 0 && std::find(orders.begin(), orders.end(), it.s.nmax) == orders.end())
+                    orders.push_back(it.s.nmax);
+            for (int nm : orders) {
+                std::vector<BatchItem> sub;
+                int64_t t = 0;
+                for (const auto &it : items) {
+                    if (it.s.reinv_k <= 0 || it.s.nmax != nm) continue;
+                    BatchItem x = it;
+                    const int w = it.walker_end - it.walker_begin;
+                    x.walker_begin = static_cast<int>(t);
+                    t += w;
+                    x.walker_end = static_cast<int>(t);
+                    sub.push_back(x);
+                }
+                BatchGroup::Reinv r{sub[0].s, nullptr, static_cast<int>(sub.size()), static_cast<int>(t)};
+                if ((rc = dev_alloc(&r.items, sub.size()))) return rc;
+                g.reinv_sets.push_back(r);
+                VMC_HIP_TRY(hipMemcpy(r.items, sub.data(), sizeof(BatchItem) * sub.size(), hipMemcpyHostToDevice));
+            }
+            if ((rc = dev_alloc(&g.queue, 1))) return rc;
+            VMC_HIP_TRY(hipMemset(g.queue, 0, sizeof(unsigned int)));
+            // Persistent grid: as many workgroups as fit on the device at once.
+            int per_cu = 0;
+            const hipError_t oc =
+                b->real_bytes == 8
+                    ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_batch<double>, b->threads, g.lds)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_batch<float>, b->threads, g.lds);
+            if (oc != hipSuccess || per_cu < 1) per_cu = 1;
+            g.grid = static_cast<int>(std::min<int64_t>(total, static_cast<int64_t>(cus) * per_cu));
+        } else {
+            g.grid = g.total;
+        }
+        if (gi > 0) {
+            VMC_HIP_TRY(hipStreamCreateWithFlags(&g.lane, hipStreamNonBlocking));
+            VMC_HIP_TRY(hipEventCreateWithFlags(&g.join, hipEventDisableTiming));
+        }
+    }
+    return VMC_OK;
+}
+
+// One sweep of one group on `stream`.
+template <typename Real>
+int batch_group_step(vmc_batch *b, const BatchGroup &g, int flags, int block_size, hipStream_t stream) {
+    const int n = static_cast<int>(g.members.size());
+    constexpr bool f32 = sizeof(Real) == 4;
+    bool ok = true;
+    if (g.family == 0) {
+        hipLaunchKernelGGL((k_sweep_batch<Real>), dim3(static_cast<unsigned>(g.grid)), dim3(b->threads), g.lds,
+                           stream, g.items, n, g.total, g.queue, b->ctr, flags, block_size);
+        for (const auto &r : g.reinv_sets)
+            reinv_batch_launch(r.shape, r.items, r.n_items, static_cast<unsigned>(r.total), stream,
+                               static_cast<int>(sizeof(Real)));
+    } else if (g.family == 1) {
+        ok = f32 ? rr_batch_step_f32(g.shape, g.items, n, static_cast<unsigned>(g.total), b->ctr, flags, block_size,
+                                     g.reinv, stream)
+                 : rr_batch_step_f64(g.shape, g.items, n, static_cast<unsigned>(g.total), b->ctr, flags, block_size,
+                                     g.reinv, stream);
+    } else {
+        ok = f32 ? rrw_batch_step_f32(g.shape, g.items, n, static_cast<unsigned>(g.total), b->ctr, flags,
+                                      block_size, g.reinv, stream)
+                 : rrw_batch_step_f64(g.shape, g.items, n, static_cast<unsigned>(g.total), b->ctr, flags,
+                                      block_size, g.reinv, stream);
+    }
+    if (!ok) return fail(VMC_ERR_INVALID, "no register-resident instantiation for a batch group");
+    return VMC_OK;
+}
+
+// n_sweeps sweeps of every group.  With overlap, the groups after the first
+// run on their own streams, forked from and joined back into `stream` around
+// each sweep (also under graph capture, where they become parallel branches).
 template <typename Real>
 int batch_launch(vmc_batch *b, int n_sweeps, int flags, int block_size, hipStream_t stream) {
+    const bool fan = b->overlap && b->groups.size() > 1;
+    unsigned int *queue = nullptr;
+    for (const auto &g : b->groups)
+        if (g.family == 0) queue = g.queue;
     for (int i = 0; i < n_sweeps; ++i) {
-        hipLaunchKernelGGL((k_sweep_batch<Real>), dim3(static_cast<unsigned>(b->grid)), dim3(b->threads),
-                           b->lds, stream, b->items, static_cast<int>(b->members.size()), b->total,
-                           b->queue, b->ctr, flags, block_size);
-        if (b->reinv)
-            hipLaunchKernelGGL((k_reinv_batch<Real>), dim3(static_cast<unsigned>(b->total)), dim3(kReinvThreads),
-                               b->reinv_lds, stream, b->items, static_cast<int>(b->members.size()), b->reinv_lds);
-        hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1), 0, stream, b->queue, b->ctr);
+        if (fan) VMC_HIP_TRY(hipEventRecord(b->fork, stream));
+        for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+            const BatchGroup &g = b->groups[gi];
+            const hipStream_t s = (fan && gi > 0) ? g.lane : stream;
+            if (s != stream) VMC_HIP_TRY(hipStreamWaitEvent(s, b->fork, 0));
+            const int rc = batch_group_step<Real>(b, g, flags, block_size, s);
+            if (rc) return rc;
+            if (s != stream) VMC_HIP_TRY(hipEventRecord(g.join, s));
+        }
+        if (fan)
+            for (size_t gi = 1; gi < b->groups.size(); ++gi)
+                VMC_HIP_TRY(hipStreamWaitEvent(stream, b->groups[gi].join, 0));
+        hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1), 0, stream, queue, b->ctr);
     }
     VMC_HIP_TRY(hipGetLastError());
     return VMC_OK;
@@ -1177,7 +1374,6 @@ int vmc_batch_create(const vmc_system *systems, const int64_t *n_walkers, const 
             return fail(VMC_ERR_INVALID, "a batch runs one working precision");
     auto *b = new vmc_batch();
     b->threads = walker_threads;
-    std::vector<BatchItem> items(static_cast<size_t>(n_systems));
     int64_t total = 0;
     for (int i = 0; i < n_systems; ++i) {
         vmc_ensemble *e = nullptr;
@@ -1188,55 +1384,68 @@ int vmc_batch_create(const vmc_system *systems, const int64_t *n_walkers, const 
         }
         e->in_batch = true;
         b->members.push_back(e);
-        const DevSys &d = e->tables.dev;
         b->real_bytes = e->tables.real_bytes;
         const int place = inv_placement ? inv_placement[i] : 0;  // 0 auto, 1 LDS, 2 HBM
-        const bool fits = lds_for(d, e->tables.real_bytes, true) <= kMaxLdsBytes;
-        if (place == 1 && !fits) {
+        if (place < 0 || place > 2) {
+            delete b;
+            return fail(VMC_ERR_INVALID, "system %d: inverse placement must be 0, 1 or 2", i);
+        }
+        if (place == 1 && lds_for(e->tables.dev, e->tables.real_bytes, true) > kMaxLdsBytes) {
             delete b;
             return fail(VMC_ERR_INVALID, "system %d: inverse does not fit in LDS", i);
         }
-        const bool inv_lds = place == 2 ? false : fits;
-        items[i].s = d;
-        items[i].st = e->st;
-        items[i].inv_lds = inv_lds ? 1 : 0;
-        items[i].walker_begin = static_cast<int>(total);
+        b->placement.push_back(place);
         total += n_walkers[i];
-        items[i].walker_end = static_cast<int>(total);
-        b->lds = std::max(b->lds, lds_for(d, e->tables.real_bytes, inv_lds));
-        if (d.reinv_k > 0) {
-            b->reinv = true;
-            const bool a_lds = reinv_lds_bytes(d.nmax, true) <= kMaxLdsBytes;
-            b->reinv_lds = std::max(b->reinv_lds, reinv_lds_bytes(d.nmax, a_lds));
-        }
     }
     if (total > (int64_t(1) << 30)) {
         delete b;
         return fail(VMC_ERR_INVALID, "too many walkers in one batch");
     }
-    b->total = static_cast<int>(total);
+    b->total = total;
     int rc;
-    if ((rc = dev_alloc(&b->items, items.size())) || (rc = dev_alloc(&b->queue, 1)) ||
-        (rc = dev_alloc(&b->ctr, 1))) {
+    if ((rc = dev_alloc(&b->ctr, 1))) {
         delete b;
         return rc;
     }
-    if (hipMemcpy(b->items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(b->queue, 0, sizeof(unsigned int)) != hipSuccess ||
-        hipMemset(b->ctr, 0, sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMemset(b->ctr, 0, sizeof(unsigned long long)) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
         delete b;
-        return fail(VMC_ERR_DEVICE, "batch initialisation copy failed");
+        return fail(VMC_ERR_DEVICE, "batch initialisation failed");
     }
-    // Persistent grid: as many workgroups as fit on the device at once.
-    int dev = 0, cus = 0, per_cu = 0;
-    VMC_HIP_TRY(hipGetDevice(&dev));
-    VMC_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    hipError_t oc = b->real_bytes == 8
-                        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_batch<double>, b->threads, b->lds)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_batch<float>, b->threads, b->lds);
-    if (oc != hipSuccess || per_cu < 1) per_cu = 1;
-    b->grid = static_cast<int>(std::min<int64_t>(total, static_cast<int64_t>(cus) * per_cu));
+    if ((rc = batch_build_groups(b))) {
+        delete b;
+        return rc;
+    }
     *out = b;
+    return VMC_OK;
+}
+
+int vmc_batch_set_kernel(vmc_batch *b, int32_t system, int32_t kernel) {
+    if (!b) return fail(VMC_ERR_INVALID, "batch is NULL");
+    if (kernel != VMC_KERNEL_AUTO && kernel != VMC_KERNEL_GENERIC && kernel != VMC_KERNEL_REGISTER)
+        return fail(VMC_ERR_INVALID, "unknown kernel %d", kernel);
+    if (system < -1 || system >= static_cast<int32_t>(b->members.size()))
+        return fail(VMC_ERR_INVALID, "system %d out of range", system);
+    for (size_t i = 0; i < b->members.size(); ++i) {
+        if (system >= 0 && static_cast<int32_t>(i) != system) continue;
+        vmc_ensemble *e = b->members[i];
+        if (kernel == VMC_KERNEL_REGISTER && batch_family(e) == 0 &&
+            !rrw_supported_sizes(e->tables.dev.ndet, e->tables.dev.n_up, e->tables.dev.n_dn, e->tables.dev.ng,
+                                 e->tables.real_bytes) &&
+            !rr_supported(e->tables.dev))
+            return fail(VMC_ERR_INVALID, "system %zu has no register-resident instantiation", i);
+    }
+    VMC_HIP_TRY(hipDeviceSynchronize());
+    for (size_t i = 0; i < b->members.size(); ++i)
+        if (system < 0 || static_cast<int32_t>(i) == system) b->members[i]->kernel = kernel;
+    return batch_build_groups(b);
+}
+
+int vmc_batch_set_overlap(vmc_batch *b, int32_t overlap) {
+    if (!b) return fail(VMC_ERR_INVALID, "batch is NULL");
+    VMC_HIP_TRY(hipDeviceSynchronize());
+    b->drop_graph();
+    b->overlap = overlap != 0;
     return VMC_OK;
 }
 
@@ -1312,10 +1521,27 @@ int vmc_batch_member(vmc_batch *b, int32_t system, vmc_ensemble **out) {
 int vmc_batch_info(const vmc_batch *b, int32_t *n_systems, int64_t *total_walkers, int32_t *grid,
                    int64_t *lds_bytes) {
     if (!b) return fail(VMC_ERR_INVALID, "batch is NULL");
+    int64_t wg = 0, lds = 0;
+    for (const auto &g : b->groups) {
+        wg += g.grid;
+        lds = std::max<int64_t>(lds, static_cast<int64_t>(g.lds));
+    }
     if (n_systems) *n_systems = static_cast<int32_t>(b->members.size());
     if (total_walkers) *total_walkers = b->total;
-    if (grid) *grid = b->grid;
-    if (lds_bytes) *lds_bytes = static_cast<int64_t>(b->lds);
+    if (grid) *grid = static_cast<int32_t>(wg);
+    if (lds_bytes) *lds_bytes = lds;
+    return VMC_OK;
+}
+
+int vmc_batch_groups(const vmc_batch *b, int32_t *n_groups, int32_t *family, int32_t *group_of, int32_t cap) {
+    if (!b) return fail(VMC_ERR_INVALID, "batch is NULL");
+    if (n_groups) *n_groups = static_cast<int32_t>(b->groups.size());
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        if (family && static_cast<int32_t>(gi) < cap) family[gi] = b->groups[gi].family;
+        if (group_of)
+            for (int m : b->groups[gi].members)
+                if (m < cap) group_of[m] = static_cast<int32_t>(gi);
+    }
     return VMC_OK;
 }
 

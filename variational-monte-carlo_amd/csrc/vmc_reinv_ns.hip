@@ -134,11 +134,10 @@ __device__ void ns_build(const DevSys &s, const double *gp, int d, double *E, in
 }
 
 template <int NT>
-__global__ __launch_bounds__(kNSThreads) void k_reinv_ns(DevSys s, DevState st) {
+__device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &st, int w) {
     using S = NSShared<NT>;
     using T = NSTiles<NT>;
     constexpr int P = S::P, PE = S::PE, PF = S::PF;
-    const int w = blockIdx.x;
     if (s.reinv_k <= 0 || st.since[w] < s.reinv_k) return;
     __shared__ S sh;
     const int t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
@@ -245,8 +244,36 @@ __global__ __launch_bounds__(kNSThreads) void k_reinv_ns(DevSys s, DevState st) 
 }
 
 template <int NT>
-void launch_ns(const DevSys &d, const DevState &st, unsigned grid, hipStream_t stream) {
-    hipLaunchKernelGGL((k_reinv_ns<NT>), dim3(grid), dim3(kNSThreads), 0, stream, d, st);
+__global__ __launch_bounds__(kNSThreads) void k_reinv_ns(DevSys s, DevState st) {
+    reinv_ns_body<NT>(s, st, blockIdx.x);
+}
+
+// Batch form over a group of systems of one size.
+template <int NT>
+__global__ __launch_bounds__(kNSThreads) void k_reinv_ns_b(const BatchItem *__restrict__ items, int n_items) {
+    const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
+    reinv_ns_body<NT>(it.s, it.st, blockIdx.x - it.walker_begin);
+}
+
+template <int NT>
+void launch_ns(const DevSys &d, const DevState &st, const BatchItem *items, int n_items, unsigned grid,
+               hipStream_t stream) {
+    if (items)
+        hipLaunchKernelGGL((k_reinv_ns_b<NT>), dim3(grid), dim3(kNSThreads), 0, stream, items, n_items);
+    else
+        hipLaunchKernelGGL((k_reinv_ns<NT>), dim3(grid), dim3(kNSThreads), 0, stream, d, st);
+}
+
+void launch_any(const DevSys &d, const DevState &st, const BatchItem *items, int n_items, unsigned grid,
+                hipStream_t stream) {
+    switch ((d.nmax + 15) / 16) {
+        case 1: launch_ns<1>(d, st, items, n_items, grid, stream); break;
+        case 2: launch_ns<2>(d, st, items, n_items, grid, stream); break;
+        case 3: launch_ns<3>(d, st, items, n_items, grid, stream); break;
+        case 4: launch_ns<4>(d, st, items, n_items, grid, stream); break;
+        case 5: launch_ns<5>(d, st, items, n_items, grid, stream); break;
+        default: launch_ns<6>(d, st, items, n_items, grid, stream); break;
+    }
 }
 
 }  // namespace
@@ -256,14 +283,12 @@ bool reinv_ns_supported(const DevSys &d, int real_bytes) {
 }
 
 void reinv_ns_launch(const DevSys &d, const DevState &st, unsigned grid, hipStream_t stream) {
-    switch ((d.nmax + 15) / 16) {
-        case 1: launch_ns<1>(d, st, grid, stream); break;
-        case 2: launch_ns<2>(d, st, grid, stream); break;
-        case 3: launch_ns<3>(d, st, grid, stream); break;
-        case 4: launch_ns<4>(d, st, grid, stream); break;
-        case 5: launch_ns<5>(d, st, grid, stream); break;
-        default: launch_ns<6>(d, st, grid, stream); break;
-    }
+    launch_any(d, st, nullptr, 0, grid, stream);
+}
+
+void reinv_ns_batch_launch(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
+                           hipStream_t stream) {
+    launch_any(shape, DevState{}, items, n_items, grid, stream);
 }
 
 }  // namespace vmc

@@ -43,21 +43,45 @@ void rr_launch_g(const DevSys &d, const DevState &st, unsigned grid, int n_sweep
     }
 }
 
+// Calls f.template operator()<NU, ND, GPL>() for the system's instantiation;
+// false when there is none.
+template <typename Real, typename F>
+bool rr_dispatch(const DevSys &d, F &&f) {
+    if (!rr_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng)) return false;
+    if (d.ndet == 2) {
+        if (d.n_up == 27) f.template operator()<27, 27, 5>();
+        else if (d.n_up == 19) f.template operator()<19, 19, 5>();
+        else f.template operator()<7, 7, 5>();
+    } else {
+        if (d.n_up == 27) f.template operator()<27, 0, 12>();
+        else if (d.n_up == 19) f.template operator()<19, 0, 12>();
+        else f.template operator()<7, 0, 12>();
+    }
+    return true;
+}
+
 template <typename Real>
 bool rr_launch(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,
                unsigned long long s0, int flags, int bs, int variant, hipStream_t stream,
                KernelTimer &timer) {
-    if (!rr_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng)) return false;
-    if (d.ndet == 2) {
-        if (d.n_up == 27) rr_launch_g<Real, 27, 27, 5>(d, st, grid, n_sweeps, s0, flags, bs, variant, stream, timer);
-        else if (d.n_up == 19) rr_launch_g<Real, 19, 19, 5>(d, st, grid, n_sweeps, s0, flags, bs, variant, stream, timer);
-        else rr_launch_g<Real, 7, 7, 5>(d, st, grid, n_sweeps, s0, flags, bs, variant, stream, timer);
-    } else {
-        if (d.n_up == 27) rr_launch_g<Real, 27, 0, 12>(d, st, grid, n_sweeps, s0, flags, bs, variant, stream, timer);
-        else if (d.n_up == 19) rr_launch_g<Real, 19, 0, 12>(d, st, grid, n_sweeps, s0, flags, bs, variant, stream, timer);
-        else rr_launch_g<Real, 7, 0, 12>(d, st, grid, n_sweeps, s0, flags, bs, variant, stream, timer);
-    }
-    return true;
+    return rr_dispatch<Real>(d, [&]<int NU, int ND, int GPL>() {
+        rr_launch_g<Real, NU, ND, GPL>(d, st, grid, n_sweeps, s0, flags, bs, variant, stream, timer);
+    });
+}
+
+template <typename Real>
+bool rr_batch_step(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
+                   const unsigned long long *ctr, int flags, int bs, bool reinv, hipStream_t stream) {
+    return rr_dispatch<Real>(shape, [&]<int NU, int ND, int GPL>() {
+        hipLaunchKernelGGL((k_sweep_rr_b<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream, items,
+                           n_items, ctr, flags);
+        if (reinv)
+            hipLaunchKernelGGL((k_reinv_rr_b<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream, items,
+                               n_items);
+        if (flags & VMC_SWEEP_MEASURE)
+            hipLaunchKernelGGL((k_measure_rr_b<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream,
+                               items, n_items, bs);
+    });
 }
 
 }  // namespace vmc

@@ -73,6 +73,12 @@ inline bool rrw_supported_sizes(int ndet, int n_up, int n_dn, int ng, int real_b
 // determinants up to 96 electrons (vmc_reinv_ns.hip).
 bool reinv_ns_supported(const DevSys &d, int real_bytes);
 void reinv_ns_launch(const DevSys &d, const DevState &st, unsigned grid, hipStream_t stream);
+void reinv_ns_batch_launch(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
+                           hipStream_t stream);
+// The re-inversion of a batch group of one size (Newton-Schulz where
+// supported, else Gauss-Jordan), `grid` = the group's walkers.
+void reinv_batch_launch(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
+                        hipStream_t stream, int real_bytes);
 
 // The FP64 re-inversion after a sweep of the multi-wavefront register and the
 // generic kernels: Newton-Schulz where supported, else Gauss-Jordan (defined
@@ -92,6 +98,20 @@ bool rrw_launch_f32(const DevSys &d, const DevState &st, unsigned grid, int n_sw
 bool rrw_launch_f64(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,
                     unsigned long long s0, int flags, int bs, hipStream_t stream, KernelTimer &timer,
                     const ReinvLaunch &reinv);
+
+// One sweep of a batch group of register-resident systems of one size
+// (`shape` = any member's tables, `grid` = the group's walkers): the sweep,
+// the re-inversion when `reinv`, and with VMC_SWEEP_MEASURE the local energy,
+// the sweep index read from *sweep_ctr.  False when the size has no
+// instantiation.
+bool rr_batch_step_f32(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
+                       const unsigned long long *sweep_ctr, int flags, int bs, bool reinv, hipStream_t stream);
+bool rr_batch_step_f64(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
+                       const unsigned long long *sweep_ctr, int flags, int bs, bool reinv, hipStream_t stream);
+bool rrw_batch_step_f32(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
+                        const unsigned long long *sweep_ctr, int flags, int bs, bool reinv, hipStream_t stream);
+bool rrw_batch_step_f64(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
+                        const unsigned long long *sweep_ctr, int flags, int bs, bool reinv, hipStream_t stream);
 
 // Launches n_sweeps sweeps (and, with VMC_SWEEP_MEASURE, one measurement per
 // sweep).  variant 0: no occupancy bound; 1: at most 128 VGPRs.

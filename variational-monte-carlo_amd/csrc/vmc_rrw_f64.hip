@@ -7,4 +7,8 @@ bool rrw_launch_f64(const DevSys &d, const DevState &st, unsigned grid, int n_sw
                    const ReinvLaunch &reinv) {
     return rrw_launch<double>(d, st, grid, n_sweeps, s0, flags, bs, stream, timer, reinv);
 }
+bool rrw_batch_step_f64(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
+                       const unsigned long long *sweep_ctr, int flags, int bs, bool reinv, hipStream_t stream) {
+    return rrw_batch_step<double>(shape, items, n_items, grid, sweep_ctr, flags, bs, reinv, stream);
+}
 }  // namespace vmc

@@ -739,14 +739,13 @@ __device__ __forceinline__ void rr_proposal(const DevSys &s, double sigma, doubl
 __device__ __attribute__((noinline)) double rr_exp_f64(double x) { return exp(x); }
 
 template <typename Real, int NU, int ND, int GPL>
-__device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &st,
+__device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &st, int w,
                                               unsigned long long sweep0, int flags) {
     using K = RRKernel<Real, NU, ND, GPL>;
     constexpr int NMAX = K::NMAX;
     constexpr int N = NU + ND;
     __shared__ RRShared<Real, NU, ND> sh;
     const RRConst<Real> cst(s);
-    const int w = blockIdx.x;
     const int lane = threadIdx.x;
     const int h = lane >> 5, r = lane & 31;
     const int nh = h ? ND : NU;
@@ -829,13 +828,13 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
 template <typename Real, int NU, int ND, int GPL>
 __global__ __launch_bounds__(kWave) void k_sweep_rr(DevSys s, DevState st,
                                                     unsigned long long sweep0, int flags) {
-    sweep_rr_body<Real, NU, ND, GPL>(s, st, sweep0, flags);
+    sweep_rr_body<Real, NU, ND, GPL>(s, st, blockIdx.x, sweep0, flags);
 }
 
 template <typename Real, int NU, int ND, int GPL>
 __global__ __launch_bounds__(kWave, 4) void k_sweep_rr_o4(DevSys s, DevState st,
                                                           unsigned long long sweep0, int flags) {
-    sweep_rr_body<Real, NU, ND, GPL>(s, st, sweep0, flags);
+    sweep_rr_body<Real, NU, ND, GPL>(s, st, blockIdx.x, sweep0, flags);
 }
 
 typedef double rr_v4d __attribute__((ext_vector_type(4)));
@@ -1019,10 +1018,9 @@ __device__ double ns_reinvert(const DevSys &s, RRReinvShared &ps,
 // re-inversions; the largest drift |I - D^-1 D| met before one is kept per
 // walker in st.drift.  Walkers not due leave at once.
 template <typename Real, int NU, int ND, int GPL>
-__global__ __launch_bounds__(kWave) void k_reinv_rr(DevSys s, DevState st) {
+__device__ __forceinline__ void reinv_rr_body(const DevSys &s, const DevState &st, int w) {
     constexpr int NMAX = NU > ND ? NU : ND;
     constexpr int N = NU + ND;
-    const int w = blockIdx.x;
     if (s.reinv_k <= 0 || st.since[w] < s.reinv_k) return;
     __shared__ RRReinvShared ps;
     const int lane = threadIdx.x;
@@ -1058,13 +1056,13 @@ __global__ __launch_bounds__(kWave) void k_reinv_rr(DevSys s, DevState st) {
 // as the sweep, feeding the FP64 accumulators and the blocking state
 // (blocking_analysis.cpp:17-39).
 template <typename Real, int NU, int ND, int GPL>
-__global__ __launch_bounds__(kWave) void k_measure_rr(DevSys s, DevState st, int block_size) {
+__device__ __forceinline__ void measure_rr_body(const DevSys &s, const DevState &st, int w,
+                                                int block_size) {
     using K = RRKernel<Real, NU, ND, GPL>;
     constexpr int NMAX = K::NMAX;
     constexpr int N = NU + ND;
     __shared__ RRShared<Real, NU, ND> sh;
     const RRConst<Real> cst(s);
-    const int w = blockIdx.x;
     const int lane = threadIdx.x;
     const int h = lane >> 5, r = lane & 31;
     const int nh = h ? ND : NU;
@@ -1106,6 +1104,42 @@ __global__ __launch_bounds__(kWave) void k_measure_rr(DevSys s, DevState st, int
             t[9] = 0.0;
         }
     }
+}
+
+template <typename Real, int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave) void k_reinv_rr(DevSys s, DevState st) {
+    reinv_rr_body<Real, NU, ND, GPL>(s, st, blockIdx.x);
+}
+
+template <typename Real, int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave) void k_measure_rr(DevSys s, DevState st, int block_size) {
+    measure_rr_body<Real, NU, ND, GPL>(s, st, blockIdx.x, block_size);
+}
+
+// Batch forms: one launch covers the walkers of every system of a batch group
+// (systems of one size); workgroup g runs walker g - walker_begin of the
+// system whose range holds g, with that system's tables and state, and the
+// sweep index comes from device memory (so a captured graph continues the
+// chains).  A system's chain is the same in a batch and alone.
+template <typename Real, int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave) void k_sweep_rr_b(const BatchItem *__restrict__ items, int n_items,
+                                                      const unsigned long long *__restrict__ sweep_ctr,
+                                                      int flags) {
+    const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
+    sweep_rr_body<Real, NU, ND, GPL>(it.s, it.st, blockIdx.x - it.walker_begin, *sweep_ctr, flags);
+}
+
+template <typename Real, int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave) void k_reinv_rr_b(const BatchItem *__restrict__ items, int n_items) {
+    const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
+    reinv_rr_body<Real, NU, ND, GPL>(it.s, it.st, blockIdx.x - it.walker_begin);
+}
+
+template <typename Real, int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave) void k_measure_rr_b(const BatchItem *__restrict__ items, int n_items,
+                                                        int block_size) {
+    const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
+    measure_rr_body<Real, NU, ND, GPL>(it.s, it.st, blockIdx.x - it.walker_begin, block_size);
 }
 
 }  // namespace vmc

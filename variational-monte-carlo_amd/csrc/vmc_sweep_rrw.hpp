@@ -108,9 +108,8 @@ struct RRW {
         return t;
     }
 
-    __device__ static void load(const DevSys &s, const DevState &st, Sh &sh, const Thread &T,
+    __device__ static void load(const DevSys &s, const DevState &st, int w, Sh &sh, const Thread &T,
                                 Real (&inv)[NMAX], double &x, double &y, double &z) {
-        const int w = blockIdx.x;
         const Real *blk = reinterpret_cast<const Real *>(st.inv) + (size_t)w * s.ninv;
         const double *gp = st.pos + (size_t)w * 3 * N;
 #pragma unroll
@@ -276,19 +275,17 @@ __device__ __forceinline__ void rrw_proposal(const DevSys &s, double sigma, doub
 }
 
 template <typename Real, int NU, int ND, int WPS>
-__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_sweep_rrw(DevSys s, DevState st,
-                                                                          unsigned long long sweep0,
-                                                                          int flags) {
+__device__ __forceinline__ void sweep_rrw_body(const DevSys &s, const DevState &st, int w,
+                                               unsigned long long sweep0, int flags) {
     using K = RRW<Real, NU, ND, WPS>;
     constexpr int N = K::N;
     constexpr int NMAX = K::NMAX;
     __shared__ typename K::Sh sh;
     const RRConst<Real> cst(s);
     const typename K::Thread T;
-    const int w = blockIdx.x;
     Real inv[NMAX];
     double x, y, z;
-    K::load(s, st, sh, T, inv, x, y, z);
+    K::load(s, st, w, sh, T, inv, x, y, z);
     if (threadIdx.x < kStatCount) sh.stat[threadIdx.x] = st.stats[(size_t)w * kStatCount + threadIdx.x];
     if (threadIdx.x == 0) sh.misc[0] = st.sigma[w];
     __syncthreads();
@@ -335,8 +332,8 @@ __global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_sweep_rrw(DevSy
 // structure factor from per-electron axis-power tables staged in LDS, the
 // workgroup's threads owning G vectors.
 template <typename Real, int NU, int ND, int WPS>
-__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_measure_rrw(DevSys s, DevState st,
-                                                                            int block_size) {
+__device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState &st, int w,
+                                                 int block_size) {
     using K = RRW<Real, NU, ND, WPS>;
     constexpr int N = K::N;
     constexpr int NMAX = K::NMAX;
@@ -345,10 +342,9 @@ __global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_measure_rrw(Dev
     __shared__ typename K::Sh sh;
     const RRConst<Real> c(s);
     const typename K::Thread T;
-    const int w = blockIdx.x;
     Real inv[NMAX];
     double x, y, z;
-    K::load(s, st, sh, T, inv, x, y, z);
+    K::load(s, st, w, sh, T, inv, x, y, z);
     __syncthreads();
     // Pair terms: Jastrow gradient / Laplacian and real-space Ewald.
     double gx = 0, gy = 0, gz = 0, lap = 0, vreal = 0;
@@ -498,6 +494,35 @@ __global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_measure_rrw(Dev
             t[9] = 0.0;
         }
     }
+}
+
+template <typename Real, int NU, int ND, int WPS>
+__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_sweep_rrw(DevSys s, DevState st,
+                                                                          unsigned long long sweep0,
+                                                                          int flags) {
+    sweep_rrw_body<Real, NU, ND, WPS>(s, st, blockIdx.x, sweep0, flags);
+}
+
+template <typename Real, int NU, int ND, int WPS>
+__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_measure_rrw(DevSys s, DevState st,
+                                                                            int block_size) {
+    measure_rrw_body<Real, NU, ND, WPS>(s, st, blockIdx.x, block_size);
+}
+
+// Batch forms over a group of systems of one size (as k_sweep_rr_b).
+template <typename Real, int NU, int ND, int WPS>
+__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_sweep_rrw_b(
+    const BatchItem *__restrict__ items, int n_items, const unsigned long long *__restrict__ sweep_ctr,
+    int flags) {
+    const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
+    sweep_rrw_body<Real, NU, ND, WPS>(it.s, it.st, blockIdx.x - it.walker_begin, *sweep_ctr, flags);
+}
+
+template <typename Real, int NU, int ND, int WPS>
+__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_measure_rrw_b(
+    const BatchItem *__restrict__ items, int n_items, int block_size) {
+    const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
+    measure_rrw_body<Real, NU, ND, WPS>(it.s, it.st, blockIdx.x - it.walker_begin, block_size);
 }
 
 }  // namespace vmc

@@ -205,6 +205,9 @@ def lib():
             "vmc_batch_member": (C.c_int, [_VP, C.c_int32, C.POINTER(_VP)]),
             "vmc_batch_info": (C.c_int, [_VP, _ip, C.POINTER(C.c_int64), _ip,
                                          C.POINTER(C.c_int64)]),
+            "vmc_batch_set_kernel": (C.c_int, [_VP, C.c_int32, C.c_int32]),
+            "vmc_batch_set_overlap": (C.c_int, [_VP, C.c_int32]),
+            "vmc_batch_groups": (C.c_int, [_VP, _ip, _ip, _ip, C.c_int32]),
             "vmc_ensemble_set_timing": (C.c_int, [_VP, C.c_int32]),
             "vmc_ensemble_read_timing": (C.c_int, [_VP, _dp, C.POINTER(C.c_int64)]),
             "vmc_ensemble_info": (C.c_int, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
@@ -405,7 +408,9 @@ def blocking_summary(stats):
 
 
 class Batch:
-    """Systems of uneven size swept by one persistent launch (vmc_batch)."""
+    """Systems of uneven size swept together (vmc_batch): one launch per
+    kernel per sweep for each group of register-resident systems of one size,
+    one persistent launch for the systems on the generic kernel."""
 
     def __init__(self, systems, walkers, seeds, step0=None, placement=None, stream=None,
                  walker_threads=64):
@@ -460,6 +465,24 @@ class Batch:
         tw, lds = C.c_int64(), C.c_int64()
         _check(lib().vmc_batch_info(self._h, C.byref(ns), C.byref(tw), C.byref(gr), C.byref(lds)))
         return dict(n_systems=ns.value, total_walkers=tw.value, grid=gr.value, lds_bytes=lds.value)
+
+    def set_kernel(self, kernel, system=-1):
+        """Kernel of one system (or all): KERNEL_AUTO, KERNEL_GENERIC or
+        KERNEL_REGISTER; regroups the batch and drops a captured graph."""
+        _check(lib().vmc_batch_set_kernel(self._h, int(system), int(kernel)))
+
+    def set_overlap(self, on):
+        _check(lib().vmc_batch_set_overlap(self._h, 1 if on else 0))
+
+    def groups(self):
+        """(family of each group, group of each system); family 0 generic,
+        1 register, 2 multi-wavefront register."""
+        n = len(self.systems)
+        ng = C.c_int32()
+        fam = np.zeros(n, dtype=np.int32)
+        grp = np.zeros(n, dtype=np.int32)
+        _check(lib().vmc_batch_groups(self._h, C.byref(ng), _ptr(fam, _ip), _ptr(grp, _ip), n))
+        return fam[:ng.value].tolist(), grp.tolist()
 
 
 def footprint(system: System, n_walkers: int) -> int:
