@@ -374,7 +374,10 @@ NS_SYSTEMS = [
     ("heg_N66_rs2_register", lambda V: V.System.heg(66, 2.0, 1.0, precision=1, reinvert_interval=16), 2),
     ("heg_N114_rs5_register", lambda V: V.System.heg(114, 5.0, 0.8, precision=1, reinvert_interval=16), 2),
     ("heg_N162_rs3_register", lambda V: V.System.heg(162, 3.0, 1.2, precision=1, reinvert_interval=16), 2),
+    ("heg_N246_rs4_register", lambda V: V.System.heg(246, 4.0, 1.0, precision=1, reinvert_interval=16), 2),
+    ("heg_N294_rs4_register", lambda V: V.System.heg(294, 4.0, 1.0, precision=1, reinvert_interval=16), 2),
     ("heg_N114_rs5_generic", lambda V: V.System.heg(114, 5.0, 0.8, precision=1, reinvert_interval=16), 1),
+    ("heg_N246_rs4_generic", lambda V: V.System.heg(246, 4.0, 1.0, precision=1, reinvert_interval=16), 1),
     ("ref_N57_generic", lambda V: dataclasses.replace(V.System.reference(57, 12.0, precision=1),
                                                       reinvert_interval=16), 1),
 ]
@@ -382,8 +385,9 @@ NS_SYSTEMS = [
 
 @pytest.mark.parametrize("name,factory,kernel", NS_SYSTEMS, ids=[s[0] for s in NS_SYSTEMS])
 def test_matrix_core_reinversion_reaches_fp32_rounding(vmc, oracle, name, factory, kernel):
-    """FP32 inverses of 33 to 81 electrons per spin are re-inverted by
-    Newton-Schulz steps on the FP64 matrix cores.  With K = 16 every walker
+    """FP32 inverses of 33 to 147 electrons per spin are re-inverted by
+    Newton-Schulz steps on the FP64 matrix cores (operands in LDS up to 96 per
+    spin, E and the residual through the caches above).  With K = 16 every walker
     re-inverts after every sweep, and the local energy is measured right
     after that, so its error against the FP64 oracle shows the quality of the
     re-inverted inverse: within 1e-6 of the sum of the magnitudes of its

@@ -55,6 +55,7 @@ struct DevState {
     int *flags;       // [W] bit0: singular determinant met
     double *scratch;  // [W][nmax^2] FP64 workspace
     double *drift;    // [W] largest |I - D^-1 D| met before a re-inversion
+    float *resid;     // [W][P^2] Newton-Schulz residual above 96 per spin (mixed), else null
 };
 
 // One system of a batch group: its tables and state, whether its inverses

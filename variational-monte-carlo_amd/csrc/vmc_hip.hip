@@ -677,6 +677,7 @@ struct vmc_ensemble {
         if (st.flags) (void)hipFree(st.flags);
         if (st.scratch) (void)hipFree(st.scratch);
         if (st.drift) (void)hipFree(st.drift);
+        if (st.resid) (void)hipFree(st.resid);
     }
 };
 
@@ -794,7 +795,8 @@ int vmc_ensemble_footprint(const vmc_system *sys, int64_t n_walkers, int64_t *by
     const int64_t nmax = std::max(sys->n_up, sys->n_dn);
     const int64_t rb = sys->precision == VMC_PREC_MIXED ? 4 : 8;
     const int64_t per_walker = 3 * N * 8 + ninv * rb + nmax * nmax * 8 + kStatCount * 8 +
-                               8 /* sigma */ + 16 /* logdet */ + 8 /* since, flags */ + 8 /* drift */;
+                               8 /* sigma */ + 16 /* logdet */ + 8 /* since, flags */ + 8 /* drift */ +
+                               4 * ns_resid_floats(static_cast<int>(nmax), static_cast<int>(rb));
     const int64_t tables = 2 * nmax * 16 + static_cast<int64_t>(ng) * (16 + 8);
     *bytes = n_walkers * per_walker + tables;
     return VMC_OK;
@@ -824,6 +826,11 @@ int vmc_ensemble_create(const vmc_system *sys, int64_t n_walkers, uint64_t seed,
         (rc = dev_alloc(&st.sigma, W)) || (rc = dev_alloc(&st.stats, W * kStatCount)) ||
         (rc = dev_alloc(&st.logdet, W * 2)) || (rc = dev_alloc(&st.since, W)) ||
         (rc = dev_alloc(&st.flags, W)) || (rc = dev_alloc(&st.scratch, W * d.nmax * d.nmax)) || (rc = dev_alloc(&st.drift, W))) {
+        delete e;
+        return rc;
+    }
+    if (const int64_t nr = ns_resid_floats(d.nmax, e->tables.real_bytes); nr > 0 && d.reinv_k > 0 &&
+                                                                          (rc = dev_alloc(&st.resid, W * static_cast<size_t>(nr)))) {
         delete e;
         return rc;
     }

@@ -1,7 +1,6 @@
 // vmc_reinv_ns.hip -- periodic FP64 re-inversion of FP32 inverses on the
-// matrix cores, for determinants of up to 96 electrons (the multi-wavefront
-// register kernels' 33, 57 and 81 per spin, and the generic kernel's sizes up
-// to 96).
+// matrix cores, for determinants of up to 160 electrons (the multi-wavefront
+// register kernels' 33 to 147 per spin, and the generic kernel's sizes).
 //
 // One 256-thread workgroup per due walker (accepted moves since its last
 // re-inversion >= K); walkers not due leave at once.  Per determinant of order
@@ -243,6 +242,194 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
     }
 }
 
+// Determinants of 97 to 160 electrons: the same Newton-Schulz steps with only
+// Y (FP32) in LDS; E (FP64, row pitch n, the identity padding supplied on
+// load) lives in the walker's Gauss-Jordan scratch and R = I - Z (FP32, row
+// pitch P) in its residual buffer, both read through the caches.  The wave
+// grid is 2 x 2, each wave an (NT/2) x (NT/2) block of output tiles with the
+// next k-step's operands loaded before the current step's MFMAs.
+template <int NT>
+struct NSGShared {
+    static constexpr int P = 16 * NT;
+    static constexpr int PF = P + 1;
+    float Y[P * PF];  // also Gauss-Jordan's gjcol and perm
+    double red[2 * kNSWaves];
+};
+
+template <int NT, typename LA, typename LB>
+__device__ __forceinline__ void nsg_mm(LA load_a, LB load_b, ns_v4d acc[(NT / 2) * (NT / 2)], int wv,
+                                       int lane) {
+    constexpr int H = NT / 2;
+    const int lo = lane & 15, hi = lane >> 4;
+    const int r0 = (wv >> 1) * H, c0 = (wv & 1) * H;
+#pragma unroll
+    for (int i = 0; i < H * H; ++i) acc[i] = ns_v4d{0.0, 0.0, 0.0, 0.0};
+    double a[H], b[H], an[H], bn[H];
+#pragma unroll
+    for (int u = 0; u < H; ++u) {
+        a[u] = load_a(16 * (r0 + u) + lo, hi);
+        b[u] = load_b(hi, 16 * (c0 + u) + lo);
+    }
+#pragma unroll 1
+    for (int kk = 0; kk < 4 * NT; ++kk) {
+        const int kn = 4 * (kk + 1 < 4 * NT ? kk + 1 : kk) + hi;
+#pragma unroll
+        for (int u = 0; u < H; ++u) {
+            an[u] = load_a(16 * (r0 + u) + lo, kn);
+            bn[u] = load_b(kn, 16 * (c0 + u) + lo);
+        }
+#pragma unroll
+        for (int u = 0; u < H; ++u)
+#pragma unroll
+            for (int v = 0; v < H; ++v)
+                acc[u * H + v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[v], acc[u * H + v], 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < H; ++u) {
+            a[u] = an[u];
+            b[u] = bn[u];
+        }
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &st, int w) {
+    using S = NSGShared<NT>;
+    constexpr int P = S::P, PF = S::PF, H = NT / 2;
+    // (an ensemble that re-inverts at these sizes always holds the residual buffer)
+    if (s.reinv_k <= 0 || st.since[w] < s.reinv_k || !st.resid) return;
+    __shared__ S sh;
+    const int t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
+    const int r0 = (wv >> 1) * H, c0 = (wv & 1) * H;
+    const double *gp = st.pos + (size_t)w * 3 * s.N;
+    float *inv = reinterpret_cast<float *>(st.inv) + (size_t)w * s.ninv;
+    double *E = st.scratch + (size_t)w * s.nmax * s.nmax;
+    float *R = st.resid + (size_t)w * P * P;
+    double drift = 0.0, ld[2] = {st.logdet[2 * w + 0], st.logdet[2 * w + 1]};
+    bool singular = false;
+    for (int d = 0; d < s.ndet; ++d) {
+        const int n = s.n[d];
+        float *Yg = inv + s.inv_off[d];
+        ns_build<NT>(s, gp, d, E, n, false);
+        for (int e = t; e < P * P; e += kNSThreads) {
+            const int i = e / P, j = e - (e / P) * P;
+            sh.Y[i * PF + j] = (i < n && j < n) ? Yg[i * n + j] : (i == j ? 1.0f : 0.0f);
+        }
+        __syncthreads();
+        // E with its identity padding; the load stays inside the n x n block.
+        const auto load_e = [&](int r, int k) {
+            const bool in = r < n && k < n;
+            const double v = E[in ? r * n + k : 0];
+            return in ? v : (r == k ? 1.0 : 0.0);
+        };
+        const auto load_y = [&](int r, int k) { return (double)sh.Y[r * PF + k]; };
+        const auto load_r = [&](int r, int k) { return (double)R[r * P + k]; };
+        double d0 = 0.0;
+        bool ok = false;
+#pragma unroll 1
+        for (int it = 0; it < kNSSteps; ++it) {
+            ns_v4d z[H * H];
+            nsg_mm<NT>(load_e, load_y, z, wv, lane);
+            // This lane's first row and column, opaque to loop-invariant code
+            // motion: otherwise every (tile, register) address and bound test
+            // below is hoisted out of the step loop and held in registers.
+            int rl = 16 * r0 + (lane >> 4), cl = 16 * c0 + (lane & 15);
+            asm volatile("" : "+v"(rl), "+v"(cl));
+            double dm = 0.0;
+#pragma unroll
+            for (int i = 0; i < H * H; ++i) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = rl + 16 * (i / H) + 4 * q;
+                    const int c = cl + 16 * (i % H);
+                    const double dv = (r == c ? 1.0 : 0.0) - z[i][q];
+                    if (r < n && c < n) dm = (fabs(dv) > dm || dv != dv) ? fabs(dv) : dm;
+                    R[r * P + c] = (float)dv;
+                }
+            }
+            dm = block_max(dm, sh.red);  // its barriers also publish R
+            if (it == 0) d0 = dm;
+            if (!(dm < 0.5)) break;
+            if (dm < 1e-15) {
+                ok = true;
+                break;
+            }
+            ns_v4d y[H * H];
+            nsg_mm<NT>(load_y, load_r, y, wv, lane);
+            float yn[H * H][4];
+            int yl = rl * PF + cl;
+            asm volatile("" : "+v"(yl));
+#pragma unroll
+            for (int i = 0; i < H * H; ++i)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    yn[i][q] = (float)((double)sh.Y[yl + (16 * (i / H) + 4 * q) * PF + 16 * (i % H)] + y[i][q]);
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < H * H; ++i)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) sh.Y[yl + (16 * (i / H) + 4 * q) * PF + 16 * (i % H)] = yn[i][q];
+            __syncthreads();
+            if (dm < kNSDone) {
+                ok = true;
+                break;
+            }
+        }
+        drift = (d0 > drift || d0 != d0) ? d0 : drift;
+        if (ok) {
+            for (int e = t; e < n * n; e += kNSThreads) {
+                const int i = e / n, j = e - (e / n) * n;
+                Yg[e] = sh.Y[i * PF + j];
+            }
+        } else {
+            // Gauss-Jordan from scratch on E (row pitch n) in FP64.
+            __syncthreads();
+            ns_build<NT>(s, gp, d, E, n, false);
+            __syncthreads();
+            double *gjcol = reinterpret_cast<double *>(sh.Y);
+            int *perm = reinterpret_cast<int *>(gjcol + P);
+            const double l = gj_invert(E, n, gjcol, perm, sh.red);
+            if (!(l > -INFINITY)) {
+                singular = true;
+                break;
+            }
+            ld[d] = l;
+            for (int e = t; e < n * n; e += kNSThreads) Yg[e] = (float)E[e];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        st.since[w] = 0;
+        st.stats[(size_t)w * kStatCount + 15] += 1.0;
+        if (singular) {
+            st.flags[w] |= 1;
+        } else {
+            st.logdet[2 * w + 0] = ld[0];
+            st.logdet[2 * w + 1] = ld[1];
+        }
+        if (drift > st.drift[w] || drift != drift) st.drift[w] = drift;
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(kNSThreads) void k_reinv_nsg(DevSys s, DevState st) {
+    reinv_nsg_body<NT>(s, st, blockIdx.x);
+}
+
+template <int NT>
+__global__ __launch_bounds__(kNSThreads) void k_reinv_nsg_b(const BatchItem *__restrict__ items, int n_items) {
+    const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
+    reinv_nsg_body<NT>(it.s, it.st, blockIdx.x - it.walker_begin);
+}
+
+template <int NT>
+void launch_nsg(const DevSys &d, const DevState &st, const BatchItem *items, int n_items, unsigned grid,
+                hipStream_t stream) {
+    if (items)
+        hipLaunchKernelGGL((k_reinv_nsg_b<NT>), dim3(grid), dim3(kNSThreads), 0, stream, items, n_items);
+    else
+        hipLaunchKernelGGL((k_reinv_nsg<NT>), dim3(grid), dim3(kNSThreads), 0, stream, d, st);
+}
+
 template <int NT>
 __global__ __launch_bounds__(kNSThreads) void k_reinv_ns(DevSys s, DevState st) {
     reinv_ns_body<NT>(s, st, blockIdx.x);
@@ -266,6 +453,8 @@ void launch_ns(const DevSys &d, const DevState &st, const BatchItem *items, int 
 
 void launch_any(const DevSys &d, const DevState &st, const BatchItem *items, int n_items, unsigned grid,
                 hipStream_t stream) {
+    if (d.nmax > 128) return launch_nsg<10>(d, st, items, n_items, grid, stream);
+    if (d.nmax > 96) return launch_nsg<8>(d, st, items, n_items, grid, stream);
     switch ((d.nmax + 15) / 16) {
         case 1: launch_ns<1>(d, st, items, n_items, grid, stream); break;
         case 2: launch_ns<2>(d, st, items, n_items, grid, stream); break;
@@ -279,7 +468,7 @@ void launch_any(const DevSys &d, const DevState &st, const BatchItem *items, int
 }  // namespace
 
 bool reinv_ns_supported(const DevSys &d, int real_bytes) {
-    return real_bytes == 4 && d.nmax <= 96;
+    return real_bytes == 4 && d.nmax <= kNSMaxOrder;
 }
 
 void reinv_ns_launch(const DevSys &d, const DevState &st, unsigned grid, hipStream_t stream) {
