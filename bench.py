@@ -227,7 +227,8 @@ def serving_run(V, precision, n_requests=40, scale=0.04, seed=5, rate_hz=8.0):
 def batch_run(V, torch, sweeps=20, seed=2024):
     """Uneven batch: 24 systems (N from {14,38,54,66,114,162}, r_s
     log-uniform in [1,10], b uniform in [0.5,2], 64-1024 walkers), mixed
-    precision, one persistent launch per sweep replayed from a HIP graph."""
+    precision, grouped by kernel family (one launch per kernel per group,
+    groups on forked streams), 10 sweeps per replay of a captured HIP graph."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -256,6 +257,40 @@ def batch_run(V, torch, sweeps=20, seed=2024):
                 ms_per_sweep=1e3 * dt / ((sweeps // 10) * 10))
 
 
+def configs_run(V, torch, sweeps=10):
+    """Per-sweep time (E_L every sweep) of the other configurations on the
+    default kernels: N=114 (r_s=5, b=0.8, 2048 walkers) in mixed precision
+    at K = 16, 64, 256 and in FP64, and N = 66, 162, 246, 294 (r_s=2, 1024
+    walkers) mixed at K=64 and FP64.  Per-kernel times from HIP events."""
+    cases = [(114, 5.0, 0.8, V.PREC_MIXED, K, 2048) for K in (16, 64, 256)]
+    cases.append((114, 5.0, 0.8, V.PREC_FP64, 0, 2048))
+    for n in (66, 162, 246, 294):
+        cases += [(n, 2.0, 1.0, V.PREC_MIXED, 64, 1024), (n, 2.0, 1.0, V.PREC_FP64, 0, 1024)]
+    out = {}
+    for n, rs, b, prec, K, W in cases:
+        s = V.System.heg(n, rs, b, precision=prec, reinvert_interval=K)
+        e = V.Ensemble(s, W, 3, step0=0.3 * rs)
+        e.set_walker_threads(V.default_walker_threads(s))
+        e.init_positions()
+        e.sweep(5, adapt=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.sweep(sweeps, measure=True)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / sweeps
+        e.timing()
+        e.set_timing(True)
+        e.sweep(sweeps, measure=True)
+        torch.cuda.synchronize()
+        kt = {k: v[0] / v[1] * 1e3 for k, v in e.timing().items() if v[1]}
+        key = f"N{n}_{'mixed_K' + str(K) if prec == V.PREC_MIXED else 'fp64'}"
+        out[key] = dict(walkers=W, ms_per_sweep=dt * 1e3, moves_per_s=W * n / dt,
+                        kernel="register" if e.kernel() == V.KERNEL_REGISTER else "generic",
+                        us_per_launch={k: round(v, 1) for k, v in kt.items()})
+        e.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -267,7 +302,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--mixed-only", action="store_true")
-    ap.add_argument("--no-serving", action="store_true", help="skip the served-stream and batch runs")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the served-stream, batch and other-configuration runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run plain device copies of 1 GiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -365,6 +401,7 @@ def main():
         line["latency_ms"]["solve_what"] = ("request latency (submit to result) of a Poisson stream of "
                                             "40 energy evaluations through the request server, mixed")
         line["batch"] = batch_run(V, torch)
+        line["configs"] = configs_run(V, torch)
     line["physics"] = {k: {"e_per_electron": v["e_per_electron"],
                            "se_per_electron": v["se_per_electron"],
                            "acceptance": v["acceptance"],
