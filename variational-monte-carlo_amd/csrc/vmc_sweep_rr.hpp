@@ -299,16 +299,15 @@ struct RRConst {
     }
     // Minimum-image separation a - b in the working precision: FP64 is the
     // reference's masked form (jastrow_pade.cpp:18-24); FP32 rounds the FP64
-    // difference once and folds it with FP32 selects.
+    // difference once and subtracts the nearest multiple of L (three VALU
+    // operations, no compare-select hazards).  The two differ only at
+    // |a - b| = L/2 exactly, where both are minimum images of equal length.
     __device__ Real mi_r(double a, double b) const {
         if constexpr (sizeof(Real) == 8) {
             return (Real)mi(a - b);
         } else {
-            Real d = (Real)(a - b);
-            const Real l = (Real)L, hl = (Real)half_L;
-            d = d <= -hl ? d + l : d;
-            d = d > hl ? d - l : d;
-            return d;
+            const Real d = (Real)(a - b);
+            return __builtin_fmaf(-(Real)L, __builtin_rintf(d * (Real)inv_L), d);
         }
     }
     // erfc(alpha r) / r with the r < 1e-12 guard and optional cutoff
@@ -784,12 +783,15 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
     const uint32_t sweep = (uint32_t)sweep0;
     if (has_e) rr_proposal<Real>(s, sh.misc[0], L.x, L.y, L.z, e, w, sweep, lane, &sh.prop[0][0], &sh.pph[0][0]);
     __syncthreads();
+    // One loop per spin: a single loop with a branch per move lets the
+    // compiler merge the two moves' Sherman-Morrison tails, splitting each
+    // FMA into a multiply and a subtract.
     int acc = 0;
 #pragma unroll 1
-    for (int p = 0; p < N; ++p) {
-        const int accepted = (ND == 0 || p < NU) ? K::template move<0>(cst, sh, L, lane, has_e, h, r, p)
-                                                 : K::template move<(ND > 0 ? 1 : 0)>(cst, sh, L, lane, has_e, h, r, p - NU);
-        acc += accepted;
+    for (int p = 0; p < NU; ++p) acc += K::template move<0>(cst, sh, L, lane, has_e, h, r, p);
+    if constexpr (ND > 0) {
+#pragma unroll 1
+        for (int p = 0; p < ND; ++p) acc += K::template move<1>(cst, sh, L, lane, has_e, h, r, p);
     }
     const int n_acc = acc;
     since += acc;
