@@ -111,10 +111,16 @@ struct RRW {
     __device__ static void load(const DevSys &s, const DevState &st, int w, Sh &sh, const Thread &T,
                                 Real (&inv)[NMAX], double &x, double &y, double &z) {
         const Real *blk = reinterpret_cast<const Real *>(st.inv) + (size_t)w * s.ninv;
-        const double *gp = st.pos + (size_t)w * 3 * N;
 #pragma unroll
         for (int m = 0; m < NMAX; ++m)
             inv[m] = (T.has_e && m < T.nh) ? blk[(T.h ? NU * NU : 0) + T.r * T.nh + m] : (Real)0;
+        load_pos(s, st, w, sh, T, x, y, z);
+    }
+
+    // The thread's electron position, also into the LDS mirror.
+    __device__ static void load_pos(const DevSys &, const DevState &st, int w, Sh &sh, const Thread &T,
+                                    double &x, double &y, double &z) {
+        const double *gp = st.pos + (size_t)w * 3 * N;
         x = T.has_e ? gp[T.e] : 0.0;
         y = T.has_e ? gp[N + T.e] : 0.0;
         z = T.has_e ? gp[2 * N + T.e] : 0.0;
@@ -342,9 +348,21 @@ __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState
     __shared__ typename K::Sh sh;
     const RRConst<Real> c(s);
     const typename K::Thread T;
-    Real inv[NMAX];
+    // Above 96 per spin the inverse row is read where the Slater loop needs
+    // it (cached global loads) instead of being held in registers.
+    constexpr bool kRowsInMemory = NMAX > 96;
+    Real inv[kRowsInMemory ? 1 : NMAX];
     double x, y, z;
-    K::load(s, st, w, sh, T, inv, x, y, z);
+    if constexpr (kRowsInMemory)
+        K::load_pos(s, st, w, sh, T, x, y, z);
+    else
+        K::load(s, st, w, sh, T, inv, x, y, z);
+    const Real *row = reinterpret_cast<const Real *>(st.inv) + (size_t)w * s.ninv + (T.h ? NU * NU : 0) +
+                      (T.has_e ? T.r * T.nh : 0);
+    const auto inv_at = [&](int m) -> Real {
+        if constexpr (kRowsInMemory) return row[m];
+        else return inv[m];
+    };
     __syncthreads();
     // Pair terms: Jastrow gradient / Laplacian and real-space Ewald.
     double gx = 0, gy = 0, gz = 0, lap = 0, vreal = 0;
@@ -393,11 +411,11 @@ __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState
             if (K::orb[m].t == 1) continue;
             const rr::Cx<Real> ph = K::phasor(K::orb[m].x, K::orb[m].y, K::orb[m].z, a);
             const Real k2 = ku * ku * (Real)(K::orb[m].x * K::orb[m].x + K::orb[m].y * K::orb[m].y + K::orb[m].z * K::orb[m].z);
-            const Real wc = inv[m];
+            const Real wc = inv_at(m);
             Real gsc = -wc * ph.im;
             Real lpc = -wc * ph.re;
             if (m + 1 < NMAX && K::orb[m + 1].t == 1) {
-                const Real ws = inv[m + 1];
+                const Real ws = inv_at(m + 1);
                 gsc += ws * ph.re;
                 lpc -= ws * ph.im;
             }
@@ -496,22 +514,35 @@ __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState
     }
 }
 
+// Waves per SIMD the sweep kernel is compiled for.  57 + 57 and 81 + 81 in
+// FP32 take four (at most 128 VGPRs, 16 and 32 B of spill per lane): eight
+// and four walkers per CU, so 2048 and 1024 walkers are resident at once.
 template <typename Real, int NU, int ND, int WPS>
-__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_sweep_rrw(DevSys s, DevState st,
+constexpr int rrw_sweep_waves() {
+    return (sizeof(Real) == 4 && ((NU == 57 && ND == 57) || (NU == 81 && ND == 81))) ? 4 : 1;
+}
+
+template <typename Real, int NU, int ND, int WPS>
+__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64, (rrw_sweep_waves<Real, NU, ND, WPS>())) void k_sweep_rrw(DevSys s, DevState st,
                                                                           unsigned long long sweep0,
                                                                           int flags) {
     sweep_rrw_body<Real, NU, ND, WPS>(s, st, blockIdx.x, sweep0, flags);
 }
 
 template <typename Real, int NU, int ND, int WPS>
-__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_measure_rrw(DevSys s, DevState st,
+constexpr int rrw_measure_waves() {
+    return (sizeof(Real) == 4 && ((NU == 81 && ND == 81) || (NU == 123 && ND == 123))) ? 4 : 1;
+}
+
+template <typename Real, int NU, int ND, int WPS>
+__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64, (rrw_measure_waves<Real, NU, ND, WPS>())) void k_measure_rrw(DevSys s, DevState st,
                                                                             int block_size) {
     measure_rrw_body<Real, NU, ND, WPS>(s, st, blockIdx.x, block_size);
 }
 
 // Batch forms over a group of systems of one size (as k_sweep_rr_b).
 template <typename Real, int NU, int ND, int WPS>
-__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_sweep_rrw_b(
+__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64, (rrw_sweep_waves<Real, NU, ND, WPS>())) void k_sweep_rrw_b(
     const BatchItem *__restrict__ items, int n_items, const unsigned long long *__restrict__ sweep_ctr,
     int flags) {
     const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
@@ -519,7 +550,7 @@ __global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_sweep_rrw_b(
 }
 
 template <typename Real, int NU, int ND, int WPS>
-__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_measure_rrw_b(
+__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64, (rrw_measure_waves<Real, NU, ND, WPS>())) void k_measure_rrw_b(
     const BatchItem *__restrict__ items, int n_items, int block_size) {
     const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
     measure_rrw_body<Real, NU, ND, WPS>(it.s, it.st, blockIdx.x - it.walker_begin, block_size);
