@@ -28,6 +28,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_measure_rr_N54_W4096_traffic.md
 # MI355X published dense peaks.
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
+PEAK_LDS_TBPS = 256 * 128 * 2.4e9 / 1e12  # 256 CUs x 128 B/clk x 2.4 GHz
 
 
 def measure_flops_per_walker(n, N, ng):
@@ -53,6 +54,21 @@ def sweep_flops_per_move(n, N, p):
     row 6 n, ratio 2 n, new Jastrow pair terms 15 (N-1); on accept (rate p)
     Sherman-Morrison 4 n^2 and the Jastrow row-sum update 15 (N-1)."""
     return 6 * n + 2 * n + 15 * (N - 1) + p * (4 * n * n + 15 * (N - 1))
+
+
+def path_work_per_move(n, nk, N, ng, p):
+    """Work per proposed move of the reference algorithm on this workload,
+    independent of this implementation (add/mul = 1, FMA = 2): orbital phases
+    6 n_k, ratio 2 n, new Jastrow pairs 40 (N-1); on accept (rate p) the
+    Sherman-Morrison update 4 n^2, the incremental structure factor 26 N_G and
+    the real-space energy update 60 (N-1); the local energy once per sweep,
+    (16 n^2 + 30 N (N-1)) / N.  LDS bytes per move of an LDS-resident FP32
+    design: ratio column 4 n, on accept the update 8 n^2 and S(G) 16 N_G, and
+    three position passes 36 N.  Returns (flops, lds_bytes)."""
+    flops = (6 * nk + 2 * n + 40 * (N - 1) + p * (4 * n * n + 26 * ng + 60 * (N - 1))
+             + (16 * n * n + 30 * N * (N - 1)) / N)
+    lds = 4 * n + p * (8 * n * n + 16 * ng) + 36 * N
+    return flops, lds
 
 
 def read_traffic():
@@ -304,6 +320,8 @@ def main():
     ap.add_argument("--mixed-only", action="store_true")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the served-stream, batch and other-configuration runs")
+    ap.add_argument("--configs-only", action="store_true",
+                    help="run only the other-configuration timings (for profiling them)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run plain device copies of 1 GiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -315,6 +333,9 @@ def main():
     import vmc_amd as V
 
     V.device_check()
+    if a.configs_only:
+        print(json.dumps({"configs": configs_run(V, torch)}))
+        return
     if a.copy_calibration:
         # Two plain 1 GiB copies for the HBM byte counters: the runtime's
         # device-to-device copy and an elementwise kernel (per-lane vector
@@ -374,6 +395,17 @@ def main():
             "sweep_kernel": {"launch_ms_avg": head["sweep_ms_avg"],
                              "achieved": head["sweep_tflops"], "frac": head["sweep_tflops"] / peak,
                              "flops_per_move": head["sweep_flops_per_move"]}}
+        acc_rate = head["acceptance"]
+        f_move, lds_move = path_work_per_move(27, 14, 54, 257, acc_rate)
+        line["roofline"]["path"] = {
+            "what": "whole path at the measured electron-moves/s, work per move of the reference "
+                    "algorithm on this workload (n=27, n_k=14, N=54, N_G=257, acceptance p)",
+            "flops_per_move": f_move, "achieved": f_move * head["moves_per_s"] / 1e12,
+            "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": f_move * head["moves_per_s"] / 1e12 / PEAK_FP32_TFLOPS,
+            "lds_bytes_per_move": lds_move, "lds_achieved_TBps": lds_move * head["moves_per_s"] / 1e12,
+            "lds_peak_TBps": PEAK_LDS_TBPS,
+            "lds_frac": lds_move * head["moves_per_s"] / 1e12 / PEAK_LDS_TBPS}
         if "reinvert_ms_avg" in head:
             line["roofline"]["reinvert_kernel"] = {
                 "launch_ms_avg": head["reinvert_ms_avg"], "achieved": head["reinvert_tflops"],
