@@ -8,7 +8,8 @@
 //   E = D^T in FP64 (E[m][i] = phi_m(r_i), axis phasors of the FP64
 //       positions), Y = the FP32 inverse invT (= E^-1 up to its drift);
 //   repeat: Z = E Y on v_mfma_f64_16x16x4f64; the drift d = max|I - Z| over
-//       the n x n block (recorded on the first pass); R = I - Z;
+//       the n x n block (recorded on the first pass); R = I - Z (in E's
+//       place, E rebuilt if another step follows);
 //       Y <- Y + Y R (FP64 products, stored back in FP32);
 //   until d < 1e-4, after which the step just taken leaves d^2 < 1e-8, below
 //   the rounding of the FP32 inverse.  A drift of 0.5 or more, a non-finite
@@ -32,14 +33,16 @@ constexpr int kNSWaves = kNSThreads / kWave;
 constexpr int kNSSteps = 6;
 constexpr double kNSDone = 1e-4;
 
+// R = I - Z takes E's storage once Z = E Y is in registers (E is rebuilt
+// for a further step, rarely needed), so a workgroup holds E and Y only:
+// 50 KB at NT = 4, three workgroups per CU.
 template <int NT>
 struct NSShared {
     static constexpr int P = 16 * NT;   // padded order
     static constexpr int PE = P + 1;    // row pitch of E in doubles
     static constexpr int PF = P + 1;    // row pitch of Y and R in floats
-    double E[P * PE];
-    float Y[P * PF];
-    float R[P * PF];                    // also Gauss-Jordan's gjcol and perm
+    double E[P * PE];                   // also R (floats, pitch PF)
+    float Y[P * PF];                    // also Gauss-Jordan's gjcol and perm
     double red[2 * kNSWaves];
 };
 
@@ -139,6 +142,7 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
     constexpr int P = S::P, PE = S::PE, PF = S::PF;
     if (s.reinv_k <= 0 || st.since[w] < s.reinv_k) return;
     __shared__ S sh;
+    float *R = reinterpret_cast<float *>(sh.E);
     const int t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
     const double *gp = st.pos + (size_t)w * 3 * s.N;
     float *inv = reinterpret_cast<float *>(st.inv) + (size_t)w * s.ninv;
@@ -159,17 +163,23 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
         for (int it = 0; it < kNSSteps; ++it) {
             ns_v4d z[T::kCount];
             ns_mm<NT, double, PE, float, PF>(sh.E, sh.Y, z, wv, lane);
+            // The lane's row and column offsets, opaque to loop-invariant code
+            // motion (else every tile's addresses and bound tests are hoisted
+            // out of the step loop and held in registers).
+            int hl = lane >> 4, ll = lane & 15;
+            asm volatile("" : "+v"(hl), "+v"(ll));
+            __syncthreads();  // every wave is done reading E: R takes its place
             double dm = 0.0;
 #pragma unroll
             for (int i = 0; i < T::kCount; ++i) {
                 if (!T::valid(i, wv)) continue;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int r = 16 * T::row(i, wv) + 4 * q + (lane >> 4);
-                    const int c = 16 * T::col(i, wv) + (lane & 15);
+                    const int r = 16 * T::row(i, wv) + 4 * q + hl;
+                    const int c = 16 * T::col(i, wv) + ll;
                     const double dv = (r == c ? 1.0 : 0.0) - z[i][q];
                     if (r < n && c < n) dm = (fabs(dv) > dm || dv != dv) ? fabs(dv) : dm;
-                    sh.R[r * PF + c] = (float)dv;
+                    R[r * PF + c] = (float)dv;
                 }
             }
             dm = block_max(dm, sh.red);  // its barriers also publish R
@@ -180,14 +190,14 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
                 break;
             }
             ns_v4d y[T::kCount];
-            ns_mm<NT, float, PF, float, PF>(sh.Y, sh.R, y, wv, lane);
+            ns_mm<NT, float, PF, float, PF>(sh.Y, R, y, wv, lane);
             float yn[T::kCount][4];
 #pragma unroll
             for (int i = 0; i < T::kCount; ++i) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int r = 16 * T::row(i, wv) + 4 * q + (lane >> 4);
-                    const int c = 16 * T::col(i, wv) + (lane & 15);
+                    const int r = 16 * T::row(i, wv) + 4 * q + hl;
+                    const int c = 16 * T::col(i, wv) + ll;
                     yn[i][q] = T::valid(i, wv) ? (float)((double)sh.Y[r * PF + c] + y[i][q]) : 0.0f;
                 }
             }
@@ -197,14 +207,15 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
                 if (!T::valid(i, wv)) continue;
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    sh.Y[(16 * T::row(i, wv) + 4 * q + (lane >> 4)) * PF + 16 * T::col(i, wv) + (lane & 15)] =
-                        yn[i][q];
+                    sh.Y[(16 * T::row(i, wv) + 4 * q + hl) * PF + 16 * T::col(i, wv) + ll] = yn[i][q];
             }
-            __syncthreads();
             if (dm < kNSDone) {
                 ok = true;
+                __syncthreads();
                 break;
             }
+            ns_build<NT>(s, gp, d, sh.E, PE, true);  // R held E's place
+            __syncthreads();
         }
         drift = (d0 > drift || d0 != d0) ? d0 : drift;
         if (ok) {
@@ -217,7 +228,7 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
             __syncthreads();
             ns_build<NT>(s, gp, d, sh.E, n, false);
             __syncthreads();
-            double *gjcol = reinterpret_cast<double *>(sh.R);
+            double *gjcol = reinterpret_cast<double *>(sh.Y);
             int *perm = reinterpret_cast<int *>(gjcol + P);
             const double l = gj_invert(sh.E, n, gjcol, perm, sh.red);
             if (!(l > -INFINITY)) {
