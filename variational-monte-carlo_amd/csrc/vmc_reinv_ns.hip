@@ -466,6 +466,7 @@ void launch_any(const DevSys &d, const DevState &st, const BatchItem *items, int
                 hipStream_t stream) {
     if (d.nmax > 128) return launch_nsg<10>(d, st, items, n_items, grid, stream);
     if (d.nmax > 96) return launch_nsg<8>(d, st, items, n_items, grid, stream);
+    if (d.nmax > 80) return launch_nsg<6>(d, st, items, n_items, grid, stream);
     switch ((d.nmax + 15) / 16) {
         case 1: launch_ns<1>(d, st, items, n_items, grid, stream); break;
         case 2: launch_ns<2>(d, st, items, n_items, grid, stream); break;

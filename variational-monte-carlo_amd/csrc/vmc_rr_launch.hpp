@@ -70,12 +70,12 @@ inline bool rrw_supported_sizes(int ndet, int n_up, int n_dn, int ng, int real_b
 }
 
 // Newton-Schulz re-inversion on the matrix cores for FP32 inverses of
-// determinants up to kNSMaxOrder electrons (vmc_reinv_ns.hip); above 96 it
+// determinants up to kNSMaxOrder electrons (vmc_reinv_ns.hip); above 80 it
 // needs the ensemble's residual buffer (ns_resid_floats per walker).
 constexpr int kNSMaxOrder = 160;
 inline int64_t ns_resid_floats(int nmax, int real_bytes) {
-    if (real_bytes != 4 || nmax <= 96 || nmax > kNSMaxOrder) return 0;
-    const int64_t p = nmax > 128 ? 160 : 128;
+    if (real_bytes != 4 || nmax <= 80 || nmax > kNSMaxOrder) return 0;
+    const int64_t p = nmax > 128 ? 160 : nmax > 96 ? 128 : 96;
     return p * p;
 }
 bool reinv_ns_supported(const DevSys &d, int real_bytes);
