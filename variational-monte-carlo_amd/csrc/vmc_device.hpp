@@ -55,8 +55,17 @@ struct DevState {
     int *flags;       // [W] bit0: singular determinant met
     double *scratch;  // [W][nmax^2] FP64 workspace
     double *drift;    // [W] largest |I - D^-1 D| met before a re-inversion
-    float *resid;     // [W][P^2] Newton-Schulz residual above 96 per spin (mixed), else null
+    float *resid;     // [W][P^2] Newton-Schulz residual above 80 per spin (mixed), else null
+    double *jrow;     // [W][N] Jastrow row sums U_i kept by the register sweeps between sweeps
 };
+
+// Walker flag bits (DevState::flags).
+constexpr int kFlagSingular = 1;  // a singular determinant was met
+constexpr int kFlagJrowValid = 2;  // jrow holds the current positions' row sums
+// The register sweeps recompute the Jastrow row sums from the positions every
+// kJrowRefresh sweeps (bounding the drift of their incremental updates) and
+// whenever another kernel moved the walker.
+constexpr uint32_t kJrowRefresh = 16;
 
 // One system of a batch group: its tables and state, whether its inverses
 // live in LDS (generic kernel), and its walkers' range in the group's walker

@@ -215,7 +215,7 @@ __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kI
         for (int k = 0; k < kStatCount; ++k) ws[k] = stat[k];
         st.sigma[w] = sigma;
         st.since[w] = since;
-        st.flags[w] = wflags;
+        st.flags[w] = wflags & ~kFlagJrowValid;  // this kernel keeps no row sums
         st.logdet[2 * w + 0] = ld0;
         st.logdet[2 * w + 1] = ld1;
         st.drift[w] = drift;
@@ -678,6 +678,7 @@ struct vmc_ensemble {
         if (st.scratch) (void)hipFree(st.scratch);
         if (st.drift) (void)hipFree(st.drift);
         if (st.resid) (void)hipFree(st.resid);
+        if (st.jrow) (void)hipFree(st.jrow);
     }
 };
 
@@ -796,7 +797,8 @@ int vmc_ensemble_footprint(const vmc_system *sys, int64_t n_walkers, int64_t *by
     const int64_t rb = sys->precision == VMC_PREC_MIXED ? 4 : 8;
     const int64_t per_walker = 3 * N * 8 + ninv * rb + nmax * nmax * 8 + kStatCount * 8 +
                                8 /* sigma */ + 16 /* logdet */ + 8 /* since, flags */ + 8 /* drift */ +
-                               4 * ns_resid_floats(static_cast<int>(nmax), static_cast<int>(rb));
+                               4 * ns_resid_floats(static_cast<int>(nmax), static_cast<int>(rb)) +
+                               8 * N /* Jastrow row sums */;
     const int64_t tables = 2 * nmax * 16 + static_cast<int64_t>(ng) * (16 + 8);
     *bytes = n_walkers * per_walker + tables;
     return VMC_OK;
@@ -825,7 +827,8 @@ int vmc_ensemble_create(const vmc_system *sys, int64_t n_walkers, uint64_t seed,
     if ((rc = dev_alloc(&st.pos, W * 3 * d.N)) || (rc = dev_alloc(reinterpret_cast<unsigned char **>(&st.inv), W * d.ninv * e->tables.real_bytes)) ||
         (rc = dev_alloc(&st.sigma, W)) || (rc = dev_alloc(&st.stats, W * kStatCount)) ||
         (rc = dev_alloc(&st.logdet, W * 2)) || (rc = dev_alloc(&st.since, W)) ||
-        (rc = dev_alloc(&st.flags, W)) || (rc = dev_alloc(&st.scratch, W * d.nmax * d.nmax)) || (rc = dev_alloc(&st.drift, W))) {
+        (rc = dev_alloc(&st.flags, W)) || (rc = dev_alloc(&st.scratch, W * d.nmax * d.nmax)) || (rc = dev_alloc(&st.drift, W)) ||
+        (rc = dev_alloc(&st.jrow, W * d.N))) {
         delete e;
         return rc;
     }

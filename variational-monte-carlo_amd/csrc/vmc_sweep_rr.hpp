@@ -773,14 +773,19 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
         if (r == m)
             L.orb = (K::orb[m].x + 4) | ((K::orb[m].y + 4) << 4) | ((K::orb[m].z + 4) << 8) | (K::orb[m].t << 12);
     __syncthreads();
-    L.U = K::jastrow_row(cst, sh, L, lane, has_e, h);
     int since = st.since[w];
     int wflags = st.flags[w];
+    const uint32_t sweep = (uint32_t)sweep0;
+    // Jastrow row sums: kept from the previous sweep unless stale or due for
+    // the periodic recomputation (the decision is walker-uniform).
+    if (!(wflags & kFlagJrowValid) || (sweep % kJrowRefresh) == 0)
+        L.U = K::jastrow_row(cst, sh, L, lane, has_e, h);
+    else
+        L.U = has_e ? st.jrow[(size_t)w * N + e] : 0.0;
     // Proposal of the lane's electron for this sweep (electron order is fixed,
     // so its position is current until its own move); staged in LDS for the
     // move that reads it.  One sweep per launch keeps this straight-line code
     // out of the move loop, so its transcendental constants die before it.
-    const uint32_t sweep = (uint32_t)sweep0;
     if (has_e) rr_proposal<Real>(s, sh.misc[0], L.x, L.y, L.z, e, w, sweep, lane, &sh.prop[0][0], &sh.pph[0][0]);
     __syncthreads();
     // One loop per spin: a single loop with a branch per move lets the
@@ -809,6 +814,7 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
         gq[e] = L.x;
         gq[N + e] = L.y;
         gq[2 * N + e] = L.z;
+        st.jrow[(size_t)w * N + e] = L.U;
     }
     __syncthreads();
     if (lane == 0) {
@@ -817,7 +823,7 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
 
         st.sigma[w] = sh.misc[0];
         st.since[w] = since;
-        st.flags[w] = wflags;
+        st.flags[w] = wflags | kFlagJrowValid;
         st.logdet[2 * w + 0] = sh.misc[1];
         st.logdet[2 * w + 1] = sh.misc[2];
     }

@@ -295,7 +295,12 @@ __device__ __forceinline__ void sweep_rrw_body(const DevSys &s, const DevState &
     if (threadIdx.x < kStatCount) sh.stat[threadIdx.x] = st.stats[(size_t)w * kStatCount + threadIdx.x];
     if (threadIdx.x == 0) sh.misc[0] = st.sigma[w];
     __syncthreads();
-    double U = K::jastrow_row(cst, sh, T, x, y, z);
+    const int wflags = st.flags[w];
+    double U;
+    if (!(wflags & kFlagJrowValid) || ((uint32_t)sweep0 % kJrowRefresh) == 0)
+        U = K::jastrow_row(cst, sh, T, x, y, z);
+    else
+        U = T.has_e ? st.jrow[(size_t)w * N + T.e] : 0.0;
     const int ob = K::packed_orbital(T.r);
     if (T.has_e)
         rrw_proposal<Real>(s, sh.misc[0], x, y, z, T.e, w, (uint32_t)sweep0, N, &sh.prop[0][0], &sh.pph[0][0]);
@@ -320,6 +325,7 @@ __device__ __forceinline__ void sweep_rrw_body(const DevSys &s, const DevState &
         gq[T.e] = x;
         gq[N + T.e] = y;
         gq[2 * N + T.e] = z;
+        st.jrow[(size_t)w * N + T.e] = U;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -327,6 +333,7 @@ __device__ __forceinline__ void sweep_rrw_body(const DevSys &s, const DevState &
         sh.stat[7] += (double)N;
         st.sigma[w] = sh.misc[0];
         st.since[w] += acc;
+        st.flags[w] = wflags | kFlagJrowValid;
     }
     __syncthreads();
     if (threadIdx.x < kStatCount) st.stats[(size_t)w * kStatCount + threadIdx.x] = sh.stat[threadIdx.x];
