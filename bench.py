@@ -34,13 +34,13 @@ PEAK_LDS_TBPS = 256 * 128 * 2.4e9 / 1e12  # 256 CUs x 128 B/clk x 2.4 GHz
 def measure_flops_per_walker(n, N, ng):
     """Algorithmic flops of the local-energy kernel for one walker (add/mul =
     1, FMA = 2; sqrt, reciprocal, exp and erfc counted as 1 each):
-      ordered pairs N(N-1) x 40   (min image 3, r^2 5, r 1, 1/(1+br) 2, u 3,
-                                   u', u'' 6, gradient 6, Laplacian 5,
-                                   erfc(ar)/r 9)
+      unordered pairs N(N-1)/2 x 48   (shared: min image 3, r^2 5, r 1,
+                                       1/(1+br) 2, u', u'' 6, erfc(ar)/r 9;
+                                       each side: gradient 6, Laplacian 5)
       Slater terms N n x 14        (orbital phase 6, gradient 6, Laplacian 2)
       structure factor N N_G x 14 + 4 N_G   (e^{iG.r} from axis powers 12,
                                               accumulate 2; w |S|^2 4)"""
-    return 40 * N * (N - 1) + 14 * N * n + 14 * N * ng + 4 * ng
+    return 24 * N * (N - 1) + 14 * N * n + 14 * N * ng + 4 * ng
 
 
 def reinvert_flops_per_walker(n, ndet=2):

@@ -412,51 +412,95 @@ struct RRKernel {
     // Jastrow row sum, gradient and Laplacian of the lane's electron and its
     // half of the real-space Ewald sum (jastrow_pade.cpp:57-138,
     // energy_tracking.cpp:93-144), reading the other positions from LDS.
+    // Pair terms of the local energy with every unordered pair evaluated
+    // once: in step k (1 <= k <= N/2) electron e pairs with (e + k) mod N and
+    // keeps its own side of the Jastrow gradient and Laplacian; the partner's
+    // side (the gradient negated, the same Laplacian term) reaches it through
+    // a cross-lane permute from the lane of (e - k) mod N.  For even N the
+    // last step's pairs are met from both ends: each keeps its own side, and
+    // their energy counts half.  Terms are summed in the working precision
+    // within a chunk of kPairChunk steps, chunk sums in FP64.
     __device__ static void pair_terms(const RRConst<Real> &c, const Sh &sh,
                                       const RRLane<Real, NMAX> &L, int lane, bool has_e, int h,
-                                      double &U, double g[3], double &lap, double &vreal) {
-        U = 0.0;
+                                      double g[3], double &lap, double &vreal) {
         g[0] = g[1] = g[2] = 0.0;
         lap = 0.0;
         vreal = 0.0;
         const Real m2b = (Real)-2 * c.b;
-        // Partners in chunks of kPairChunk: terms summed in the working
-        // precision within a chunk, chunk sums in FP64.
+        constexpr int K = N / 2;
+        constexpr bool kEven = N % 2 == 0;
         constexpr int kPairChunk = sizeof(Real) == 4 ? 8 : 1;
-        for (int j0 = 0; j0 < N; j0 += kPairChunk) {
-            Real u_c = 0, gx_c = 0, gy_c = 0, gz_c = 0, l_c = 0, v_c = 0;
+        const int e = h ? NU + (lane & 31) : (lane & 31);
+        for (int k0 = 1; k0 <= K; k0 += kPairChunk) {
+            Real gx_c = 0, gy_c = 0, gz_c = 0, l_c = 0, v_c = 0;
 #pragma unroll
-            for (int jj = 0; jj < kPairChunk; ++jj) {
-                const int j = j0 + jj;
-                if (j >= N) break;
+            for (int kk = 0; kk < kPairChunk; ++kk) {
+                const int k = k0 + kk;
+                if (k > K) break;
+                const bool last_even = kEven && k == K;
+                int j = e + k;
+                j = j >= N ? j - N : j;
+                int src = e - k;
+                src = src < 0 ? src + N : src;
                 const int lj = lane_of(j);
-                const double xj = sh.pos[0][lj], yj = sh.pos[1][lj], zj = sh.pos[2][lj];
-                if (!has_e || lj == lane) continue;
-                const Real dx = c.mi_r(L.x, xj);
-                const Real dy = c.mi_r(L.y, yj);
-                const Real dz = c.mi_r(L.z, zj);
-                const Real r = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
-                const Real a = ((lj >> 5) == h) ? c.a_same : c.a_anti;
-                const Real den = wrcp<Real>((Real)1 + c.b * r);
-                u_c += a * r * den;
-                v_c += c.ewald(r);
-                if (r < (Real)1e-12) continue;
-                const Real d1 = a * den * den;
-                const Real inv_r = wrcp<Real>(r);
-                const Real gf = d1 * inv_r;
-                gx_c += gf * dx;
-                gy_c += gf * dy;
-                gz_c += gf * dz;
-                l_c += d1 * (m2b * den + (Real)2 * inv_r);
+                Real pgx = 0, pgy = 0, pgz = 0, pl = 0;
+                if (has_e) {
+                    const double xj = sh.pos[0][lj], yj = sh.pos[1][lj], zj = sh.pos[2][lj];
+                    const Real dx = c.mi_r(L.x, xj);
+                    const Real dy = c.mi_r(L.y, yj);
+                    const Real dz = c.mi_r(L.z, zj);
+                    const Real r = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+                    const Real a = ((lj >> 5) == h) ? c.a_same : c.a_anti;
+                    const Real den = wrcp<Real>((Real)1 + c.b * r);
+                    const Real v = c.ewald(r);
+                    v_c += last_even ? (Real)0.5 * v : v;
+                    if (r >= (Real)1e-12) {
+                        const Real d1 = a * den * den;
+                        const Real inv_r = wrcp<Real>(r);
+                        const Real gf = d1 * inv_r;
+                        const Real l = d1 * (m2b * den + (Real)2 * inv_r);
+                        gx_c += gf * dx;
+                        gy_c += gf * dy;
+                        gz_c += gf * dz;
+                        l_c += l;
+                        if (!last_even) {
+                            pgx = -gf * dx;
+                            pgy = -gf * dy;
+                            pgz = -gf * dz;
+                            pl = l;
+                        }
+                    }
+                }
+                // The partner's side, from the lane that paired with this
+                // lane's electron in this step (all lanes take part).
+                const int sl = (has_e ? lane_of(src) : lane) << 2;
+                const Real rgx = permute(sl, pgx), rgy = permute(sl, pgy);
+                const Real rgz = permute(sl, pgz), rl = permute(sl, pl);
+                if (has_e && !last_even) {
+                    gx_c += rgx;
+                    gy_c += rgy;
+                    gz_c += rgz;
+                    l_c += rl;
+                }
             }
-            U += (double)u_c;
             vreal += (double)v_c;
             g[0] += (double)gx_c;
             g[1] += (double)gy_c;
             g[2] += (double)gz_c;
             lap += (double)l_c;
         }
-        vreal *= 0.5;
+    }
+
+    // Value of `v` on the lane whose byte address is `addr` (ds_bpermute).
+    __device__ static Real permute(int addr, Real v) {
+        if constexpr (sizeof(Real) == 4) {
+            return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+        } else {
+            const long long b = __double_as_longlong(v);
+            const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffLL));
+            const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+            return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+        }
     }
 
     // Jastrow row sum U_i of the lane's electron (jastrow_pade.cpp:7-55 rows).
@@ -481,9 +525,8 @@ struct RRKernel {
     // kinetic / potential parts.
     __device__ static Measured measure(const DevSys &s, const RRConst<Real> &cst, Sh &sh,
                                        RRLane<Real, NMAX> &L, int lane, bool has_e, int h) {
-        double g[3], lap, vreal, U;
-        pair_terms(cst, sh, L, lane, has_e, h, U, g, lap, vreal);
-        if (has_e) L.U = U;
+        double g[3], lap, vreal;
+        pair_terms(cst, sh, L, lane, has_e, h, g, lap, vreal);
         // Slater gradient and Laplacian of ln|D_h| at the lane's electron:
         // sum_m invT[r][m] grad phi_m(r) with phi_m from the axis phasors
         // (slater_plane_wave.cpp:425-499).
