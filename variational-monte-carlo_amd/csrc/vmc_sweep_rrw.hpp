@@ -371,39 +371,83 @@ __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState
         else return inv[m];
     };
     __syncthreads();
-    // Pair terms: Jastrow gradient / Laplacian and real-space Ewald.
+    // Pair terms: Jastrow gradient / Laplacian and real-space Ewald, each
+    // unordered pair once.  In step k (1 <= k <= N/2) electron e pairs with
+    // (e + k) mod N and keeps its side; the partner's side (gradient negated,
+    // same Laplacian term) goes through LDS -- the sweep's proposal arrays,
+    // unused here -- kXch steps at a time between two barriers.  For even N
+    // the last step's pairs are met from both ends: each keeps its own side
+    // and their energy counts half.  Working-precision sums per chunk, FP64
+    // across chunks.
+    constexpr int kSteps = N / 2;
+    constexpr bool kEven = N % 2 == 0;
+    constexpr int kXch = (int)((sizeof(sh.prop) + sizeof(sh.pph)) / (4 * N * sizeof(Real)));
+    static_assert(kXch >= 1, "exchange buffer");
+    static_assert(offsetof(typename K::Sh, pph) == offsetof(typename K::Sh, prop) + sizeof(sh.prop),
+                  "prop and pph are contiguous");
+    Real *xch = reinterpret_cast<Real *>(&sh.prop[0][0]);  // [kXch][4][N]
     double gx = 0, gy = 0, gz = 0, lap = 0, vreal = 0;
-    if (T.has_e) {
+    {
         const Real m2b = (Real)-2 * c.b;
-        for (int j0 = 0; j0 < N; j0 += 8) {
+        for (int k0 = 1; k0 <= kSteps; k0 += kXch) {
             Real gx_c = 0, gy_c = 0, gz_c = 0, l_c = 0, v_c = 0;
 #pragma unroll
-            for (int jj = 0; jj < 8; ++jj) {
-                const int j = j0 + jj;
-                if (j >= N || j == T.e) continue;
+            for (int kk = 0; kk < kXch; ++kk) {
+                const int k = k0 + kk;
+                if (k > kSteps || !T.has_e) break;
+                const bool last_even = kEven && k == kSteps;
+                int j = T.e + k;
+                j = j >= N ? j - N : j;
                 const Real dx = c.mi_r(x, sh.pos[0][j]);
                 const Real dy = c.mi_r(y, sh.pos[1][j]);
                 const Real dz = c.mi_r(z, sh.pos[2][j]);
                 const Real r = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
                 const Real a = ((ND > 0 && j >= NU) ? 1 : 0) == T.h ? c.a_same : c.a_anti;
                 const Real den = wrcp<Real>((Real)1 + c.b * r);
-                v_c += c.ewald(r);
-                if (r < (Real)1e-12) continue;
-                const Real d1 = a * den * den;
-                const Real inv_r = wrcp<Real>(r);
-                const Real gf = d1 * inv_r;
-                gx_c += gf * dx;
-                gy_c += gf * dy;
-                gz_c += gf * dz;
-                l_c += d1 * (m2b * den + (Real)2 * inv_r);
+                const Real v = c.ewald(r);
+                v_c += last_even ? (Real)0.5 * v : v;
+                Real pgx = 0, pgy = 0, pgz = 0, pl = 0;
+                if (r >= (Real)1e-12) {
+                    const Real d1 = a * den * den;
+                    const Real inv_r = wrcp<Real>(r);
+                    const Real gf = d1 * inv_r;
+                    const Real l = d1 * (m2b * den + (Real)2 * inv_r);
+                    gx_c += gf * dx;
+                    gy_c += gf * dy;
+                    gz_c += gf * dz;
+                    l_c += l;
+                    pgx = -gf * dx;
+                    pgy = -gf * dy;
+                    pgz = -gf * dz;
+                    pl = l;
+                }
+                if (!last_even) {
+                    Real *slot = xch + (size_t)kk * 4 * N;
+                    slot[j] = pgx;
+                    slot[N + j] = pgy;
+                    slot[2 * N + j] = pgz;
+                    slot[3 * N + j] = pl;
+                }
             }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < kXch; ++kk) {
+                const int k = k0 + kk;
+                if (k > kSteps || !T.has_e) break;
+                if (kEven && k == kSteps) break;
+                const Real *slot = xch + (size_t)kk * 4 * N;
+                gx_c += slot[T.e];
+                gy_c += slot[N + T.e];
+                gz_c += slot[2 * N + T.e];
+                l_c += slot[3 * N + T.e];
+            }
+            __syncthreads();
             gx += (double)gx_c;
             gy += (double)gy_c;
             gz += (double)gz_c;
             lap += (double)l_c;
             vreal += (double)v_c;
         }
-        vreal *= 0.5;
     }
     // Slater gradient and Laplacian of ln|D_h| at the thread's electron.
     double sx = 0, sy = 0, sz = 0, sl = 0;
