@@ -62,3 +62,25 @@ def test_invalid_arguments_rejected_without_device(vmc_nodev):
         V.frozen_eval(V.System.reference(7, 5.0), pos, [9], [[1.0, 1.0, 1.0]])
     with pytest.raises(V.VmcError):
         V.frozen_eval(V.System.reference(7, 5.0), pos, [0], [[6.0, 1.0, 1.0]])
+
+
+def test_footprint_counts_every_per_walker_buffer(vmc_nodev):
+    """vmc_ensemble_footprint (the server's admission measure) counts what an
+    ensemble allocates per walker: positions, the working-precision inverses,
+    the FP64 Gauss-Jordan scratch, statistics, step state, drift, the Jastrow
+    row sums and -- for mixed ensembles above 80 electrons per spin that
+    re-invert -- the Newton-Schulz residual (padded order^2 floats)."""
+    V = vmc_nodev
+
+    def per_walker(s):
+        return V.footprint(s, 2) - V.footprint(s, 1)
+
+    N, n = 54, 27
+    s = V.System.heg(N, 2.0, 1.0, precision=V.PREC_MIXED, reinvert_interval=64)
+    assert per_walker(s) == 3 * N * 8 + 2 * n * n * 4 + n * n * 8 + 16 * 8 + 8 + 16 + 8 + 8 + 8 * N
+    s64 = V.System.heg(N, 2.0, 1.0, precision=V.PREC_FP64)
+    assert per_walker(s64) - per_walker(s) == 2 * n * n * 4
+    for N, p in ((162, 96), (246, 128), (294, 160)):
+        k = V.System.heg(N, 2.0, 1.0, precision=V.PREC_MIXED, reinvert_interval=64)
+        k0 = V.System.heg(N, 2.0, 1.0, precision=V.PREC_MIXED, reinvert_interval=0)
+        assert per_walker(k) - per_walker(k0) == p * p * 4

@@ -797,7 +797,9 @@ int vmc_ensemble_footprint(const vmc_system *sys, int64_t n_walkers, int64_t *by
     const int64_t rb = sys->precision == VMC_PREC_MIXED ? 4 : 8;
     const int64_t per_walker = 3 * N * 8 + ninv * rb + nmax * nmax * 8 + kStatCount * 8 +
                                8 /* sigma */ + 16 /* logdet */ + 8 /* since, flags */ + 8 /* drift */ +
-                               4 * ns_resid_floats(static_cast<int>(nmax), static_cast<int>(rb)) +
+                               (sys->reinvert_interval > 0
+                                    ? 4 * ns_resid_floats(static_cast<int>(nmax), static_cast<int>(rb))
+                                    : 0) +
                                8 * N /* Jastrow row sums */;
     const int64_t tables = 2 * nmax * 16 + static_cast<int64_t>(ng) * (16 + 8);
     *bytes = n_walkers * per_walker + tables;
