@@ -11,8 +11,8 @@
 //       the n x n block (recorded on the first pass); R = I - Z (in E's
 //       place, E rebuilt if another step follows);
 //       Y <- Y + Y R (FP64 products, stored back in FP32);
-//   until d < 1e-4, after which the step just taken leaves d^2 < 1e-8, below
-//   the rounding of the FP32 inverse.  A drift of 0.5 or more, a non-finite
+//   until d < 5e-4, after which the step just taken leaves d^2 < 2.5e-7, at
+//   the rounding floor of the FP32 inverse.  A drift of 0.5 or more, a non-finite
 //   one, or no convergence within kNSSteps falls back to Gauss-Jordan from
 //   scratch, which also refreshes log|det D|.
 // The identity padding keeps the padded block exact: Z, R and Y stay the
@@ -31,7 +31,10 @@ typedef double ns_v4d __attribute__((ext_vector_type(4)));
 constexpr int kNSThreads = 256;
 constexpr int kNSWaves = kNSThreads / kWave;
 constexpr int kNSSteps = 6;
-constexpr double kNSDone = 1e-4;
+// A step taken from drift d leaves about d^2 plus the rounding floor of an
+// FP32 inverse (max|I - E Y| of a correctly rounded inverse is a few 1e-7):
+// from d < 5e-4 one step reaches that floor.
+constexpr double kNSDone = 5e-4;
 
 // R = I - Z takes E's storage once Z = E Y is in registers (E is rebuilt
 // for a further step, rarely needed), so a workgroup holds E and Y only:

@@ -967,9 +967,9 @@ constexpr int kNSMaxSteps = 6;
 // with E = D^T (E[m][i] = phi_m(r_i), built from the positions in FP64) and
 // Y = invT, Z = E Y measures the drift d = |I - Z| (= |I - D^-1 D|), and
 // Y <- Y (2 I - Z) leaves a drift of about d^2.  Steps repeat (E rebuilt each
-// time) until d^2 is below the rounding of the stored inverse (1e-8 for an
-// FP32 inverse, 1e-16 for FP64): one step for the usual drift of 1e-6..1e-4,
-// at most kNSMaxSteps from any drift below 0.5.  A drift of 0.5 or more (or a
+// time) until d^2 is at the rounding floor of the stored inverse (a few 1e-7
+// in max|I - Z| for an FP32 inverse, 1e-16 for FP64): one step for drifts up
+// to 5e-4 in FP32, at most kNSMaxSteps from any drift below 0.5.  A drift of 0.5 or more (or a
 // non-finite one) falls back to Gauss-Jordan from scratch.
 // Returns the drift found before refinement (negative if singular); a
 // Gauss-Jordan rebuild also refreshes log|det D| into `logdet`.
@@ -988,7 +988,7 @@ __device__ double ns_reinvert(const DevSys &s, RRReinvShared &ps,
     }
     __syncthreads();
     constexpr int TT = NP > 16 ? 2 : 1;
-    constexpr double kDone = sizeof(Real) == 4 ? 1e-4 : 1e-8;  // d below this: d^2 is rounding
+    constexpr double kDone = sizeof(Real) == 4 ? 5e-4 : 1e-8;  // d below this: d^2 is at the rounding floor
     double drift0 = -1.0;
 #pragma unroll 1
     for (int it = 0; it < kNSMaxSteps; ++it) {
