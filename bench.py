@@ -277,7 +277,10 @@ def configs_run(V, torch, sweeps=10):
     """Per-sweep time (E_L every sweep) of the other configurations on the
     default kernels: N=114 (r_s=5, b=0.8, 2048 walkers) in mixed precision
     at K = 16, 64, 256 and in FP64, and N = 66, 162, 246, 294 (r_s=2, 1024
-    walkers) mixed at K=64 and FP64.  Per-kernel times from HIP events."""
+    walkers) mixed at K=64 and FP64.  Per-kernel times from HIP events; for
+    mixed precision the inverse drift met before re-inversions."""
+    import numpy as np
+
     cases = [(114, 5.0, 0.8, V.PREC_MIXED, K, 2048) for K in (16, 64, 256)]
     cases.append((114, 5.0, 0.8, V.PREC_FP64, 0, 2048))
     for n in (66, 162, 246, 294):
@@ -303,6 +306,9 @@ def configs_run(V, torch, sweeps=10):
         out[key] = dict(walkers=W, ms_per_sweep=dt * 1e3, moves_per_s=W * n / dt,
                         kernel="register" if e.kernel() == V.KERNEL_REGISTER else "generic",
                         us_per_launch={k: round(v, 1) for k, v in kt.items()})
+        if prec == V.PREC_MIXED:
+            d = e.drift()
+            out[key]["drift"] = {"max": float(d.max()), "median": float(np.median(d))}
         e.close()
     return out
 
