@@ -415,3 +415,32 @@ def test_matrix_core_reinversion_reaches_fp32_rounding(vmc, oracle, name, factor
         worst = max(worst, abs(st[w, vmc.STAT["LAST_E"]] - e) / scale)
     print(f"{name}: E_L error / parts max {worst:.2e}, drift max {d.max():.2e}")
     assert worst <= 1e-6, worst
+
+
+@pytest.mark.parametrize("name,factory", [("heg_N54_rs2", lambda V: V.System.heg(54, 2.0, 1.0)),
+                                          ("heg_N114_rs5", lambda V: V.System.heg(114, 5.0, 0.8))],
+                         ids=["heg_N54_rs2", "heg_N114_rs5"])
+def test_kernel_switches_keep_the_oracle_twin(vmc, oracle, name, factory):
+    """The register kernels keep each walker's Jastrow row sums between
+    sweeps (recomputed every 16 sweeps); a sweep of the generic kernel in
+    between moves the electrons without them and marks them stale.  A chain
+    that switches kernels (register, generic, register ...) over 20 sweeps,
+    across the 16-sweep recomputation, follows the oracle's FP64 twin move
+    for move."""
+    s = factory(vmc)
+    W, seed = 3, 24680
+    step = 0.3 * s.box_length / 10.0 * 3.0
+    ens = vmc.Ensemble(s, W, seed, step0=step)
+    ens.init_positions()
+    osys = oracle_system(oracle, s)
+    walkers = [oracle.Walker(osys, oracle.init_positions(osys, seed, w)[0]) for w in range(W)]
+    plan = [2, 2, 1, 2, 2, 2, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 1, 2, 2]
+    for sweep, kernel in enumerate(plan):
+        ens.set_kernel(kernel)
+        ens.sweep(1, measure=True, adapt=False)
+        st, pos = ens.stats(), ens.positions()
+        for w in range(W):
+            _, _, e = walkers[w].sweep(seed, w, sweep, step, move_kind=s.move_kind, adapt=False,
+                                       measure=True)
+            np.testing.assert_allclose(pos[w], walkers[w].positions(), rtol=0, atol=1e-9)
+            assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-9 * max(abs(e), 1e-3 * s.N), (sweep, w)
