@@ -8,12 +8,17 @@
 //   stats   double [W][16]     FP64 accumulators (VMC_STAT_*)
 //   logdet  double [W][2]
 //   since   int    [W]         accepted moves since the last FP64 re-inversion
-//   flags   int    [W]         bit0: singular determinant met
-//   scratch double [W][nmax^2] FP64 Gauss-Jordan workspace
+//   flags   int    [W]         bit0: singular determinant met; bit1: jrow current
+//   scratch double [W][nmax^2] FP64 Gauss-Jordan workspace (and Newton-Schulz E
+//                              above 80 electrons per spin)
 //   drift   double [W]         largest |I - D^-1 D| met at a re-inversion
-// One wavefront (one 64-lane workgroup) runs one walker; its positions,
-// Jastrow row sums and draws stay in LDS for a whole launch, and so do the
-// inverses when they fit.
+//   jrow    double [W][N]      Jastrow row sums kept by the register sweeps
+//   resid   float  [W][P^2]    Newton-Schulz residual, mixed above 80 per spin
+// The generic kernel in this file runs one walker per 64-, 128- or 256-thread
+// workgroup; its positions, Jastrow row sums and draws stay in LDS for a whole
+// launch, and so do the inverses when they fit.  The register-resident
+// kernels (vmc_sweep_rr.hpp, vmc_sweep_rrw.hpp) and the Newton-Schulz
+// re-inversion (vmc_reinv_ns.hip) live in their own translation units.
 #include "../../include/vmc_hip.h"
 #include "vmc_device.hpp"
 #include "vmc_tables.hpp"
