@@ -30,8 +30,8 @@ void rr_launch_g(const DevSys &d, const DevState &st, unsigned grid, int n_sweep
         if (timed) timer.end(stream, ev, kTimeSweep);
         if (reinv) {
             timed = timer.begin(stream, ev);
-            hipLaunchKernelGGL((k_reinv_rr<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream,
-                               d, st);
+            hipLaunchKernelGGL((k_reinv_rr<Real, NU, ND, GPL>), dim3(grid * (ND > 0 ? 2 : 1)), dim3(kWave), 0,
+                               stream, d, st);
             if (timed) timer.end(stream, ev, kTimeReinv);
         }
         if (measure) {
@@ -76,8 +76,8 @@ bool rr_batch_step(const DevSys &shape, const BatchItem *items, int n_items, uns
         hipLaunchKernelGGL((k_sweep_rr_b<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream, items,
                            n_items, ctr, flags);
         if (reinv)
-            hipLaunchKernelGGL((k_reinv_rr_b<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream, items,
-                               n_items);
+            hipLaunchKernelGGL((k_reinv_rr_b<Real, NU, ND, GPL>), dim3(grid * (ND > 0 ? 2 : 1)), dim3(kWave), 0,
+                               stream, items, n_items);
         if (flags & VMC_SWEEP_MEASURE)
             hipLaunchKernelGGL((k_measure_rr_b<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream,
                                items, n_items, bs);

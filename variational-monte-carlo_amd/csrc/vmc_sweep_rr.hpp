@@ -243,13 +243,15 @@ __device__ __forceinline__ Real *rr_stage(RRShared<Real, NU, ND> &sh) {
 // each block moves with coalesced loads or stores of consecutive elements and
 // every lane reads or writes its own row in LDS (odd row stride, no bank
 // conflicts).
-template <typename Real, int NU, int ND, int NMAX>
+// kOnly >= 0 moves spin kOnly's block alone.
+template <typename Real, int NU, int ND, int NMAX, int kOnly = -1>
 __device__ __forceinline__ void rr_inverse_in(const Real *blk, Real *stage, Real (&inv)[NMAX],
                                               int lane, int h, int r) {
 #pragma unroll
     for (int m = 0; m < NMAX; ++m) inv[m] = (Real)0;
 #pragma unroll
     for (int hp = 0; hp < (ND > 0 ? 2 : 1); ++hp) {
+        if (kOnly >= 0 && hp != kOnly) continue;
         constexpr int kU = NU;
         const int np = hp ? ND : kU;
         const Real *src = blk + (hp ? kU * kU : 0);
@@ -264,11 +266,12 @@ __device__ __forceinline__ void rr_inverse_in(const Real *blk, Real *stage, Real
     }
 }
 
-template <typename Real, int NU, int ND, int NMAX>
+template <typename Real, int NU, int ND, int NMAX, int kOnly = -1>
 __device__ __forceinline__ void rr_inverse_out(Real *blk, Real *stage, const Real (&inv)[NMAX],
                                                int lane, int h, int r) {
 #pragma unroll
     for (int hp = 0; hp < (ND > 0 ? 2 : 1); ++hp) {
+        if (kOnly >= 0 && hp != kOnly) continue;
         constexpr int kU = NU;
         const int np = hp ? ND : kU;
         __syncthreads();
@@ -817,6 +820,7 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
             L.orb = (K::orb[m].x + 4) | ((K::orb[m].y + 4) << 4) | ((K::orb[m].z + 4) << 8) | (K::orb[m].t << 12);
     __syncthreads();
     int since = st.since[w];
+    if (s.reinv_k > 0 && since >= s.reinv_k) since = 0;  // re-inverted after the last sweep
     int wflags = st.flags[w];
     const uint32_t sweep = (uint32_t)sweep0;
     // Jastrow row sums: kept from the previous sweep unless stale or due for
@@ -1068,8 +1072,14 @@ __device__ double ns_reinvert(const DevSys &s, RRReinvShared &ps,
 // accepted moves since its last one reached K.  Stat VMC_STAT_REINV counts
 // re-inversions; the largest drift |I - D^-1 D| met before one is kept per
 // walker in st.drift.  Walkers not due leave at once.
-template <typename Real, int NU, int ND, int GPL>
-__device__ __forceinline__ void reinv_rr_body(const DevSys &s, const DevState &st, int w) {
+// One spin of one walker: spin HP's FP64 re-inversion when the walker's
+// accepted moves since its last one reached K.  The two spins of a walker run
+// in two workgroups at once; they share no LDS and write disjoint inverse
+// blocks and log|det| entries, and only spin 0 counts the re-inversion.  The
+// walker's move count since re-inversion is cleared by its next sweep (both
+// workgroups read it here).
+template <typename Real, int NU, int ND, int GPL, int HP>
+__device__ __forceinline__ void reinv_rr_spin(const DevSys &s, const DevState &st, int w) {
     constexpr int NMAX = NU > ND ? NU : ND;
     constexpr int N = NU + ND;
     if (s.reinv_k <= 0 || st.since[w] < s.reinv_k) return;
@@ -1082,24 +1092,35 @@ __device__ __forceinline__ void reinv_rr_body(const DevSys &s, const DevState &s
     RRLane<Real, NMAX> L;
     Real *inv_hbm = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
     const double *gp = st.pos + (size_t)w * 3 * N;
-    rr_inverse_in<Real, NU, ND, NMAX>(inv_hbm, reinterpret_cast<Real *>(ps.E), L.inv, lane, h, r);
+    rr_inverse_in<Real, NU, ND, NMAX, HP>(inv_hbm, reinterpret_cast<Real *>(ps.E), L.inv, lane, h, r);
     L.x = has_e ? gp[e] : 0.0;
     L.y = has_e ? gp[N + e] : 0.0;
     L.z = has_e ? gp[2 * N + e] : 0.0;
-    double ld0 = st.logdet[2 * w + 0], ld1 = st.logdet[2 * w + 1];
-    double d = ns_reinvert<Real, NU, ND, GPL, 0>(s, ps, L, lane, h, r, ld0);
-    if constexpr (ND > 0) {
-        const double d1 = ns_reinvert<Real, NU, ND, GPL, 1>(s, ps, L, lane, h, r, ld1);
-        d = (d < 0.0 || d1 < 0.0) ? -1.0 : fmax(d, d1);
-    }
-    rr_inverse_out<Real, NU, ND, NMAX>(inv_hbm, reinterpret_cast<Real *>(ps.E), L.inv, lane, h, r);
+    double ld = st.logdet[2 * w + HP];
+    const double d = ns_reinvert<Real, NU, ND, GPL, HP>(s, ps, L, lane, h, r, ld);
+    rr_inverse_out<Real, NU, ND, NMAX, HP>(inv_hbm, reinterpret_cast<Real *>(ps.E), L.inv, lane, h, r);
     if (lane == 0) {
-        st.since[w] = 0;
-        st.logdet[2 * w + 0] = ld0;
-        st.logdet[2 * w + 1] = ld1;
-        if (d < 0.0) st.flags[w] |= 1;
-        st.stats[(size_t)w * kStatCount + 15] += 1.0;
-        if (d > st.drift[w]) st.drift[w] = d;
+        st.logdet[2 * w + HP] = ld;
+        if (d < 0.0) {
+            atomicOr(&st.flags[w], kFlagSingular);
+        } else {
+            // drift >= 0 (or +inf / NaN): its bit pattern orders as the value
+            atomicMax(reinterpret_cast<unsigned long long *>(&st.drift[w]),
+                      (unsigned long long)__double_as_longlong(d));
+        }
+        if (HP == 0) st.stats[(size_t)w * kStatCount + 15] += 1.0;
+    }
+}
+
+// Workgroup g of a re-inversion launch: walker g / 2, spin g % 2 (walker g
+// for one determinant).
+template <typename Real, int NU, int ND, int GPL>
+__device__ __forceinline__ void reinv_rr_body(const DevSys &s, const DevState &st, int g) {
+    if constexpr (ND > 0) {
+        if (g & 1) reinv_rr_spin<Real, NU, ND, GPL, 1>(s, st, g >> 1);
+        else reinv_rr_spin<Real, NU, ND, GPL, 0>(s, st, g >> 1);
+    } else {
+        reinv_rr_spin<Real, NU, ND, GPL, 0>(s, st, g);
     }
 }
 
@@ -1182,8 +1203,10 @@ __global__ __launch_bounds__(kWave) void k_sweep_rr_b(const BatchItem *__restric
 
 template <typename Real, int NU, int ND, int GPL>
 __global__ __launch_bounds__(kWave) void k_reinv_rr_b(const BatchItem *__restrict__ items, int n_items) {
-    const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
-    reinv_rr_body<Real, NU, ND, GPL>(it.s, it.st, blockIdx.x - it.walker_begin);
+    constexpr int kSpins = ND > 0 ? 2 : 1;
+    const int g = blockIdx.x / kSpins, sp = blockIdx.x % kSpins;
+    const BatchItem &it = items[batch_find(items, n_items, g)];
+    reinv_rr_body<Real, NU, ND, GPL>(it.s, it.st, (g - it.walker_begin) * kSpins + sp);
 }
 
 template <typename Real, int NU, int ND, int GPL>
