@@ -5,7 +5,8 @@
 // (wave v, lane l) owns row r = (v mod WPS) * 64 + l of spin h = v / WPS:
 // that electron's position, its Jastrow row sum U and its row of the spin's
 // transposed inverse in registers.  A move of electron p (spin hp) is three
-// phases separated by two workgroup barriers:
+// phases, A and B separated by a workgroup barrier only when a spin spans
+// several wavefronts, B and C always:
 //   A  every thread forms its new pair term against the proposal (DPP wave
 //      sum -> LDS); the rows of spin hp form the orbital row of the proposal;
 //   B  spin hp's rows form s_k = sum_m phi_m invT[k][m]; the mover publishes
@@ -33,10 +34,11 @@ struct RRWShared {
     Real pph[6][N];      // axis phasors of the proposals
     Real tab[kRRChunk * 3 * (kRRMaxPow + 1) * 2];
     Real phi[NMAX];      // orbital row of the pending move
-    Real row[NMAX];      // inverse row of the mover
+    Real row[2][NMAX];   // inverse row of the mover (by move parity)
     double red[NWAVE];   // per-wave partial sums
     double red4[NWAVE][4];
-    double mv[8];        // mover: R, U, x, y, z
+    double jred[2][NWAVE];  // per-wave new Jastrow pair sums (by move parity)
+    double mv[2][8];     // mover: R, U, x, y, z (by move parity)
     double stat[kStatCount];
     double misc[4];      // sigma
 };
@@ -52,6 +54,7 @@ struct RRW {
     static_assert(ND == 0 || ND == NU, "polarised or equal spins");
     static_assert(NMAX <= WPS * kWave, "rows per spin exceed the spin's wavefronts");
     static constexpr int kMaxC = rr::max_component<NMAX>();
+    static constexpr bool kOneBarrier = WPS == 1 && NMAX >= 57;
     static_assert(kMaxC <= 3, "orbital phasors use powers up to 3");
 
     // e^{2 pi i n.u} from the axis phasors a and their powers.
@@ -175,11 +178,25 @@ struct RRW {
             const Real a = (T.h == hp) ? c.a_same : c.a_anti;
             un = a * rn * wrcp<Real>((Real)1 + c.b * rn);
         }
+        const int par = p & 1;
         {
             const Real ws = rr::wave_total(un);
-            if (T.lane == 0) sh.red[T.v] = (double)ws;
+            if (T.lane == 0) sh.jred[par][T.v] = (double)ws;
         }
-        __syncthreads();
+        // With one wavefront per spin the orbital row is written and read by
+        // the mover's wavefront alone, in issue order: phases A and B need no
+        // workgroup barrier between them (used from 57 per spin; at 33 the
+        // pair sums read after the one barrier cost more than the barrier).  The mover's values and the pair
+        // sums that phase C reads alternate between two buffers by move
+        // parity, so a wavefront already in the next move's phase A or B
+        // cannot overwrite what a slower one still reads.
+        if constexpr (kOneBarrier) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            __syncthreads();
+        }
         // Phase B.
         Real sk = (Real)0;
         if (T.h == hp) {
@@ -191,23 +208,31 @@ struct RRW {
             }
             sk = s0 + s1;
             if (T.r == rp) {
-                sh.mv[0] = (double)sk;
-                sh.mv[1] = U;
-                sh.mv[2] = x;
-                sh.mv[3] = y;
-                sh.mv[4] = z;
+                sh.mv[par][0] = (double)sk;
+                sh.mv[par][1] = U;
+                sh.mv[par][2] = x;
+                sh.mv[par][3] = y;
+                sh.mv[par][4] = z;
 #pragma unroll
-                for (int m = 0; m < NMAX; ++m) sh.row[m] = inv[m];
+                for (int m = 0; m < NMAX; ++m) sh.row[par][m] = inv[m];
             }
         }
+        // The pair sums: complete since the first barrier when there is one
+        // (read before the second, off phase C's path), else after this one.
         double jn = 0.0;
+        if constexpr (!kOneBarrier) {
 #pragma unroll
-        for (int i = 0; i < NWAVE; ++i) jn += sh.red[i];
+            for (int i = 0; i < NWAVE; ++i) jn += sh.jred[par][i];
+        }
         __syncthreads();
+        if constexpr (kOneBarrier) {
+#pragma unroll
+            for (int i = 0; i < NWAVE; ++i) jn += sh.jred[par][i];
+        }
         // Phase C: the same decision in every thread (simulation.cpp:125-131),
         // exact FP64 exponential within 1e-5 of the threshold.
-        const double R = sh.mv[0];
-        const double dj = jn - sh.mv[1];
+        const double R = sh.mv[par][0];
+        const double dj = jn - sh.mv[par][1];
         const double ua = sh.prop[3][p];
         const double r2 = R * R;
         const float e32 = __builtin_amdgcn_exp2f((float)(2.0 * dj * 1.4426950408889634));
@@ -222,12 +247,12 @@ struct RRW {
             Real f = (T.r == rp) ? (Real)1 - inv_ratio : sk * inv_ratio;
             f = T.has_e ? f : (Real)0;
 #pragma unroll
-            for (int m = 0; m < NMAX; ++m) inv[m] -= sh.row[m] * f;
+            for (int m = 0; m < NMAX; ++m) inv[m] -= sh.row[par][m] * f;
         }
         if (T.has_e && T.e != p) {
-            const Real dx = c.mi_r(sh.mv[2], x);
-            const Real dy = c.mi_r(sh.mv[3], y);
-            const Real dz = c.mi_r(sh.mv[4], z);
+            const Real dx = c.mi_r(sh.mv[par][2], x);
+            const Real dy = c.mi_r(sh.mv[par][3], y);
+            const Real dz = c.mi_r(sh.mv[par][4], z);
             const Real ro = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
             const Real a = (T.h == hp) ? c.a_same : c.a_anti;
             U += (double)un - (double)(a * ro * wrcp<Real>((Real)1 + c.b * ro));
@@ -568,9 +593,12 @@ __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState
 // Waves per SIMD the sweep kernel is compiled for.  57 + 57 and 81 + 81 in
 // FP32 take four (at most 128 VGPRs, 16 and 32 B of spill per lane): eight
 // and four walkers per CU, so 2048 and 1024 walkers are resident at once.
+// 123 + 123 takes three (168 VGPRs, 32 B of spill); 147 + 147 stays at two
+// (three would spill 144 B of its inverse row per lane and run at half speed).
 template <typename Real, int NU, int ND, int WPS>
 constexpr int rrw_sweep_waves() {
-    return (sizeof(Real) == 4 && ((NU == 57 && ND == 57) || (NU == 81 && ND == 81))) ? 4 : 1;
+    return (sizeof(Real) == 4 && ((NU == 57 && ND == 57) || (NU == 81 && ND == 81))) ? 4
+           : (sizeof(Real) == 4 && NU == 123 && ND == 123) ? 3 : 1;
 }
 
 template <typename Real, int NU, int ND, int WPS>
