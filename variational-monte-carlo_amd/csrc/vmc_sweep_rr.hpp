@@ -213,9 +213,9 @@ constexpr int kRRMaxPow = 7;
 template <typename Real, int NU, int ND>
 struct RRShared {
     double pos[3][kWave];          // positions by lane (pair loops of the measurement)
-    double prop[4][kWave];         // this sweep's proposal x, y, z and accept uniform
+    alignas(16) double prop[4][kWave];  // this sweep's proposal x, y, z and accept uniform
     Real pph[6][kWave];            // axis phasors of the proposal (re, im per axis)
-    Real tab[kRRChunk * 3 * (kRRMaxPow + 1) * 2];  // axis powers of a chunk of electrons
+    Real tab[kRRChunk * 3 * (kRRMaxPow + 1) * 2];  // with prop and pph: staging (below)
     double stat[kStatCount];
     double misc[4];                // sigma, log|det| up, log|det| down
     Real phi[32];                  // orbital row of the pending move
@@ -224,8 +224,170 @@ struct RRShared {
     int perm[32];
 };
 
+// Structure factor S(G) = sum_j e^{i G.r_j} (energy_tracking.cpp:72-91) by
+// threads that own G vectors (thread t owns G t + NT k).  Electrons go
+// through in chunks of kRRChunk: the thread of each electron stages its axis
+// phasor powers X^0..X^7, Y^-7..Y^7 and Z^-7..Z^7 in LDS as [entry][electron]
+// (complex, padded pitch), and every thread adds X^mx Y^my Z^mz of its G
+// vectors: three conflict-free LDS reads whose electron index is the reads'
+// immediate offset, one complex product and four fused accumulations.
+namespace sf {
+
+constexpr int kEntries = (kRRMaxPow + 1) + 2 * (2 * kRRMaxPow + 1);
+constexpr int kY0 = kRRMaxPow + 1 + kRRMaxPow;      // entry of Y^0
+constexpr int kZ0 = kY0 + 2 * kRRMaxPow + 1;        // entry of Z^0
+// Complex values per entry: one per electron of the chunk and a pad that
+// spreads the entries of one FP32 read over distinct LDS banks.
+constexpr int kPitch = kRRChunk + 1;
+
+// A table value, aligned so that it moves with one LDS read.
+template <typename Real>
+struct alignas(2 * sizeof(Real)) Pair {
+    Real re, im;
+};
+
+template <typename Real>
+constexpr int table_reals() {
+    return 2 * kEntries * kPitch;
+}
+
+// Axis powers of the electron in slot i of the chunk.
+template <typename Real>
+__device__ __forceinline__ void stage(Pair<Real> *tab, int i, double x, double y, double z, double inv_L) {
+    // Computed here, in the chunk of the electron: hoisted out of the chunk
+    // loop, the 38 values would stay live through the whole loop.
+    asm volatile("" : "+v"(x), "+v"(y), "+v"(z));
+    Pair<Real> *t = tab + i;
+    rr::Cx<Real> a = rr::axis_phasor<Real>(x, inv_L), p{(Real)1, (Real)0};
+#pragma unroll
+    for (int m = 0; m <= kRRMaxPow; ++m) {
+        t[m * kPitch] = {p.re, p.im};
+        p = rr::cmul(p, a);
+    }
+    const double u[2] = {y, z};
+    const int c[2] = {kY0, kZ0};
+#pragma unroll
+    for (int ax = 0; ax < 2; ++ax) {
+        a = rr::axis_phasor<Real>(u[ax], inv_L);
+        p = {(Real)1, (Real)0};
+#pragma unroll
+        for (int m = 0; m <= kRRMaxPow; ++m) {
+            t[(c[ax] + m) * kPitch] = {p.re, p.im};
+            t[(c[ax] - m) * kPitch] = {p.re, -p.im};
+            p = rr::cmul(p, a);
+        }
+    }
+}
+
+// term_k += X Y Z of the staged electron at pointer offset i of the entries.
+template <typename Real, int GPT>
+__device__ __forceinline__ void add_one(const Pair<Real> *tab, int i, int k, const int (&ox)[GPT],
+                                        const int (&oy)[GPT], const int (&oz)[GPT], Real (&re)[GPT],
+                                        Real (&im)[GPT]) {
+    const Pair<Real> X = tab[ox[k] + i], Y = tab[oy[k] + i], Z = tab[oz[k] + i];
+    const Real xr = X.re * Y.re - X.im * Y.im, xi = X.re * Y.im + X.im * Y.re;
+    re[k] = fma(xr, Z.re, re[k]);
+    re[k] = fma(-xi, Z.im, re[k]);
+    im[k] = fma(xr, Z.im, im[k]);
+    im[k] = fma(xi, Z.re, im[k]);
+}
+
+// Adds the kRRChunk staged electrons to the sums of the thread's G vectors
+// (ox, oy, oz: table offsets of their entries), two electrons at a time (the
+// reads in flight stay within the register budget).  Up to 5 G vectors per
+// thread the electron index is the reads' immediate offset; above, the
+// electron pairs are a loop, taken once per group of 5 G vectors.
+template <typename Real, int GPT>
+__device__ __forceinline__ void add(const Pair<Real> *tab, const int (&ox)[GPT], const int (&oy)[GPT],
+                                    const int (&oz)[GPT], Real (&re)[GPT], Real (&im)[GPT]) {
+    constexpr int kG = 5;
+    if constexpr (GPT <= kG) {
+#pragma unroll
+        for (int i = 0; i < kRRChunk; ++i) {
+#pragma unroll
+            for (int k = 0; k < GPT; ++k) add_one(tab, i, k, ox, oy, oz, re, im);
+            if (i % 2 == 1) {
+                asm volatile("" ::: "memory");
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k0 = 0; k0 < GPT; k0 += kG) {
+#pragma unroll 1
+            for (int i = 0; i < kRRChunk; i += 2) {
+#pragma unroll
+                for (int k = k0; k < (k0 + kG < GPT ? k0 + kG : GPT); ++k) {
+                    add_one(tab, i, k, ox, oy, oz, re, im);
+                    add_one(tab, i + 1, k, ox, oy, oz, re, im);
+                }
+            }
+        }
+    }
+}
+
+// sum over the thread's G vectors of w_G |S_G|^2 (to be reduced over the
+// threads).  e: the thread's electron (-1 if none) at (x, y, z).  tab: LDS of
+// table_reals<Real>(); every thread of the workgroup takes part.
+template <typename Real, int N, int GPT, int NT>
+__device__ double energy_g(const DevSys &s, Pair<Real> *tab, int tid, int e, double x, double y, double z) {
+    constexpr int P = kPitch;
+    int ox[GPT], oy[GPT], oz[GPT];
+    Real re[GPT], im[GPT];
+    // The G table reads stay here (not hoisted to the kernel's start, where
+    // their values would be live through the rest of the measurement).
+    asm volatile("" : "+v"(tid));
+#pragma unroll
+    for (int k = 0; k < GPT; ++k) {
+        const int g = tid + NT * k;
+        // Absent G vectors read the entries of G = 0 and get no weight.
+        const int4 G = g < s.ng ? s.gvec[g] : make_int4(0, 0, 0, 0);
+        ox[k] = G.x * P;
+        oy[k] = (kY0 + G.y) * P;
+        oz[k] = (kZ0 + G.z) * P;
+        re[k] = (Real)0;
+        im[k] = (Real)0;
+    }
+    // The last chunk's empty slots hold zeros, so every chunk is a full one.
+    constexpr int kChunks = (N + kRRChunk - 1) / kRRChunk, kTail = N % kRRChunk;
+#pragma unroll 1
+    for (int c = 0; c < kChunks; ++c) {
+        __syncthreads();
+        const int i = e - c * kRRChunk;
+        if (e >= 0 && i >= 0 && i < kRRChunk) stage(tab, i, x, y, z, s.inv_L);
+        if (kTail > 0 && c == kChunks - 1 && tid >= kTail && tid < kRRChunk) {
+#pragma unroll
+            for (int m = 0; m < kEntries; ++m) tab[m * kPitch + tid] = {(Real)0, (Real)0};
+        }
+        __syncthreads();
+        add<Real, GPT>(tab, ox, oy, oz, re, im);
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < GPT; ++k) {
+        const int g = tid + NT * k;
+        if (g < s.ng) acc += s.gw[g] * ((double)re[k] * (double)re[k] + (double)im[k] * (double)im[k]);
+    }
+    __syncthreads();
+    return acc;
+}
+
+// As energy_g with GPT slots per thread, and with fewer when the G set of the
+// Ewald sum with alpha = 5/L (at most 5 per wavefront lane) fits them.
+template <typename Real, int N, int GPT, int NT>
+__device__ double energy(const DevSys &s, Pair<Real> *tab, int tid, int e, double x, double y, double z) {
+    constexpr int kSmall = (5 * kWave + NT - 1) / NT;
+    if constexpr (kSmall < GPT) {
+        if (s.ng <= kSmall * NT) return energy_g<Real, N, kSmall, NT>(s, tab, tid, e, x, y, z);
+    }
+    return energy_g<Real, N, GPT, NT>(s, tab, tid, e, x, y, z);
+}
+
+}  // namespace sf
+
 // The LDS run prop | pph | tab of RRShared doubles as the stage through which
-// inverse blocks move between HBM and the lanes' registers.
+// inverse blocks move between HBM and the lanes' registers, and holds the
+// structure-factor tables of the measurement.
 template <typename Real, int NU, int ND>
 __device__ __forceinline__ Real *rr_stage(RRShared<Real, NU, ND> &sh) {
     using Sh = RRShared<Real, NU, ND>;
@@ -571,6 +733,9 @@ struct RRKernel {
         Measured m;
         m.kinetic = -0.5 * rr::wave_total(kin_lane);
         m.vreal = rr::wave_total(has_e ? vreal : 0.0);
+        // Both are complete here: their inputs (the inverse rows among them)
+        // are not carried through the structure-factor loop.
+        asm volatile("" : "+v"(m.kinetic), "+v"(m.vreal));
         m.vrecip = s.pref * recip(s, sh, L, lane, has_e);
         m.e_local = m.kinetic + m.vreal + m.vrecip + s.e_const;
         return m;
@@ -580,64 +745,11 @@ struct RRKernel {
     // staged in LDS (energy_tracking.cpp:72-91,146-191).
     __device__ static double recip(const DevSys &s, Sh &sh, const RRLane<Real, NMAX> &L, int lane,
                                    bool has_e) {
-        constexpr int P = kRRMaxPow + 1;
-        int gk[GPL];  // packed (mx, my + 8, mz + 8), -1 when absent
-        Real sre[GPL], sim[GPL];
-#pragma unroll
-        for (int k = 0; k < GPL; ++k) {
-            const int g = lane + kWave * k;
-            const int4 G = g < s.ng ? s.gvec[g] : make_int4(0, 0, 0, 0);
-            gk[k] = g < s.ng ? (G.x | ((G.y + 8) << 4) | ((G.z + 8) << 8)) : -1;
-            sre[k] = (Real)0;
-            sim[k] = (Real)0;
-        }
+        static_assert(sizeof(sh.prop) + sizeof(sh.pph) + sizeof(sh.tab) >= sizeof(Real) * sf::table_reals<Real>(),
+                      "structure-factor tables exceed the stage run");
         const int lane_e = has_e ? ((lane < 32) ? lane : NU + (lane - 32)) : -1;
-        for (int c0 = 0; c0 < N; c0 += kRRChunk) {
-            const int cn = N - c0 < kRRChunk ? N - c0 : kRRChunk;
-            __syncthreads();
-            if (lane_e >= c0 && lane_e < c0 + cn) {
-                const int i = lane_e - c0;
-                const double u[3] = {L.x, L.y, L.z};
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) {
-                    const rr::Cx<Real> a = rr::axis_phasor<Real>(u[ax], s.inv_L);
-                    rr::Cx<Real> p{(Real)1, (Real)0};
-                    Real *t = sh.tab + ((i * 3 + ax) * P) * 2;
-#pragma unroll
-                    for (int m = 0; m < P; ++m) {
-                        t[2 * m] = p.re;
-                        t[2 * m + 1] = p.im;
-                        p = rr::cmul(p, a);
-                    }
-                }
-            }
-            __syncthreads();
-            for (int i = 0; i < cn; ++i) {
-                const Real *t = sh.tab + (i * 3 * P) * 2;
-#pragma unroll
-                for (int k = 0; k < GPL; ++k) {
-                    if (gk[k] < 0) continue;
-                    const int mx = gk[k] & 15, my = ((gk[k] >> 4) & 15) - 8, mz = ((gk[k] >> 8) & 15) - 8;
-                    const int ay = my < 0 ? -my : my, az = mz < 0 ? -mz : mz;
-                    const rr::Cx<Real> X{t[2 * mx], t[2 * mx + 1]};
-                    rr::Cx<Real> Y{t[2 * (P + ay)], t[2 * (P + ay) + 1]};
-                    rr::Cx<Real> Z{t[2 * (2 * P + az)], t[2 * (2 * P + az) + 1]};
-                    if (my < 0) Y.im = -Y.im;
-                    if (mz < 0) Z.im = -Z.im;
-                    const rr::Cx<Real> e = rr::cmul(rr::cmul(X, Y), Z);
-                    sre[k] += e.re;
-                    sim[k] += e.im;
-                }
-            }
-        }
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < GPL; ++k) {
-            const int g = lane + kWave * k;
-            if (g < s.ng) acc += s.gw[g] * ((double)sre[k] * (double)sre[k] + (double)sim[k] * (double)sim[k]);
-        }
-        __syncthreads();
-        return rr::wave_total(acc);
+        auto *tab = reinterpret_cast<sf::Pair<Real> *>(&sh.prop[0][0]);
+        return rr::wave_total(sf::energy<Real, N, GPL, kWave>(s, tab, lane, lane_e, L.x, L.y, L.z));
     }
 
     // One move of electron (HP, rp).  Returns 1 if accepted.
@@ -1183,8 +1295,18 @@ __global__ __launch_bounds__(kWave) void k_reinv_rr(DevSys s, DevState st) {
     reinv_rr_body<Real, NU, ND, GPL>(s, st, blockIdx.x);
 }
 
+// Waves per SIMD the measurement is built for: the structure-factor loop
+// would otherwise hold many staged values in flight at low occupancy.
+// FP64 measurements take the registers they need.
+template <typename Real, int N, int GPL>
+constexpr int rr_measure_waves() {
+    if constexpr (sizeof(Real) == 8) return 1;
+    return GPL <= 5 ? 5 : (N <= kRRChunk ? 1 : 4);
+}
+
 template <typename Real, int NU, int ND, int GPL>
-__global__ __launch_bounds__(kWave) void k_measure_rr(DevSys s, DevState st, int block_size) {
+__global__ __launch_bounds__(kWave, (rr_measure_waves<Real, NU + ND, GPL>())) void k_measure_rr(DevSys s, DevState st,
+                                                                               int block_size) {
     measure_rr_body<Real, NU, ND, GPL>(s, st, blockIdx.x, block_size);
 }
 
@@ -1210,8 +1332,8 @@ __global__ __launch_bounds__(kWave) void k_reinv_rr_b(const BatchItem *__restric
 }
 
 template <typename Real, int NU, int ND, int GPL>
-__global__ __launch_bounds__(kWave) void k_measure_rr_b(const BatchItem *__restrict__ items, int n_items,
-                                                        int block_size) {
+__global__ __launch_bounds__(kWave, (rr_measure_waves<Real, NU + ND, GPL>())) void k_measure_rr_b(
+    const BatchItem *__restrict__ items, int n_items, int block_size) {
     const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
     measure_rr_body<Real, NU, ND, GPL>(it.s, it.st, blockIdx.x - it.walker_begin, block_size);
 }

@@ -30,7 +30,7 @@ struct RRWShared {
     static constexpr int NMAX = NU > ND ? NU : ND;
     static constexpr int NWAVE = (ND > 0 ? 2 : 1) * WPS;
     double pos[3][N];
-    double prop[4][N];   // this sweep's proposals (x, y, z) and accept uniforms
+    alignas(16) double prop[4][N];  // this sweep's proposals (x, y, z) and accept uniforms
     Real pph[6][N];      // axis phasors of the proposals
     Real tab[kRRChunk * 3 * (kRRMaxPow + 1) * 2];
     Real phi[NMAX];      // orbital row of the pending move
@@ -506,63 +506,15 @@ __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState
     const double kin_t = T.has_e ? (slap + lap + tx * tx + ty * ty + tz * tz) : 0.0;
     const double kinetic = -0.5 * K::block_total(kin_t, sh.red);
     const double vr = K::block_total(T.has_e ? vreal : 0.0, sh.red);
-    // Structure factor S(G) = sum_j e^{i G.r_j}: threads own G vectors.
-    constexpr int P = kRRMaxPow + 1;
-    int gk[GPT];
-    Real sre[GPT], sim[GPT];
-#pragma unroll
-    for (int k = 0; k < GPT; ++k) {
-        const int g = (int)threadIdx.x + NT * k;
-        const int4 G = g < s.ng ? s.gvec[g] : make_int4(0, 0, 0, 0);
-        gk[k] = g < s.ng ? (G.x | ((G.y + 8) << 4) | ((G.z + 8) << 8)) : -1;
-        sre[k] = 0;
-        sim[k] = 0;
-    }
-    for (int c0 = 0; c0 < N; c0 += kRRChunk) {
-        const int cn = N - c0 < kRRChunk ? N - c0 : kRRChunk;
-        __syncthreads();
-        if (T.has_e && T.e >= c0 && T.e < c0 + cn) {
-            const int i = T.e - c0;
-            const double u[3] = {x, y, z};
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                const rr::Cx<Real> a = rr::axis_phasor<Real>(u[ax], s.inv_L);
-                rr::Cx<Real> q{(Real)1, (Real)0};
-                Real *t = sh.tab + ((i * 3 + ax) * P) * 2;
-#pragma unroll
-                for (int m = 0; m < P; ++m) {
-                    t[2 * m] = q.re;
-                    t[2 * m + 1] = q.im;
-                    q = rr::cmul(q, a);
-                }
-            }
-        }
-        __syncthreads();
-        for (int i = 0; i < cn; ++i) {
-            const Real *t = sh.tab + (i * 3 * P) * 2;
-#pragma unroll
-            for (int k = 0; k < GPT; ++k) {
-                if (gk[k] < 0) continue;
-                const int mx = gk[k] & 15, my = ((gk[k] >> 4) & 15) - 8, mz = ((gk[k] >> 8) & 15) - 8;
-                const int ay = my < 0 ? -my : my, az = mz < 0 ? -mz : mz;
-                const rr::Cx<Real> X{t[2 * mx], t[2 * mx + 1]};
-                rr::Cx<Real> Y{t[2 * (P + ay)], t[2 * (P + ay) + 1]};
-                rr::Cx<Real> Z{t[2 * (2 * P + az)], t[2 * (2 * P + az) + 1]};
-                if (my < 0) Y.im = -Y.im;
-                if (mz < 0) Z.im = -Z.im;
-                const rr::Cx<Real> ez = rr::cmul(rr::cmul(X, Y), Z);
-                sre[k] += ez.re;
-                sim[k] += ez.im;
-            }
-        }
-    }
-    double acc = 0.0;
-#pragma unroll
-    for (int k = 0; k < GPT; ++k) {
-        const int g = (int)threadIdx.x + NT * k;
-        if (g < s.ng) acc += s.gw[g] * ((double)sre[k] * (double)sre[k] + (double)sim[k] * (double)sim[k]);
-    }
-    __syncthreads();
+    // Structure factor S(G) = sum_j e^{i G.r_j}: threads own G vectors; the
+    // tables are staged in the run prop | pph | tab.
+    using Sh = typename K::Sh;
+    static_assert(offsetof(Sh, pph) == offsetof(Sh, prop) + sizeof(sh.prop) &&
+                      offsetof(Sh, tab) == offsetof(Sh, pph) + sizeof(sh.pph) &&
+                      sizeof(sh.prop) + sizeof(sh.pph) + sizeof(sh.tab) >= sizeof(Real) * sf::table_reals<Real>(),
+                  "structure-factor tables exceed the run prop | pph | tab");
+    const double acc = sf::energy<Real, N, GPT, NT>(s, reinterpret_cast<sf::Pair<Real> *>(&sh.prop[0][0]), (int)threadIdx.x,
+                                                    T.has_e ? T.e : -1, x, y, z);
     const double vrecip = s.pref * K::block_total(acc, sh.red);
     if (threadIdx.x == 0) {
         double *t = st.stats + (size_t)w * kStatCount;
@@ -610,6 +562,7 @@ __global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64, (rrw_sweep_waves<Real,
 
 template <typename Real, int NU, int ND, int WPS>
 constexpr int rrw_measure_waves() {
+    if constexpr (sizeof(Real) == 4 && ((NU == 33 && ND == 33) || (NU == 147 && ND == 147))) return 5;
     return (sizeof(Real) == 4 && ((NU == 81 && ND == 81) || (NU == 123 && ND == 123))) ? 4 : 1;
 }
 
