@@ -226,11 +226,12 @@ struct RRShared {
 
 // Structure factor S(G) = sum_j e^{i G.r_j} (energy_tracking.cpp:72-91) by
 // threads that own G vectors (thread t owns G t + NT k).  Electrons go
-// through in chunks of kRRChunk: the thread of each electron stages its axis
-// phasor powers X^0..X^7, Y^-7..Y^7 and Z^-7..Z^7 in LDS as [entry][electron]
-// (complex, padded pitch), and every thread adds X^mx Y^my Z^mz of its G
-// vectors: three conflict-free LDS reads whose electron index is the reads'
-// immediate offset, one complex product and four fused accumulations.
+// through in chunks of kRRChunk: 3 kRRChunk threads stage the chunk's axis
+// phasor powers X^0..X^7, Y^-7..Y^7 and Z^-7..Z^7 (one axis of one electron
+// each) in LDS as [entry][electron] (complex, padded pitch), and every thread
+// adds X^mx Y^my Z^mz of its G vectors: three conflict-free LDS reads whose
+// electron index is the reads' immediate offset, one complex product and four
+// fused accumulations.
 namespace sf {
 
 constexpr int kEntries = (kRRMaxPow + 1) + 2 * (2 * kRRMaxPow + 1);
@@ -251,31 +252,24 @@ constexpr int table_reals() {
     return 2 * kEntries * kPitch;
 }
 
-// Axis powers of the electron in slot i of the chunk.
+// Powers of one axis phasor (coordinate u) of the electron in slot i of the
+// chunk (entries centre + m and, for Y and Z, centre - m), or zeros for an
+// empty slot.
 template <typename Real>
-__device__ __forceinline__ void stage(Pair<Real> *tab, int i, double x, double y, double z, double inv_L) {
-    // Computed here, in the chunk of the electron: hoisted out of the chunk
-    // loop, the 38 values would stay live through the whole loop.
-    asm volatile("" : "+v"(x), "+v"(y), "+v"(z));
+__device__ __forceinline__ void stage_axis(Pair<Real> *tab, int i, int ax, bool present, double u,
+                                           double inv_L) {
+    const int centre = ax == 0 ? 0 : (ax == 1 ? kY0 : kZ0);
+    rr::Cx<Real> a{(Real)0, (Real)0}, p{(Real)0, (Real)0};
+    if (present) {
+        a = rr::axis_phasor<Real>(u, inv_L);
+        p = {(Real)1, (Real)0};
+    }
     Pair<Real> *t = tab + i;
-    rr::Cx<Real> a = rr::axis_phasor<Real>(x, inv_L), p{(Real)1, (Real)0};
 #pragma unroll
     for (int m = 0; m <= kRRMaxPow; ++m) {
-        t[m * kPitch] = {p.re, p.im};
+        t[(centre + m) * kPitch] = {p.re, p.im};
+        if (ax != 0) t[(centre - m) * kPitch] = {p.re, -p.im};
         p = rr::cmul(p, a);
-    }
-    const double u[2] = {y, z};
-    const int c[2] = {kY0, kZ0};
-#pragma unroll
-    for (int ax = 0; ax < 2; ++ax) {
-        a = rr::axis_phasor<Real>(u[ax], inv_L);
-        p = {(Real)1, (Real)0};
-#pragma unroll
-        for (int m = 0; m <= kRRMaxPow; ++m) {
-            t[(c[ax] + m) * kPitch] = {p.re, p.im};
-            t[(c[ax] - m) * kPitch] = {p.re, -p.im};
-            p = rr::cmul(p, a);
-        }
     }
 }
 
@@ -327,10 +321,10 @@ __device__ __forceinline__ void add(const Pair<Real> *tab, const int (&ox)[GPT],
 }
 
 // sum over the thread's G vectors of w_G |S_G|^2 (to be reduced over the
-// threads).  e: the thread's electron (-1 if none) at (x, y, z).  tab: LDS of
+// threads).  pos_of(ax, e): coordinate ax of electron e (LDS).  tab: LDS of
 // table_reals<Real>(); every thread of the workgroup takes part.
-template <typename Real, int N, int GPT, int NT>
-__device__ double energy_g(const DevSys &s, Pair<Real> *tab, int tid, int e, double x, double y, double z) {
+template <typename Real, int N, int GPT, int NT, typename PosOf>
+__device__ double energy_g(const DevSys &s, Pair<Real> *tab, int tid, PosOf pos_of) {
     constexpr int P = kPitch;
     int ox[GPT], oy[GPT], oz[GPT];
     Real re[GPT], im[GPT];
@@ -349,15 +343,14 @@ __device__ double energy_g(const DevSys &s, Pair<Real> *tab, int tid, int e, dou
         im[k] = (Real)0;
     }
     // The last chunk's empty slots hold zeros, so every chunk is a full one.
-    constexpr int kChunks = (N + kRRChunk - 1) / kRRChunk, kTail = N % kRRChunk;
+    constexpr int kChunks = (N + kRRChunk - 1) / kRRChunk;
 #pragma unroll 1
     for (int c = 0; c < kChunks; ++c) {
         __syncthreads();
-        const int i = e - c * kRRChunk;
-        if (e >= 0 && i >= 0 && i < kRRChunk) stage(tab, i, x, y, z, s.inv_L);
-        if (kTail > 0 && c == kChunks - 1 && tid >= kTail && tid < kRRChunk) {
-#pragma unroll
-            for (int m = 0; m < kEntries; ++m) tab[m * kPitch + tid] = {(Real)0, (Real)0};
+        // Thread 3 i + ax stages axis ax of slot i.
+        if (tid < 3 * kRRChunk) {
+            const int i = tid / 3, ax = tid - 3 * i, e = c * kRRChunk + i;
+            stage_axis(tab, i, ax, e < N, e < N ? pos_of(ax, e) : 0.0, s.inv_L);
         }
         __syncthreads();
         add<Real, GPT>(tab, ox, oy, oz, re, im);
@@ -374,13 +367,13 @@ __device__ double energy_g(const DevSys &s, Pair<Real> *tab, int tid, int e, dou
 
 // As energy_g with GPT slots per thread, and with fewer when the G set of the
 // Ewald sum with alpha = 5/L (at most 5 per wavefront lane) fits them.
-template <typename Real, int N, int GPT, int NT>
-__device__ double energy(const DevSys &s, Pair<Real> *tab, int tid, int e, double x, double y, double z) {
+template <typename Real, int N, int GPT, int NT, typename PosOf>
+__device__ double energy(const DevSys &s, Pair<Real> *tab, int tid, PosOf pos_of) {
     constexpr int kSmall = (5 * kWave + NT - 1) / NT;
     if constexpr (kSmall < GPT) {
-        if (s.ng <= kSmall * NT) return energy_g<Real, N, kSmall, NT>(s, tab, tid, e, x, y, z);
+        if (s.ng <= kSmall * NT) return energy_g<Real, N, kSmall, NT>(s, tab, tid, pos_of);
     }
-    return energy_g<Real, N, GPT, NT>(s, tab, tid, e, x, y, z);
+    return energy_g<Real, N, GPT, NT>(s, tab, tid, pos_of);
 }
 
 }  // namespace sf
@@ -747,9 +740,10 @@ struct RRKernel {
                                    bool has_e) {
         static_assert(sizeof(sh.prop) + sizeof(sh.pph) + sizeof(sh.tab) >= sizeof(Real) * sf::table_reals<Real>(),
                       "structure-factor tables exceed the stage run");
-        const int lane_e = has_e ? ((lane < 32) ? lane : NU + (lane - 32)) : -1;
         auto *tab = reinterpret_cast<sf::Pair<Real> *>(&sh.prop[0][0]);
-        return rr::wave_total(sf::energy<Real, N, GPL, kWave>(s, tab, lane, lane_e, L.x, L.y, L.z));
+        // Electron e sits in lane e (up) or 32 + e - NU (down).
+        const auto pos_of = [&sh](int ax, int e) { return sh.pos[ax][e < NU ? e : 32 + e - NU]; };
+        return rr::wave_total(sf::energy<Real, N, GPL, kWave>(s, tab, lane, pos_of));
     }
 
     // One move of electron (HP, rp).  Returns 1 if accepted.

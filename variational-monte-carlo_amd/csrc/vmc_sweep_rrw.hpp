@@ -513,8 +513,9 @@ __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState
                       offsetof(Sh, tab) == offsetof(Sh, pph) + sizeof(sh.pph) &&
                       sizeof(sh.prop) + sizeof(sh.pph) + sizeof(sh.tab) >= sizeof(Real) * sf::table_reals<Real>(),
                   "structure-factor tables exceed the run prop | pph | tab");
-    const double acc = sf::energy<Real, N, GPT, NT>(s, reinterpret_cast<sf::Pair<Real> *>(&sh.prop[0][0]), (int)threadIdx.x,
-                                                    T.has_e ? T.e : -1, x, y, z);
+    const auto pos_of = [](int ax, int e) { return sh.pos[ax][e]; };
+    const double acc = sf::energy<Real, N, GPT, NT>(s, reinterpret_cast<sf::Pair<Real> *>(&sh.prop[0][0]),
+                                                    (int)threadIdx.x, pos_of);
     const double vrecip = s.pref * K::block_total(acc, sh.red);
     if (threadIdx.x == 0) {
         double *t = st.stats + (size_t)w * kStatCount;
