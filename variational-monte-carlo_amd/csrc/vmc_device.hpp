@@ -308,9 +308,42 @@ __device__ inline Real erfc_as7126(Real x) {
     return tau * wexp<Real>(-x * x);
 }
 
+// FP32 erfc for 0 <= x <= kErfcFitMax (alpha r inside the cutoff radius L/2
+// with alpha = 5/L): erfc(x) = exp(-x^2) P(t), t = (x - 1.5) / (x + 1.5),
+// where P is a degree-8 least-squares Chebyshev fit of erfcx(x) = exp(x^2)
+// erfc(x) (coefficients and their FP32 error from fit_erfc.py: P within
+// 2.0e-7 relative).  exp(-x^2) keeps the rounding of x^2 and of x^2 log2(e)
+// as a first-order correction, so the exponential factor adds about one
+// rounding of v_exp_f32.
+constexpr float kErfcFitMax = 2.5f;
+
+__device__ inline float erfc_fit(float x) {
+    const float t = (x - 1.5f) * __builtin_amdgcn_rcpf(x + 1.5f);
+    float p = 1.670535858e-04f;
+    p = fmaf(p, t, 1.198613783e-03f);
+    p = fmaf(p, t, 2.984628081e-03f);
+    p = fmaf(p, t, -4.575148632e-04f);
+    p = fmaf(p, t, -1.864113472e-02f);
+    p = fmaf(p, t, -9.285487235e-03f);
+    p = fmaf(p, t, 1.944909096e-01f);
+    p = fmaf(p, t, -4.908687174e-01f);
+    p = fmaf(p, t, 3.215854168e-01f);
+    constexpr float kLog2e = 1.44269502e+00f, kLog2eLo = 1.92596303e-08f, kLn2 = 6.93147182e-01f;
+    const float h = x * x, l = fmaf(x, x, -h);       // x^2 = h + l
+    const float y = -h * kLog2e;                      // exponent of 2, rounded
+    float ye = fmaf(-h, kLog2e, -y);                  // its rounding
+    ye = fmaf(-h, kLog2eLo, ye);
+    ye = fmaf(-l, kLog2e, ye);
+    const float e = __builtin_amdgcn_exp2f(y);
+    return fmaf(e, ye * kLn2, e) * p;
+}
+
 template <typename Real>
 __device__ inline Real erfc_r(Real x, int kind) {
     if (kind == 0) return erfc_as7126<Real>(x);
+    if constexpr (sizeof(Real) == 4) {
+        if (x >= 0.0f && x <= kErfcFitMax) return erfc_fit(x);
+    }
     return erfc(x);
 }
 
