@@ -812,10 +812,9 @@ struct RRKernel {
         // 367-423) as one FMA per element and lane: rows k != p of spin HP
         // lose (s_k / R) row_p; row p itself (row_p is its own registers)
         // loses (1 - 1/R) of itself, i.e. becomes row_p / R; every other lane
-        // subtracts 0 x (a finite row).  In FP32 the update runs on every move
-        // with f = 0 when rejected -- straight-line code keeps the rows in
-        // their registers; in FP64 (half-rate FMAs) only on accept.
-        if (sizeof(Real) == 8 && !accepted) return 0;
+        // subtracts 0 x (a finite row).  A rejected move skips the update on
+        // a wave-uniform branch.
+        if (!accepted) return 0;
         if (lane == lp) {
 #pragma unroll
             for (int m = 0; m < NP; ++m) sh.row[m] = L.inv[m];
