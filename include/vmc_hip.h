@@ -281,8 +281,9 @@ int vmc_ensemble_set_walker_threads(vmc_ensemble *e, int32_t threads);
 /* Device bytes an ensemble of this system and walker count holds (state and
  * tables), the admission measure of the request server: per walker the
  * positions, inverses, FP64 Gauss-Jordan scratch, statistics, step state,
- * drift and Jastrow row sums, and for mixed ensembles above 80 electrons per
- * spin that re-invert the Newton-Schulz residual.  No reference counterpart. */
+ * drift, Jastrow row sums and the re-inversion list entry, and for mixed
+ * ensembles above 80 electrons per spin that re-invert the Newton-Schulz
+ * residual.  No reference counterpart. */
 int vmc_ensemble_footprint(const vmc_system *sys, int64_t n_walkers, int64_t *bytes);
 
 /* --------------------------------------------------------------- autotuner

@@ -68,8 +68,9 @@ def test_footprint_counts_every_per_walker_buffer(vmc_nodev):
     """vmc_ensemble_footprint (the server's admission measure) counts what an
     ensemble allocates per walker: positions, the working-precision inverses,
     the FP64 Gauss-Jordan scratch, statistics, step state, drift, the Jastrow
-    row sums and -- for mixed ensembles above 80 electrons per spin that
-    re-invert -- the Newton-Schulz residual (padded order^2 floats)."""
+    row sums, the re-inversion list entry and -- for mixed ensembles above 80
+    electrons per spin that re-invert -- the Newton-Schulz residual (padded
+    order^2 floats)."""
     V = vmc_nodev
 
     def per_walker(s):
@@ -77,7 +78,7 @@ def test_footprint_counts_every_per_walker_buffer(vmc_nodev):
 
     N, n = 54, 27
     s = V.System.heg(N, 2.0, 1.0, precision=V.PREC_MIXED, reinvert_interval=64)
-    assert per_walker(s) == 3 * N * 8 + 2 * n * n * 4 + n * n * 8 + 16 * 8 + 8 + 16 + 8 + 8 + 8 * N
+    assert per_walker(s) == 3 * N * 8 + 2 * n * n * 4 + n * n * 8 + 16 * 8 + 8 + 16 + 12 + 8 + 8 * N
     s64 = V.System.heg(N, 2.0, 1.0, precision=V.PREC_FP64)
     assert per_walker(s64) - per_walker(s) == 2 * n * n * 4
     for N, p in ((162, 96), (246, 128), (294, 160)):

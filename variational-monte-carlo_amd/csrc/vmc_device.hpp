@@ -57,7 +57,14 @@ struct DevState {
     double *drift;    // [W] largest |I - D^-1 D| met before a re-inversion
     float *resid;     // [W][P^2] Newton-Schulz residual above 80 per spin (mixed), else null
     double *jrow;     // [W][N] Jastrow row sums U_i kept by the register sweeps between sweeps
+    unsigned *due;    // [2] walkers listed in due_list, finished workgroups of the re-inversion
+    int *due_list;    // [W] walkers a register sweep found due for re-inversion
 };
+
+// Internal sweep flag (beside VMC_SWEEP_*): the register sweep lists the
+// walkers that reach K accepted moves in DevState::due_list for the compacted
+// re-inversion launch that follows it.
+constexpr int kSweepListDue = 1 << 30;
 
 // Walker flag bits (DevState::flags).
 constexpr int kFlagSingular = 1;  // a singular determinant was met

@@ -13,6 +13,8 @@
 //                              above 80 electrons per spin)
 //   drift   double [W]         largest |I - D^-1 D| met at a re-inversion
 //   jrow    double [W][N]      Jastrow row sums kept by the register sweeps
+//   due_list int   [W]         walkers a register sweep found due for
+//                              re-inversion (count in due[0])
 //   resid   float  [W][P^2]    Newton-Schulz residual, mixed above 80 per spin
 // The generic kernel in this file runs one walker per 64-, 128- or 256-thread
 // workgroup; its positions, Jastrow row sums and draws stay in LDS for a whole
@@ -684,6 +686,8 @@ struct vmc_ensemble {
         if (st.drift) (void)hipFree(st.drift);
         if (st.resid) (void)hipFree(st.resid);
         if (st.jrow) (void)hipFree(st.jrow);
+        if (st.due) (void)hipFree(st.due);
+        if (st.due_list) (void)hipFree(st.due_list);
     }
 };
 
@@ -801,12 +805,12 @@ int vmc_ensemble_footprint(const vmc_system *sys, int64_t n_walkers, int64_t *by
     const int64_t nmax = std::max(sys->n_up, sys->n_dn);
     const int64_t rb = sys->precision == VMC_PREC_MIXED ? 4 : 8;
     const int64_t per_walker = 3 * N * 8 + ninv * rb + nmax * nmax * 8 + kStatCount * 8 +
-                               8 /* sigma */ + 16 /* logdet */ + 8 /* since, flags */ + 8 /* drift */ +
+                               8 /* sigma */ + 16 /* logdet */ + 12 /* since, flags, due list */ + 8 /* drift */ +
                                (sys->reinvert_interval > 0
                                     ? 4 * ns_resid_floats(static_cast<int>(nmax), static_cast<int>(rb))
                                     : 0) +
                                8 * N /* Jastrow row sums */;
-    const int64_t tables = 2 * nmax * 16 + static_cast<int64_t>(ng) * (16 + 8);
+    const int64_t tables = 2 * nmax * 16 + static_cast<int64_t>(ng) * (16 + 8) + 8 /* due counters */;
     *bytes = n_walkers * per_walker + tables;
     return VMC_OK;
 }
@@ -835,7 +839,7 @@ int vmc_ensemble_create(const vmc_system *sys, int64_t n_walkers, uint64_t seed,
         (rc = dev_alloc(&st.sigma, W)) || (rc = dev_alloc(&st.stats, W * kStatCount)) ||
         (rc = dev_alloc(&st.logdet, W * 2)) || (rc = dev_alloc(&st.since, W)) ||
         (rc = dev_alloc(&st.flags, W)) || (rc = dev_alloc(&st.scratch, W * d.nmax * d.nmax)) || (rc = dev_alloc(&st.drift, W)) ||
-        (rc = dev_alloc(&st.jrow, W * d.N))) {
+        (rc = dev_alloc(&st.jrow, W * d.N)) || (rc = dev_alloc(&st.due, 2)) || (rc = dev_alloc(&st.due_list, W))) {
         delete e;
         return rc;
     }
@@ -850,7 +854,8 @@ int vmc_ensemble_create(const vmc_system *sys, int64_t n_walkers, uint64_t seed,
         hipMemset(st.since, 0, sizeof(int) * W) != hipSuccess ||
         hipMemset(st.flags, 0, sizeof(int) * W) != hipSuccess ||
         hipMemset(st.logdet, 0, sizeof(double) * W * 2) != hipSuccess ||
-        hipMemset(st.drift, 0, sizeof(double) * W) != hipSuccess) {
+        hipMemset(st.drift, 0, sizeof(double) * W) != hipSuccess ||
+        hipMemset(st.due, 0, sizeof(unsigned) * 2) != hipSuccess) {
         delete e;
         return fail(VMC_ERR_DEVICE, "ensemble initialisation copy failed");
     }

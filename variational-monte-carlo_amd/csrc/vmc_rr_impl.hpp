@@ -17,6 +17,12 @@ void rr_launch_g(const DevSys &d, const DevState &st, unsigned grid, int n_sweep
                  KernelTimer &timer) {
     const bool measure = flags & VMC_SWEEP_MEASURE;
     const bool reinv = d.reinv_k > 0;
+    // The sweep lists the due walkers; the re-inversion runs on at most
+    // kListGrid workgroups (eight per CU) over that list.
+    constexpr unsigned kListGrid = 2048;
+    const unsigned items = grid * (ND > 0 ? 2 : 1);
+    const unsigned rgrid = items < kListGrid ? items : kListGrid;
+    if (reinv) flags |= kSweepListDue;
     for (int i = 0; i < n_sweeps; ++i) {
         const unsigned long long si = s0 + (unsigned long long)i;
         hipEvent_t ev = nullptr;
@@ -30,8 +36,9 @@ void rr_launch_g(const DevSys &d, const DevState &st, unsigned grid, int n_sweep
         if (timed) timer.end(stream, ev, kTimeSweep);
         if (reinv) {
             timed = timer.begin(stream, ev);
-            hipLaunchKernelGGL((k_reinv_rr<Real, NU, ND, GPL>), dim3(grid * (ND > 0 ? 2 : 1)), dim3(kWave), 0,
-                               stream, d, st);
+            hipLaunchKernelGGL((k_reinv_rr_list<Real, NU, ND, GPL>), dim3(rgrid), dim3(kWave), 0, stream, d,
+                               st);
+            if (!measure) hipLaunchKernelGGL(k_due_reset<Real>, dim3(1), dim3(1), 0, stream, st);
             if (timed) timer.end(stream, ev, kTimeReinv);
         }
         if (measure) {

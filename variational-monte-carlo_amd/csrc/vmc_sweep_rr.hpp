@@ -976,6 +976,10 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
         st.sigma[w] = sh.misc[0];
         st.since[w] = since;
         st.flags[w] = wflags | kFlagJrowValid;
+        if ((flags & kSweepListDue) && s.reinv_k > 0 && since >= s.reinv_k) {
+            const unsigned i = atomicAdd(&st.due[0], 1u);
+            if (i < (unsigned)st.W) st.due_list[i] = w;
+        }
         st.logdet[2 * w + 0] = sh.misc[1];
         st.logdet[2 * w + 1] = sh.misc[2];
     }
@@ -1257,6 +1261,8 @@ __device__ __forceinline__ void measure_rr_body(const DevSys &s, const DevState 
     sh.pos[2][lane] = L.z;
     __syncthreads();
     const Measured m = K::measure(s, cst, sh, L, lane, has_e, h);
+    // The re-inversion that ran before this launch has read the list.
+    if (w == 0 && lane == 0 && st.due) st.due[0] = 0;
     if (lane == 0) {
         double *t = st.stats + (size_t)w * kStatCount;
         const double en = m.e_local;
@@ -1283,9 +1289,24 @@ __device__ __forceinline__ void measure_rr_body(const DevSys &s, const DevState 
     }
 }
 
+// The re-inversion over the walkers the preceding sweep listed (sweep
+// flag kSweepListDue): workgroup g takes items g, g + G, ... (item i: walker
+// due_list[i / spins], spin i % spins), so a launch of G workgroups costs the
+// due walkers' work rather than one workgroup per walker and spin.  The next
+// kernel on the stream empties the list (k_measure_rr, else k_due_reset).
 template <typename Real, int NU, int ND, int GPL>
-__global__ __launch_bounds__(kWave) void k_reinv_rr(DevSys s, DevState st) {
-    reinv_rr_body<Real, NU, ND, GPL>(s, st, blockIdx.x);
+__global__ __launch_bounds__(kWave) void k_reinv_rr_list(DevSys s, DevState st) {
+    constexpr unsigned kSpins = ND > 0 ? 2 : 1;
+    const unsigned n = st.due[0] < (unsigned)st.W ? st.due[0] : (unsigned)st.W;
+    for (unsigned i = blockIdx.x; i < n * kSpins; i += gridDim.x)
+        reinv_rr_body<Real, NU, ND, GPL>(s, st, (int)((unsigned)st.due_list[i / kSpins] * kSpins + i % kSpins));
+}
+
+// Empties the re-inversion list after k_reinv_rr_list when no local energy
+// follows.
+template <typename Real>
+__global__ void k_due_reset(DevState st) {
+    st.due[0] = 0;
 }
 
 // Waves per SIMD the measurement is built for: the structure-factor loop
