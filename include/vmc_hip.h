@@ -153,7 +153,7 @@ int vmc_ensemble_get_positions(vmc_ensemble *e, double *pos, void *stream);
  * per-walker accumulators and blocking (block_size measured sweeps per
  * block); VMC_SWEEP_ADAPT applies the warmup step rule
  * s *= exp(0.25 (acc - 0.5)), s <= L/2 after each sweep
- * (simulation.cpp:178-194). */
+ * (simulation.cpp:178-194).  Other flag bits are rejected (VMC_ERR_INVALID). */
 int vmc_ensemble_sweep(vmc_ensemble *e, int32_t n_sweeps, int32_t flags, int32_t block_size,
                        void *stream);
 

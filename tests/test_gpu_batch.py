@@ -144,6 +144,26 @@ def test_batch_regroup_drops_the_graph(vmc):
     b.close()
 
 
+def test_unknown_sweep_flags_are_rejected(vmc):
+    """Only VMC_SWEEP_MEASURE and VMC_SWEEP_ADAPT are sweep flags; other bits
+    (the library's internal ones among them) fail before any launch."""
+    sy = systems(vmc)[:2]
+    e = vmc.Ensemble(sy[0], 4, 7)
+    e.init_positions()
+    b = vmc.Batch(sy, [4, 4], [1, 2])
+    b.init_positions()
+    L = vmc.lib()
+    for bit in (4, 1 << 30):
+        assert L.vmc_ensemble_sweep(e._h, 1, bit, 1, e.stream) != 0
+        assert L.vmc_batch_sweep(b._h, 1, bit, 1, b.stream) != 0
+        assert L.vmc_batch_capture(b._h, 1, bit, 1) != 0
+    assert e.sweeps_done() == 0
+    e.sweep(1, measure=True)
+    assert e.sweeps_done() == 1
+    e.close()
+    b.close()
+
+
 def test_batch_members_refuse_direct_sweeps(vmc):
     b = vmc.Batch(systems(vmc)[:2], [4, 4], [1, 2])
     b.init_positions()

@@ -928,6 +928,7 @@ int vmc_ensemble_sweep(vmc_ensemble *e, int32_t n_sweeps, int32_t flags, int32_t
     if (e->in_batch) return fail(VMC_ERR_STATE, "ensemble belongs to a batch; sweep the batch");
     if (n_sweeps < 0) return fail(VMC_ERR_INVALID, "n_sweeps must be >= 0");
     if (block_size < 1) return fail(VMC_ERR_INVALID, "block_size must be >= 1");
+    if (flags & ~(VMC_SWEEP_MEASURE | VMC_SWEEP_ADAPT)) return fail(VMC_ERR_INVALID, "unknown sweep flags");
     if (n_sweeps == 0) return VMC_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     int rc = e->tables.real_bytes == 8 ? launch_sweep<double>(e, n_sweeps, flags, block_size, s)
@@ -1488,6 +1489,7 @@ int vmc_batch_init_positions(vmc_batch *b, void *stream) {
 int vmc_batch_sweep(vmc_batch *b, int32_t n_sweeps, int32_t flags, int32_t block_size, void *stream) {
     if (!b) return fail(VMC_ERR_INVALID, "batch is NULL");
     if (n_sweeps < 0 || block_size < 1) return fail(VMC_ERR_INVALID, "bad sweep arguments");
+    if (flags & ~(VMC_SWEEP_MEASURE | VMC_SWEEP_ADAPT)) return fail(VMC_ERR_INVALID, "unknown sweep flags");
     for (auto *m : b->members)
         if (!m->positions_ready) return fail(VMC_ERR_STATE, "positions not initialised");
     const int rc = batch_sweep_any(b, n_sweeps, flags, block_size, static_cast<hipStream_t>(stream));
@@ -1499,6 +1501,7 @@ int vmc_batch_sweep(vmc_batch *b, int32_t n_sweeps, int32_t flags, int32_t block
 int vmc_batch_capture(vmc_batch *b, int32_t n_sweeps, int32_t flags, int32_t block_size) {
     if (!b) return fail(VMC_ERR_INVALID, "batch is NULL");
     if (n_sweeps < 1 || block_size < 1) return fail(VMC_ERR_INVALID, "bad capture arguments");
+    if (flags & ~(VMC_SWEEP_MEASURE | VMC_SWEEP_ADAPT)) return fail(VMC_ERR_INVALID, "unknown sweep flags");
     if (b->exec) {
         VMC_HIP_TRY(hipGraphExecDestroy(b->exec));
         b->exec = nullptr;
