@@ -144,6 +144,31 @@ def test_batch_regroup_drops_the_graph(vmc):
     b.close()
 
 
+def test_reinversion_list_with_more_items_than_workgroups(vmc):
+    """4000 walkers of N=14 at K=16: a sweep accepts about 7 moves per walker,
+    so well over a third of the walkers are due after each sweep and the lone
+    ensemble's re-inversion list holds more (walker, spin) items than its 2048
+    workgroups, which then take several items each; the batch path
+    re-inverts with one workgroup per item.  Both give the same chains."""
+    s = vmc.System.heg(14, 1.0, 1.0, precision=1, reinvert_interval=16)
+    W, seed = 4000, 77
+    b = vmc.Batch([s], [W], [seed])
+    b.init_positions()
+    b.sweep(2, adapt=True, block_size=2)
+    b.sweep(3, measure=True, block_size=2)
+    got = snapshot_one(b)
+    b.close()
+    ref = alone(vmc, s, W, seed, 0)
+    assert_same([got], [ref])
+    assert np.all(ref[1][:, vmc.STAT["NMEAS"]] == 3)
+    assert ref[1][:, vmc.STAT["REINV"]].sum() > W
+
+
+def snapshot_one(b):
+    m = b.member(0)
+    return (m.positions(), m.stats(), m.steps())
+
+
 def test_unknown_sweep_flags_are_rejected(vmc):
     """Only VMC_SWEEP_MEASURE and VMC_SWEEP_ADAPT are sweep flags; other bits
     (the library's internal ones among them) fail before any launch."""
