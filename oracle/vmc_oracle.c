@@ -1154,13 +1154,21 @@ int orc_init_positions(const orc_system *sys, uint64_t seed, uint64_t walker, do
     return used;
 }
 
+/* Electron moved at step t of a sweep: 0..N-1 in order, except that with
+   equal spins the two spins alternate (up 0, down 0, up 1, down 1, ...). */
+static int sweep_visit(int n_up, int n_dn, int t) {
+    if (n_up != n_dn) return t;
+    return (t & 1) ? n_up + (t >> 1) : (t >> 1);
+}
+
 int orc_sweep(orc_walker *w, const orc_sweep_config *cfg, uint64_t walker, uint64_t sweep,
               double *step, int adapt, double *e_local) {
     const int N = w->N;
     const double L = w->sys.L;
     const double inv_L = 1.0 / L;
     int accepted = 0;
-    for (int p = 0; p < N; ++p) {
+    for (int t = 0; t < N; ++t) {
+        const int p = sweep_visit(w->sys.n_up, w->sys.n_dn, t);
         uint32_t r[4], ra[4];
         philox_draw(cfg->seed, (uint32_t)p, walker, sweep, TAG_MOVE, r);
         philox_draw(cfg->seed, (uint32_t)p, walker, sweep, TAG_ACCP, ra);

@@ -166,8 +166,9 @@ typedef struct {
    -1.  pos receives AoS [N][3]. */
 int orc_init_positions(const orc_system *sys, uint64_t seed, uint64_t walker, double *pos);
 
-/* One sequential sweep (electrons 0..N-1) of walker `walker` at global sweep
-   index `sweep`.  Returns the number of accepted moves; *step is adapted when
+/* One sequential sweep of walker `walker` at global sweep index `sweep`:
+   electrons 0..N-1 in order, or, with equal spins, the spins alternating
+   (up 0, down 0, up 1, down 1, ...).  Returns the number of accepted moves; *step is adapted when
    adapt != 0.  When e_local != NULL the local energy after the sweep is
    written there. */
 int orc_sweep(orc_walker *w, const orc_sweep_config *cfg, uint64_t walker, uint64_t sweep,

@@ -66,6 +66,13 @@ struct DevState {
 // re-inversion launch that follows it.
 constexpr int kSweepListDue = 1 << 30;
 
+// Electron moved at step t of a sweep: 0..N-1 in order, except that with
+// equal spins the two spins alternate (up 0, down 0, up 1, down 1, ...), so a
+// register sweep can evaluate an up and a down move together.
+__host__ __device__ constexpr int sweep_visit(int n_up, int n_dn, int t) {
+    return n_up != n_dn ? t : ((t & 1) ? n_up + (t >> 1) : (t >> 1));
+}
+
 // Walker flag bits (DevState::flags).
 constexpr int kFlagSingular = 1;  // a singular determinant was met
 constexpr int kFlagJrowValid = 2;  // jrow holds the current positions' row sums

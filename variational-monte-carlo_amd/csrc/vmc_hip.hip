@@ -160,7 +160,8 @@ __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kI
         }
         wave_sync();
         int acc = 0;
-        for (int p = 0; p < N; ++p) {
+        for (int t = 0; t < N; ++t) {
+            const int p = sweep_visit(s.n_up, s.n_dn, t);
             double x = v.px[p] + sigma * v.draw[p];
             double y = v.py[p] + sigma * v.draw[N + p];
             double z = v.pz[p] + sigma * v.draw[2 * N + p];

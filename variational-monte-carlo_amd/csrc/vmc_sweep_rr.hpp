@@ -940,15 +940,19 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
     // out of the move loop, so its transcendental constants die before it.
     if (has_e) rr_proposal<Real>(s, sh.misc[0], L.x, L.y, L.z, e, w, sweep, lane, &sh.prop[0][0], &sh.pph[0][0]);
     __syncthreads();
-    // One loop per spin: a single loop with a branch per move lets the
-    // compiler merge the two moves' Sherman-Morrison tails, splitting each
-    // FMA into a multiply and a subtract.
+    // Sweep order (sweep_visit): with two determinants (ND == NU) the spins
+    // alternate, up p then down p.
     int acc = 0;
-#pragma unroll 1
-    for (int p = 0; p < NU; ++p) acc += K::template move<0>(cst, sh, L, lane, has_e, h, r, p);
     if constexpr (ND > 0) {
+        static_assert(ND == NU, "two determinants of equal size");
 #pragma unroll 1
-        for (int p = 0; p < ND; ++p) acc += K::template move<1>(cst, sh, L, lane, has_e, h, r, p);
+        for (int p = 0; p < NU; ++p) {
+            acc += K::template move<0>(cst, sh, L, lane, has_e, h, r, p);
+            acc += K::template move<1>(cst, sh, L, lane, has_e, h, r, p);
+        }
+    } else {
+#pragma unroll 1
+        for (int p = 0; p < NU; ++p) acc += K::template move<0>(cst, sh, L, lane, has_e, h, r, p);
     }
     const int n_acc = acc;
     since += acc;

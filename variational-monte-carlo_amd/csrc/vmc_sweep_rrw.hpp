@@ -151,9 +151,10 @@ struct RRW {
         return U;
     }
 
-    // One move of electron p.  Returns 1 if accepted (uniform).
+    // One move of electron p, the sweep's move number par (mod 2) selecting
+    // the LDS buffers phase C reads.  Returns 1 if accepted (uniform).
     __device__ static int move(const RRConst<Real> &c, Sh &sh, const Thread &T, Real (&inv)[NMAX],
-                               double &x, double &y, double &z, double &U, int ob, int p) {
+                               double &x, double &y, double &z, double &U, int ob, int p, int par) {
         const int hp = (ND > 0 && p >= NU) ? 1 : 0;
         const int rp = hp ? p - NU : p;
         const int NP = hp ? ND : NU;
@@ -178,7 +179,6 @@ struct RRW {
             const Real a = (T.h == hp) ? c.a_same : c.a_anti;
             un = a * rn * wrcp<Real>((Real)1 + c.b * rn);
         }
-        const int par = p & 1;
         {
             const Real ws = rr::wave_total(un);
             if (T.lane == 0) sh.jred[par][T.v] = (double)ws;
@@ -332,7 +332,7 @@ __device__ __forceinline__ void sweep_rrw_body(const DevSys &s, const DevState &
     __syncthreads();
     int acc = 0;
 #pragma unroll 1
-    for (int p = 0; p < N; ++p) acc += K::move(cst, sh, T, inv, x, y, z, U, ob, p);
+    for (int t = 0; t < N; ++t) acc += K::move(cst, sh, T, inv, x, y, z, U, ob, sweep_visit(NU, ND, t), t & 1);
     __syncthreads();
     if (flags & VMC_SWEEP_ADAPT) {
         const double f = exp(0.25 * ((double)acc / (double)N - 0.5));
