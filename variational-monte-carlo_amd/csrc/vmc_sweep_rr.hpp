@@ -218,7 +218,7 @@ struct RRShared {
     Real tab[kRRChunk * 3 * (kRRMaxPow + 1) * 2];  // with prop and pph: staging (below)
     double stat[kStatCount];
     double misc[4];                // sigma, log|det| up, log|det| down
-    Real phi[32];                  // orbital row of the pending move
+    Real phi[2][32];               // orbital rows of the pending moves (by spin)
     Real row[32];                  // inverse row of the moved electron
     double gjcol[32];
     int perm[32];
@@ -746,6 +746,143 @@ struct RRKernel {
         return rr::wave_total(sf::energy<Real, N, GPL, kWave>(s, tab, lane, pos_of));
     }
 
+    // Metropolis test u < R^2 exp(2 dJ) (simulation.cpp:125-131).  The
+    // hardware exp2 (relative error < 1e-6 here) decides unless u lies within
+    // 1e-5 of the threshold, where the exact FP64 exp decides, so the outcome
+    // equals the FP64 comparison.  Inputs are wave-uniform; so is the result.
+    __device__ static int accept_test(double R, double dj, double ua) {
+        const double r2 = R * R;
+        const float e32 = __builtin_amdgcn_exp2f((float)(2.0 * dj * 1.4426950408889634));
+        const double approx = r2 * (double)e32;
+        int acc;
+        if (!(approx == approx)) {
+            acc = 0;
+        } else if (fabs(ua - approx) > 1e-5 * approx || !(fabs(dj) < 40.0)) {
+            acc = ua < approx;
+        } else {
+            acc = ua < r2 * rr_exp_f64(2.0 * dj);
+        }
+        return wave_uniform(acc);
+    }
+
+    // Pade pair value of the lane's electron with the point (xq, yq, zq).
+    __device__ static Real pair_with(const RRConst<Real> &c, const RRLane<Real, NMAX> &L, double xq, double yq,
+                                     double zq, Real a) {
+        const Real dx = c.mi_r(xq, L.x);
+        const Real dy = c.mi_r(yq, L.y);
+        const Real dz = c.mi_r(zq, L.z);
+        const Real rn = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+        return a * rn * wrcp<Real>((Real)1 + c.b * rn);
+    }
+
+    // Sherman-Morrison for an accepted move of electron (hm, i) with ratio R
+    // (row s_k of every lane of spin hm), then the Jastrow row sums: every
+    // other lane adds its new pair term un minus its old one, recomputed from
+    // the mover's old position.  Lane lm takes the proposal (xn, yn, zn) and
+    // the row sum jn.
+    __device__ static void apply_move(const RRConst<Real> &c, Sh &sh, RRLane<Real, NMAX> &L, int lane,
+                                      bool has_e, int h, int r, int hm, int i, double R, Real sk, Real un,
+                                      double jn, double xn, double yn, double zn) {
+        constexpr int NP = NU;
+        const int lm = hm * 32 + i;
+        if (lane == lm) {
+#pragma unroll
+            for (int m = 0; m < NP; ++m) sh.row[m] = L.inv[m];
+        }
+        __syncthreads();
+        {
+            const Real inv_ratio = wrcp<Real>((Real)R);
+            Real f = (r == i) ? (Real)1 - inv_ratio : sk * inv_ratio;
+            f = (h == hm && r < NP) ? f : (Real)0;
+#pragma unroll
+            for (int m = 0; m < NP; ++m) L.inv[m] -= sh.row[m] * f;
+        }
+        const double xo = rr::readlane(L.x, lm), yo = rr::readlane(L.y, lm), zo = rr::readlane(L.z, lm);
+        if (has_e && lane != lm) {
+            const Real ao = (h == hm) ? c.a_same : c.a_anti;
+            L.U += (double)un - (double)pair_with(c, L, xo, yo, zo, ao);
+        }
+        if (lane == lm) {
+            L.U = jn;
+            L.x = xn;
+            L.y = yn;
+            L.z = zn;
+            sh.pos[0][lane] = xn;
+            sh.pos[1][lane] = yn;
+            sh.pos[2][lane] = zn;
+        }
+    }
+
+    // The moves of up electron i and then down electron i (the sweep order
+    // of equal spins), evaluated together: the two determinants are
+    // independent, so the wavefront's two halves form the orbital rows and
+    // ratios of both proposals at once, and every lane forms its pair terms
+    // with both.  The down move's Jastrow change uses the up electron where
+    // the up move left it: its pair term with the down proposal is formed for
+    // both outcomes.  Same decisions and updates as two calls of move.
+    // Returns the number of accepted moves.
+    __device__ static int move_pair(const RRConst<Real> &c, Sh &sh, RRLane<Real, NMAX> &L, int lane,
+                                    bool has_e, int h, int r, int i) {
+        static_assert(ND == NU, "paired moves need equal spins");
+        constexpr int NP = NU;
+        const int lu = i, ld = 32 + i;
+        const double xu = sh.prop[0][lu], yu = sh.prop[1][lu], zu = sh.prop[2][lu];
+        const double xd = sh.prop[0][ld], yd = sh.prop[1][ld], zd = sh.prop[2][ld];
+        // Orbital rows: lane (h, j) evaluates orbital j at its spin's proposal.
+        {
+            const int lp = h ? ld : lu;
+            rr::Cx<Real> a[3], a2[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                a[k].re = sh.pph[2 * k][lp];
+                a[k].im = sh.pph[2 * k + 1][lp];
+            }
+            squares(a, a2);
+            if (r < NP) {
+                const rr::Cx<Real> ph = rr::orbital_phasor<Real>(orb_nx(L.orb), orb_ny(L.orb), orb_nz(L.orb), a, a2);
+                sh.phi[h][r] = orb_t(L.orb) ? ph.im : ph.re;
+            }
+        }
+        // Jastrow: new pair terms against both proposals; the down proposal's
+        // pair with the up proposal (taken when the up move is accepted).
+        Real un_u = (Real)0, un_d = (Real)0;
+        if (has_e && lane != lu) un_u = pair_with(c, L, xu, yu, zu, h == 0 ? c.a_same : c.a_anti);
+        if (has_e && lane != ld) un_d = pair_with(c, L, xd, yd, zd, h == 1 ? c.a_same : c.a_anti);
+        Real ud_new;
+        {
+            const Real dx = c.mi_r(xd, xu), dy = c.mi_r(yd, yu), dz = c.mi_r(zd, zu);
+            const Real rn = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+            ud_new = c.a_anti * rn * wrcp<Real>((Real)1 + c.b * rn);
+        }
+        __syncthreads();
+        // s_k = sum_m phi_m invT[k][m] with the lane's spin's orbital row.
+        Real sk0 = (Real)0, sk1 = (Real)0;
+#pragma unroll
+        for (int m = 0; m < NP; m += 2) {
+            sk0 += sh.phi[h][m] * L.inv[m];
+            if (m + 1 < NP) sk1 += sh.phi[h][m + 1] * L.inv[m + 1];
+        }
+        const Real sk = sk0 + sk1;
+        const double Ru = (double)rr::readlane(sk, lu), Rd = (double)rr::readlane(sk, ld);
+        const double jn_u = (double)rr::wave_total(un_u);
+        const double jn_d0 = (double)rr::wave_total(un_d);    // up electron at its old position
+        const Real ud_old = rr::readlane(un_d, lu);
+        // Up move.
+        const int acc_u = accept_test(Ru, jn_u - rr::readlane(L.U, lu), sh.prop[3][lu]);
+        if (acc_u) {
+            apply_move(c, sh, L, lane, has_e, h, r, 0, i, Ru, sk, un_u, jn_u, xu, yu, zu);
+            if (lane == lu) un_d = ud_new;
+        }
+        // Down move, after the up move's outcome.
+        const double jn_d = acc_u ? jn_d0 + ((double)ud_new - (double)ud_old) : jn_d0;
+        const int acc_d = accept_test(Rd, jn_d - rr::readlane(L.U, ld), sh.prop[3][ld]);
+        if (acc_d) {
+            __syncthreads();  // the up move's reads of the broadcast row are done
+            apply_move(c, sh, L, lane, has_e, h, r, 1, i, Rd, sk, un_d, jn_d, xd, yd, zd);
+        }
+        return acc_u + acc_d;
+    }
+
     // One move of electron (HP, rp).  Returns 1 if accepted.
     template <int HP>
     __device__ static int move(const RRConst<Real> &c, Sh &sh, RRLane<Real, NMAX> &L, int lane,
@@ -763,7 +900,7 @@ struct RRKernel {
         // Orbital row of the proposal: lane (HP, j) evaluates orbital j.
         if (h == HP && r < NP) {
             const rr::Cx<Real> ph = rr::orbital_phasor<Real>(orb_nx(L.orb), orb_ny(L.orb), orb_nz(L.orb), a, a2);
-            sh.phi[r] = orb_t(L.orb) ? ph.im : ph.re;
+            sh.phi[HP][r] = orb_t(L.orb) ? ph.im : ph.re;
         }
         // Jastrow: new pair terms against every other electron.
         const Real apair = (h == HP) ? c.a_same : c.a_anti;
@@ -781,8 +918,8 @@ struct RRKernel {
         Real sk0 = (Real)0, sk1 = (Real)0;
 #pragma unroll
         for (int m = 0; m < NP; m += 2) {
-            sk0 += sh.phi[m] * L.inv[m];
-            if (m + 1 < NP) sk1 += sh.phi[m + 1] * L.inv[m + 1];
+            sk0 += sh.phi[HP][m] * L.inv[m];
+            if (m + 1 < NP) sk1 += sh.phi[HP][m + 1] * L.inv[m + 1];
         }
         const Real sk = sk0 + sk1;
         const double R = (double)rr::readlane(sk, lp);
@@ -946,10 +1083,7 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
     if constexpr (ND > 0) {
         static_assert(ND == NU, "two determinants of equal size");
 #pragma unroll 1
-        for (int p = 0; p < NU; ++p) {
-            acc += K::template move<0>(cst, sh, L, lane, has_e, h, r, p);
-            acc += K::template move<1>(cst, sh, L, lane, has_e, h, r, p);
-        }
+        for (int p = 0; p < NU; ++p) acc += K::move_pair(cst, sh, L, lane, has_e, h, r, p);
     } else {
 #pragma unroll 1
         for (int p = 0; p < NU; ++p) acc += K::template move<0>(cst, sh, L, lane, has_e, h, r, p);
@@ -993,8 +1127,16 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
 
 // The sweep kernel, without and with a 4-waves-per-SIMD occupancy bound (the
 // bound caps VGPRs at 128 at the price of some spills; the autotuner picks).
+// Waves per SIMD the sweep is built for (its registers at that occupancy).
+template <typename Real, int NU, int ND>
+constexpr int rr_sweep_waves() {
+    constexpr bool kLarge = (NU > ND ? NU : ND) >= 19;
+    if constexpr (sizeof(Real) == 8) return kLarge ? 3 : 4;
+    return kLarge ? 5 : 6;
+}
+
 template <typename Real, int NU, int ND, int GPL>
-__global__ __launch_bounds__(kWave) void k_sweep_rr(DevSys s, DevState st,
+__global__ __launch_bounds__(kWave, (rr_sweep_waves<Real, NU, ND>())) void k_sweep_rr(DevSys s, DevState st,
                                                     unsigned long long sweep0, int flags) {
     sweep_rr_body<Real, NU, ND, GPL>(s, st, blockIdx.x, sweep0, flags);
 }
@@ -1334,7 +1476,8 @@ __global__ __launch_bounds__(kWave, (rr_measure_waves<Real, NU + ND, GPL>())) vo
 // sweep index comes from device memory (so a captured graph continues the
 // chains).  A system's chain is the same in a batch and alone.
 template <typename Real, int NU, int ND, int GPL>
-__global__ __launch_bounds__(kWave) void k_sweep_rr_b(const BatchItem *__restrict__ items, int n_items,
+__global__ __launch_bounds__(kWave, (rr_sweep_waves<Real, NU, ND>())) void k_sweep_rr_b(
+    const BatchItem *__restrict__ items, int n_items,
                                                       const unsigned long long *__restrict__ sweep_ctr,
                                                       int flags) {
     const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
