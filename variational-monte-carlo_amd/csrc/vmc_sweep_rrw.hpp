@@ -33,12 +33,15 @@ struct RRWShared {
     alignas(16) double prop[4][N];  // this sweep's proposals (x, y, z) and accept uniforms
     Real pph[6][N];      // axis phasors of the proposals
     Real tab[kRRChunk * 3 * (kRRMaxPow + 1) * 2];
-    Real phi[NMAX];      // orbital row of the pending move
-    Real row[2][NMAX];   // inverse row of the mover (by move parity)
+    Real phi[2][NMAX];   // orbital rows of the pending moves (by spin)
+    Real row[2][NMAX];   // inverse row of the (up) mover (by move parity)
+    Real row2[2][NMAX];  // inverse row of the down mover of a paired move
     double red[NWAVE];   // per-wave partial sums
     double red4[NWAVE][4];
-    double jred[2][NWAVE];  // per-wave new Jastrow pair sums (by move parity)
-    double mv[2][8];     // mover: R, U, x, y, z (by move parity)
+    double jred[2][NWAVE];   // per-wave new Jastrow pair sums (by move parity)
+    double jred2[2][NWAVE];  // the same for the down proposal of a paired move
+    double mv[2][8];     // (up) mover: R, U, x, y, z (by move parity)
+    double mv2[2][8];    // down mover of a paired move
     double stat[kStatCount];
     double misc[4];      // sigma
 };
@@ -55,6 +58,11 @@ struct RRW {
     static_assert(NMAX <= WPS * kWave, "rows per spin exceed the spin's wavefronts");
     static constexpr int kMaxC = rr::max_component<NMAX>();
     static constexpr bool kOneBarrier = WPS == 1 && NMAX >= 57;
+    // Sizes whose sweep moves an up and a down electron per pass (move_pair):
+    // measured faster there; at 81 and 123 per spin in FP32 (and 33, 57 in
+    // FP64) the paired pass's registers cost more than it saves.
+    static constexpr bool kPaired = ND == NU && (sizeof(Real) == 4 ? (NMAX == 33 || NMAX == 57 || NMAX == 147)
+                                                                  : NMAX == 81);
     static_assert(kMaxC <= 3, "orbital phasors use powers up to 3");
 
     // e^{2 pi i n.u} from the axis phasors a and their powers.
@@ -151,6 +159,172 @@ struct RRW {
         return U;
     }
 
+    // Pade pair value of the point (xn, yn, zn) with (x, y, z), as the moves
+    // form it.
+    __device__ static Real pair_at(const RRConst<Real> &c, double xn, double yn, double zn, double x, double y,
+                                   double z, Real a) {
+        const Real dx = c.mi_r(xn, x);
+        const Real dy = c.mi_r(yn, y);
+        const Real dz = c.mi_r(zn, z);
+        const Real rn = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+        return a * rn * wrcp<Real>((Real)1 + c.b * rn);
+    }
+
+    // Metropolis test (simulation.cpp:125-131), exact FP64 exponential within
+    // 1e-5 of the threshold; wave-uniform inputs and result.
+    __device__ static int accept_test(double R, double dj, double ua) {
+        const double r2 = R * R;
+        const float e32 = __builtin_amdgcn_exp2f((float)(2.0 * dj * 1.4426950408889634));
+        const double approx = r2 * (double)e32;
+        int acc;
+        if (!(approx == approx)) acc = 0;
+        else if (fabs(ua - approx) > 1e-5 * approx || !(fabs(dj) < 40.0)) acc = ua < approx;
+        else acc = ua < r2 * rr_exp_f64(2.0 * dj);
+        return wave_uniform(acc);
+    }
+
+    // The moves of up electron i and then down electron i (the sweep order
+    // of equal spins) in one pass: each spin's wavefronts form their orbital
+    // row and ratios at their spin's proposal, and every thread forms its pair
+    // terms with both proposals.  The down move then takes the up move's
+    // outcome into its Jastrow change (the pair terms of the up electron with
+    // the down electron, formed as the threads form them).  Same decisions
+    // and updates as two calls of move.  Returns the accepted moves.
+    __device__ static int move_pair(const RRConst<Real> &c, Sh &sh, const Thread &T, Real (&inv)[NMAX],
+                                    double &x, double &y, double &z, double &U, int ob, int i, int par) {
+        static_assert(ND == NU, "paired moves need equal spins");
+        const int pu = i, pd = NU + i;
+        const double xu = sh.prop[0][pu], yu = sh.prop[1][pu], zu = sh.prop[2][pu];
+        const double xd = sh.prop[0][pd], yd = sh.prop[1][pd], zd = sh.prop[2][pd];
+        // Phase A.
+        if (T.has_e) {
+            const int pp = T.h ? pd : pu;
+            rr::Cx<Real> a[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                a[k].re = sh.pph[2 * k][pp];
+                a[k].im = sh.pph[2 * k + 1][pp];
+            }
+            const rr::Cx<Real> q = phasor(orb_nx(ob), orb_ny(ob), orb_nz(ob), a);
+            sh.phi[T.h][T.r] = orb_t(ob) ? q.im : q.re;
+        }
+        Real un_u = (Real)0, un_d = (Real)0;
+        if (T.has_e && T.e != pu) un_u = pair_at(c, xu, yu, zu, x, y, z, T.h == 0 ? c.a_same : c.a_anti);
+        if (T.has_e && T.e != pd) un_d = pair_at(c, xd, yd, zd, x, y, z, T.h == 1 ? c.a_same : c.a_anti);
+        {
+            const Real wu = rr::wave_total(un_u), wd = rr::wave_total(un_d);
+            if (T.lane == 0) {
+                sh.jred[par][T.v] = (double)wu;
+                sh.jred2[par][T.v] = (double)wd;
+            }
+        }
+        if constexpr (kOneBarrier) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            __syncthreads();
+        }
+        // Phase B: every thread's s_k with its spin's orbital row; the two
+        // movers publish their values and rows.
+        Real sk;
+        {
+            Real s0 = 0, s1 = 0;
+#pragma unroll
+            for (int m = 0; m < NMAX; m += 2) {
+                s0 += sh.phi[T.h][m] * inv[m];
+                if (m + 1 < NMAX) s1 += sh.phi[T.h][m + 1] * inv[m + 1];
+            }
+            sk = s0 + s1;
+        }
+        if (T.r == i) {
+            double *mvp = T.h ? sh.mv2[par] : sh.mv[par];
+            Real *rowp = T.h ? sh.row2[par] : sh.row[par];
+            mvp[0] = (double)sk;
+            mvp[1] = U;
+            mvp[2] = x;
+            mvp[3] = y;
+            mvp[4] = z;
+#pragma unroll
+            for (int m = 0; m < NMAX; ++m) rowp[m] = inv[m];
+        }
+        double jn_u = 0.0, jn_d = 0.0;
+        if constexpr (!kOneBarrier) {
+#pragma unroll
+            for (int v = 0; v < NWAVE; ++v) {
+                jn_u += sh.jred[par][v];
+                jn_d += sh.jred2[par][v];
+            }
+        }
+        __syncthreads();
+        if constexpr (kOneBarrier) {
+#pragma unroll
+            for (int v = 0; v < NWAVE; ++v) {
+                jn_u += sh.jred[par][v];
+                jn_d += sh.jred2[par][v];
+            }
+        }
+        // Phase C, up move.
+        const double Ru = sh.mv[par][0];
+        const double uxo = sh.mv[par][2], uyo = sh.mv[par][3], uzo = sh.mv[par][4];
+        const int acc_u = accept_test(Ru, jn_u - sh.mv[par][1], sh.prop[3][pu]);
+        double Ud = sh.mv2[par][1];
+        if (acc_u) {
+            if (T.h == 0) {
+                const Real inv_ratio = wrcp<Real>((Real)Ru);
+                Real f = (T.r == i) ? (Real)1 - inv_ratio : sk * inv_ratio;
+                f = T.has_e ? f : (Real)0;
+#pragma unroll
+                for (int m = 0; m < NMAX; ++m) inv[m] -= sh.row[par][m] * f;
+            }
+            if (T.has_e && T.e != pu)
+                U += (double)un_u - (double)pair_at(c, uxo, uyo, uzo, x, y, z, T.h == 0 ? c.a_same : c.a_anti);
+            if (T.has_e && T.e == pu) {
+                U = jn_u;
+                x = xu;
+                y = yu;
+                z = zu;
+                sh.pos[0][pu] = xu;
+                sh.pos[1][pu] = yu;
+                sh.pos[2][pu] = zu;
+            }
+            // The down mover's row sum after this update, and the down
+            // proposal's pair with the up electron where it now is.
+            const double dxo = sh.mv2[par][2], dyo = sh.mv2[par][3], dzo = sh.mv2[par][4];
+            Ud += (double)pair_at(c, xu, yu, zu, dxo, dyo, dzo, c.a_anti) -
+                  (double)pair_at(c, uxo, uyo, uzo, dxo, dyo, dzo, c.a_anti);
+            const Real ud_new = pair_at(c, xd, yd, zd, xu, yu, zu, c.a_anti);
+            const Real ud_old = pair_at(c, xd, yd, zd, uxo, uyo, uzo, c.a_anti);
+            jn_d += (double)ud_new - (double)ud_old;
+            if (T.has_e && T.e == pu) un_d = ud_new;
+        }
+        // Phase C, down move.
+        const double Rd = sh.mv2[par][0];
+        const int acc_d = accept_test(Rd, jn_d - Ud, sh.prop[3][pd]);
+        if (acc_d) {
+            if (T.h == 1) {
+                const Real inv_ratio = wrcp<Real>((Real)Rd);
+                Real f = (T.r == i) ? (Real)1 - inv_ratio : sk * inv_ratio;
+                f = T.has_e ? f : (Real)0;
+#pragma unroll
+                for (int m = 0; m < NMAX; ++m) inv[m] -= sh.row2[par][m] * f;
+            }
+            const double dxo = sh.mv2[par][2], dyo = sh.mv2[par][3], dzo = sh.mv2[par][4];
+            if (T.has_e && T.e != pd)
+                U += (double)un_d - (double)pair_at(c, dxo, dyo, dzo, x, y, z, T.h == 1 ? c.a_same : c.a_anti);
+            if (T.has_e && T.e == pd) {
+                U = jn_d;
+                x = xd;
+                y = yd;
+                z = zd;
+                sh.pos[0][pd] = xd;
+                sh.pos[1][pd] = yd;
+                sh.pos[2][pd] = zd;
+            }
+        }
+        return acc_u + acc_d;
+    }
+
     // One move of electron p, the sweep's move number par (mod 2) selecting
     // the LDS buffers phase C reads.  Returns 1 if accepted (uniform).
     __device__ static int move(const RRConst<Real> &c, Sh &sh, const Thread &T, Real (&inv)[NMAX],
@@ -168,7 +342,7 @@ struct RRW {
                 a[k].im = sh.pph[2 * k + 1][p];
             }
             const rr::Cx<Real> q = phasor(orb_nx(ob), orb_ny(ob), orb_nz(ob), a);
-            sh.phi[T.r] = orb_t(ob) ? q.im : q.re;
+            sh.phi[hp][T.r] = orb_t(ob) ? q.im : q.re;
         }
         Real un = (Real)0;
         if (T.has_e && T.e != p) {
@@ -203,8 +377,8 @@ struct RRW {
             Real s0 = 0, s1 = 0;
 #pragma unroll
             for (int m = 0; m < NMAX; m += 2) {
-                s0 += sh.phi[m] * inv[m];
-                if (m + 1 < NMAX) s1 += sh.phi[m + 1] * inv[m + 1];
+                s0 += sh.phi[hp][m] * inv[m];
+                if (m + 1 < NMAX) s1 += sh.phi[hp][m + 1] * inv[m + 1];
             }
             sk = s0 + s1;
             if (T.r == rp) {
@@ -331,8 +505,13 @@ __device__ __forceinline__ void sweep_rrw_body(const DevSys &s, const DevState &
         rrw_proposal<Real>(s, sh.misc[0], x, y, z, T.e, w, (uint32_t)sweep0, N, &sh.prop[0][0], &sh.pph[0][0]);
     __syncthreads();
     int acc = 0;
+    if constexpr (K::kPaired) {
 #pragma unroll 1
-    for (int t = 0; t < N; ++t) acc += K::move(cst, sh, T, inv, x, y, z, U, ob, sweep_visit(NU, ND, t), t & 1);
+        for (int i = 0; i < NU; ++i) acc += K::move_pair(cst, sh, T, inv, x, y, z, U, ob, i, i & 1);
+    } else {
+#pragma unroll 1
+        for (int t = 0; t < N; ++t) acc += K::move(cst, sh, T, inv, x, y, z, U, ob, sweep_visit(NU, ND, t), t & 1);
+    }
     __syncthreads();
     if (flags & VMC_SWEEP_ADAPT) {
         const double f = exp(0.25 * ((double)acc / (double)N - 0.5));
