@@ -1127,11 +1127,14 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
 
 // The sweep kernel, without and with a 4-waves-per-SIMD occupancy bound (the
 // bound caps VGPRs at 128 at the price of some spills; the autotuner picks).
-// Waves per SIMD the sweep is built for (its registers at that occupancy).
+// Waves per SIMD the sweep is built for (its registers at that occupancy;
+// the paired FP32 move at 27 per spin needs 100 VGPRs, four waves, without
+// spilling).
 template <typename Real, int NU, int ND>
 constexpr int rr_sweep_waves() {
     constexpr bool kLarge = (NU > ND ? NU : ND) >= 19;
     if constexpr (sizeof(Real) == 8) return kLarge ? 3 : 4;
+    if constexpr ((NU > ND ? NU : ND) >= 27) return 4;
     return kLarge ? 5 : 6;
 }
 
