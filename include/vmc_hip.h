@@ -147,7 +147,8 @@ int vmc_ensemble_set_positions(vmc_ensemble *e, const double *pos, void *stream)
 int vmc_ensemble_get_positions(vmc_ensemble *e, double *pos, void *stream);
 
 /* n_sweeps sequential sweeps (every electron proposed once per sweep, in
- * index order) of every walker in one launch.  Replaces the warmup/measure
+ * index order, or with equal spins alternating up 0, down 0, up 1, ...) of
+ * every walker in one launch.  Replaces the warmup/measure
  * loops over Simulation::metropolis_step (simulation.cpp:80-263).
  * VMC_SWEEP_MEASURE evaluates the local energy after each sweep and feeds the
  * per-walker accumulators and blocking (block_size measured sweeps per

@@ -97,8 +97,8 @@ __global__ __launch_bounds__(kWave) void k_init(DevSys s, DevState st, int draw)
 }
 
 // n_sweeps sequential sweeps of walker blockIdx.x (simulation.cpp:80-263 as
-// an ensemble): per electron p a Philox proposal (tag MOVE, counter
-// {p, walker, sweep}), ratio and Jastrow change, Metropolis test with a
+// an ensemble): per electron p, in the sweep order of sweep_visit, a Philox
+// proposal (tag MOVE, counter {p, walker, sweep}), ratio and Jastrow change, Metropolis test with a
 // Philox uniform (tag ACCP), Sherman-Morrison on accept; after each sweep the
 // optional step adaptation and local-energy measurement.
 template <typename Real>

@@ -6,7 +6,8 @@
 // the measured sweeps and returns a MeasurementSummary.  Behind it, the
 // walker ensemble lives on the device (include/vmc_hip.h): every walker is an
 // independent chain (the reference runs one chain per std::async thread,
-// src/main.cpp:50-68), electrons are proposed in order once per sweep, and
+// src/main.cpp:50-68), each electron is proposed once per sweep in a fixed
+// order (index order, or the two spins alternating when they are equal), and
 // the local energy is measured after every sweep.  Walker 0's frames stream
 // to the writer, the way the reference's thread 0 streams its own.
 //
