@@ -33,9 +33,12 @@ struct RRWShared {
     alignas(16) double prop[4][N];  // this sweep's proposals (x, y, z) and accept uniforms
     Real pph[6][N];      // axis phasors of the proposals
     Real tab[kRRChunk * 3 * (kRRMaxPow + 1) * 2];
-    Real phi[2][NMAX];   // orbital rows of the pending moves (by spin)
-    Real row[2][NMAX];   // inverse row of the (up) mover (by move parity)
-    Real row2[2][NMAX];  // inverse row of the down mover of a paired move
+    // Rows padded to 16 bytes, so that a row picked at run time still moves
+    // with 16-byte LDS reads.
+    static constexpr int kRowPad = (NMAX + 3) / 4 * 4;
+    alignas(16) Real phi[2][kRowPad];   // orbital rows of the pending moves (by spin)
+    alignas(16) Real row[2][kRowPad];   // inverse row of the (up) mover (by move parity)
+    alignas(16) Real row2[2][kRowPad];  // inverse row of the down mover of a paired move
     double red[NWAVE];   // per-wave partial sums
     double red4[NWAVE][4];
     double jred[2][NWAVE];   // per-wave new Jastrow pair sums (by move parity)
@@ -739,7 +742,7 @@ __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState
 template <typename Real, int NU, int ND, int WPS>
 constexpr int rrw_sweep_waves() {
     return (sizeof(Real) == 4 && ((NU == 57 && ND == 57) || (NU == 81 && ND == 81))) ? 4
-           : (sizeof(Real) == 4 && NU == 123 && ND == 123) ? 3 : 1;
+           : (NU == 123 && ND == 123) || (sizeof(Real) == 8 && NU == 57 && ND == 57) ? 3 : 1;
 }
 
 template <typename Real, int NU, int ND, int WPS>
