@@ -127,6 +127,7 @@ __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kI
     for (int k = 0; k < kStatCount; ++k) stat[k] = ws[k];
     double sigma = st.sigma[w];
     int since = st.since[w];
+    if (s.reinv_k > 0 && since >= s.reinv_k) since = 0;  // re-inverted after the last sweep
     int wflags = st.flags[w];
     double ld0 = st.logdet[2 * w + 0], ld1 = st.logdet[2 * w + 1];
     double drift = st.drift[w];

@@ -2,8 +2,10 @@
 // matrix cores, for determinants of up to 160 electrons (the multi-wavefront
 // register kernels' 33 to 147 per spin, and the generic kernel's sizes).
 //
-// One 256-thread workgroup per due walker (accepted moves since its last
-// re-inversion >= K); walkers not due leave at once.  Per determinant of order
+// One 256-thread workgroup per due walker and determinant up to 80 electrons
+// per spin (E in LDS; the walker's next sweep clears its move count), one per
+// due walker above (accepted moves since its last re-inversion >= K); walkers
+// not due leave at once.  Per determinant of order
 // n, padded to P = 16 NT with an identity block:
 //   E = D^T in FP64 (E[m][i] = phi_m(r_i), axis phasors of the FP64
 //       positions), Y = the FP32 inverse invT (= E^-1 up to its drift);
@@ -139,7 +141,7 @@ __device__ void ns_build(const DevSys &s, const double *gp, int d, double *E, in
 }
 
 template <int NT>
-__device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &st, int w) {
+__device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &st, int w, int d_only) {
     using S = NSShared<NT>;
     using T = NSTiles<NT>;
     constexpr int P = S::P, PE = S::PE, PF = S::PF;
@@ -151,7 +153,7 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
     float *inv = reinterpret_cast<float *>(st.inv) + (size_t)w * s.ninv;
     double drift = 0.0, ld[2] = {st.logdet[2 * w + 0], st.logdet[2 * w + 1]};
     bool singular = false;
-    for (int d = 0; d < s.ndet; ++d) {
+    for (int d = d_only; d <= d_only; ++d) {
         const int n = s.n[d];
         float *Yg = inv + s.inv_off[d];
         ns_build<NT>(s, gp, d, sh.E, PE, true);
@@ -243,16 +245,15 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
         }
         __syncthreads();
     }
+    // One workgroup per determinant: the walker's next sweep clears its move
+    // count; the drift is an atomic maximum (a non-negative double, or +inf /
+    // NaN, orders as its bit pattern).
     if (t == 0) {
-        st.since[w] = 0;
-        st.stats[(size_t)w * kStatCount + 15] += 1.0;
-        if (singular) {
-            st.flags[w] |= 1;
-        } else {
-            st.logdet[2 * w + 0] = ld[0];
-            st.logdet[2 * w + 1] = ld[1];
-        }
-        if (drift > st.drift[w] || drift != drift) st.drift[w] = drift;
+        if (d_only == 0) st.stats[(size_t)w * kStatCount + 15] += 1.0;
+        if (singular) atomicOr(&st.flags[w], kFlagSingular);
+        else st.logdet[2 * w + d_only] = ld[d_only];
+        atomicMax(reinterpret_cast<unsigned long long *>(&st.drift[w]),
+                  (unsigned long long)__double_as_longlong(drift));
     }
 }
 
@@ -444,25 +445,29 @@ void launch_nsg(const DevSys &d, const DevState &st, const BatchItem *items, int
         hipLaunchKernelGGL((k_reinv_nsg<NT>), dim3(grid), dim3(kNSThreads), 0, stream, d, st);
 }
 
+// Workgroup g re-inverts determinant g % ndet of walker g / ndet.
 template <int NT>
 __global__ __launch_bounds__(kNSThreads) void k_reinv_ns(DevSys s, DevState st) {
-    reinv_ns_body<NT>(s, st, blockIdx.x);
+    reinv_ns_body<NT>(s, st, blockIdx.x / s.ndet, blockIdx.x % s.ndet);
 }
 
-// Batch form over a group of systems of one size.
+// Batch form over a group of systems of one size (and determinant count).
 template <int NT>
 __global__ __launch_bounds__(kNSThreads) void k_reinv_ns_b(const BatchItem *__restrict__ items, int n_items) {
-    const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
-    reinv_ns_body<NT>(it.s, it.st, blockIdx.x - it.walker_begin);
+    const int ndet = items[0].s.ndet;
+    const int g = blockIdx.x / ndet;
+    const BatchItem &it = items[batch_find(items, n_items, g)];
+    reinv_ns_body<NT>(it.s, it.st, g - it.walker_begin, blockIdx.x % ndet);
 }
 
 template <int NT>
 void launch_ns(const DevSys &d, const DevState &st, const BatchItem *items, int n_items, unsigned grid,
                hipStream_t stream) {
+    const unsigned g = grid * (unsigned)d.ndet;
     if (items)
-        hipLaunchKernelGGL((k_reinv_ns_b<NT>), dim3(grid), dim3(kNSThreads), 0, stream, items, n_items);
+        hipLaunchKernelGGL((k_reinv_ns_b<NT>), dim3(g), dim3(kNSThreads), 0, stream, items, n_items);
     else
-        hipLaunchKernelGGL((k_reinv_ns<NT>), dim3(grid), dim3(kNSThreads), 0, stream, d, st);
+        hipLaunchKernelGGL((k_reinv_ns<NT>), dim3(g), dim3(kNSThreads), 0, stream, d, st);
 }
 
 void launch_any(const DevSys &d, const DevState &st, const BatchItem *items, int n_items, unsigned grid,

@@ -548,7 +548,8 @@ __device__ __forceinline__ void sweep_rrw_body(const DevSys &s, const DevState &
         sh.stat[6] += (double)acc;
         sh.stat[7] += (double)N;
         st.sigma[w] = sh.misc[0];
-        st.since[w] += acc;
+        // (a count at K was re-inverted after the last sweep)
+        st.since[w] = (s.reinv_k > 0 && st.since[w] >= s.reinv_k ? 0 : st.since[w]) + acc;
         st.flags[w] = wflags | kFlagJrowValid;
     }
     __syncthreads();
