@@ -158,6 +158,54 @@ int vmc_ensemble_get_positions(vmc_ensemble *e, double *pos, void *stream);
 int vmc_ensemble_sweep(vmc_ensemble *e, int32_t n_sweeps, int32_t flags, int32_t block_size,
                        void *stream);
 
+/* Values recorded per move by vmc_ensemble_sweep_given() and per walker by
+ * vmc_ensemble_measure(). */
+enum {
+    VMC_MV_RATIO = 0,    /* determinant ratio R (slater_plane_wave.cpp:351-365) */
+    VMC_MV_DELTA_J = 1,  /* Jastrow change dJ (jastrow_pade.cpp:140-200) */
+    VMC_MV_ACCEPTED = 2, /* 1 if u < R^2 exp(2 dJ) (simulation.cpp:125-131), else 0 */
+    VMC_MV_COUNT = 3
+};
+enum {
+    VMC_PART_KINETIC = 0, /* -1/2 sum(lap + |grad|^2) of ln psi (energy_tracking.cpp:253-278) */
+    VMC_PART_VREAL = 1,   /* real-space Ewald (energy_tracking.cpp:93-144) */
+    VMC_PART_VRECIP = 2,  /* reciprocal Ewald (energy_tracking.cpp:72-91) */
+    VMC_PART_ELOCAL = 3,  /* kinetic + potential + Ewald constants (energy_tracking.cpp:367-382) */
+    VMC_PART_LOGDET0 = 4, /* log|det D0| (slater_plane_wave.cpp:202-318): FP64 at the last
+                             (re)build, then tracked through the accepted moves' ratios */
+    VMC_PART_LOGDET1 = 5, /* log|det D1| (0 when polarised) */
+    VMC_PART_JASTROW = 6, /* J = sum_{i<j} u(r_ij) (jastrow_pade.cpp:7-55) */
+    VMC_PART_COUNT = 7
+};
+
+/* One sweep of every walker on the kernel the ensemble's sweeps use, with
+ * the caller's moves in place of the Philox draws: electron p of walker w is
+ * proposed at proposals[w][p][0..2] (in [0, L)) and accepted iff
+ * uniforms[w][p] < R^2 exp(2 dJ) -- the reference's
+ * Simulation::metropolis_step (simulation.cpp:80-156) with its proposal and
+ * uniform supplied.  The electron order is vmc_ensemble_sweep's; each
+ * move's ratio, Jastrow change and decision go to trace[w][p][VMC_MV_*].
+ * No measurement or step adaptation; the Philox stream position does not
+ * advance; the accept counters count the moves and the walkers that came due
+ * are re-inverted as after a sweep.  A uniform of 0 accepts every move with
+ * R != 0.  For parity checks of the timed kernels against a reference
+ * implementation on the same moves. */
+int vmc_ensemble_sweep_given(vmc_ensemble *e, const double *proposals, const double *uniforms, double *trace,
+                             void *stream);
+
+/* The local energy and its parts, log|det| and J of every walker at its
+ * current positions, evaluated by the local-energy kernel the ensemble's
+ * measured sweeps use (WaveFunction::evaluate_derivatives +
+ * EnergyTracker::eval_total_energy, wavefunction.cpp:12-58,
+ * energy_tracking.cpp:367-382).  parts [W][VMC_PART_COUNT]; the statistics
+ * are left alone. */
+int vmc_ensemble_measure(vmc_ensemble *e, double *parts, void *stream);
+
+/* The walkers' transposed inverses as doubles, [W][n_up^2 + n_dn^2]
+ * (invT[k][j] = (D^-1)[j][k], slater_plane_wave.cpp:312-314; spin up
+ * first); mixed-precision inverses are widened from FP32. */
+int vmc_ensemble_get_inverse(vmc_ensemble *e, double *inv, void *stream);
+
 /* Choose the sweep kernel: VMC_KERNEL_GENERIC keeps the inverses in LDS (or
  * HBM) for any N; VMC_KERNEL_REGISTER keeps each inverse row in a lane's
  * registers (at most 32 electrons per spin, equal spins or polarised).
@@ -207,8 +255,10 @@ int vmc_ensemble_info(const vmc_ensemble *e, int64_t *n_walkers, uint64_t *sweep
  * builds D and its FP64 inverse, evaluates every energy term, proposes the
  * move of electron moved[c] to newpos[c] (ratio, delta J), applies it
  * unconditionally (Sherman-Morrison, incremental S_G and V_real) and
- * re-evaluates.  Uses the same device routines as the sweep, in the
- * system's precision.  Host buffers:
+ * re-evaluates.  Uses the generic kernel's device routines: the per-move
+ * values (ratio, delta J, inverses, incremental Ewald updates) in the
+ * system's precision, log|det|, J, the derivatives, S_G and the energies in
+ * FP64 (as the mixed-precision sweeps measure).  Host buffers:
  *   pos [C][N][3], moved [C], newpos [C][3]        (inputs)
  *   scalars [C][VMC_FZ_COUNT]                       (required)
  *   grad_lap [C][N][4]   grad x,y,z and lap of ln psi before the move (optional)
@@ -283,8 +333,9 @@ int vmc_ensemble_set_walker_threads(vmc_ensemble *e, int32_t threads);
  * tables), the admission measure of the request server: per walker the
  * positions, inverses, FP64 Gauss-Jordan scratch, statistics, step state,
  * drift, Jastrow row sums and the re-inversion list entry, and for mixed
- * ensembles above 80 electrons per spin that re-invert the Newton-Schulz
- * residual.  No reference counterpart. */
+ * ensembles the FP64 inverses a measured sweep re-inverts into and, above
+ * 80 electrons per spin, the Newton-Schulz residual.  No reference
+ * counterpart. */
 int vmc_ensemble_footprint(const vmc_system *sys, int64_t n_walkers, int64_t *bytes);
 
 /* --------------------------------------------------------------- autotuner
@@ -326,19 +377,21 @@ typedef struct vmc_request {
     int64_t n_walkers;
     int32_t warmup_sweeps;    /* adaptive (simulation.cpp:159-196) */
     int32_t measure_sweeps;   /* measured, E_L per sweep */
-    int32_t block_size;       /* measured sweeps per blocking block */
+    int32_t block_size;       /* measured sweeps per blocking block; below 1 the first
+                                 measured sweep fails and ends the request */
     uint64_t seed;
     double step0;
 } vmc_request;
 
 typedef struct vmc_result {
     int64_t id;
-    int32_t status;           /* VMC_OK or the error that ended the request */
+    int32_t status;           /* VMC_OK or the error that ended the request (its
+                                 admission or one of its sweeps) */
     int64_t walkers;
     double mean_energy;       /* walker aggregate as src/main.cpp:79-104 */
     double standard_error;    /* < 0 when no walker completed two blocks */
     double acceptance_rate;
-    double moves;             /* electron moves performed */
+    double moves;             /* electron moves of the sweeps that ran */
     double queued_s;          /* submit -> admission */
     double latency_s;         /* submit -> result */
 } vmc_result;

@@ -68,9 +68,9 @@ def test_footprint_counts_every_per_walker_buffer(vmc_nodev):
     """vmc_ensemble_footprint (the server's admission measure) counts what an
     ensemble allocates per walker: positions, the working-precision inverses,
     the FP64 Gauss-Jordan scratch, statistics, step state, drift, the Jastrow
-    row sums, the re-inversion list entry and -- for mixed ensembles above 80
-    electrons per spin that re-invert -- the Newton-Schulz residual (padded
-    order^2 floats)."""
+    row sums, the re-inversion list entry and -- for mixed ensembles -- the
+    FP64 inverses of a measured sweep and, above 80 electrons per spin, the
+    Newton-Schulz residual (padded order^2 floats)."""
     V = vmc_nodev
 
     def per_walker(s):
@@ -78,10 +78,12 @@ def test_footprint_counts_every_per_walker_buffer(vmc_nodev):
 
     N, n = 54, 27
     s = V.System.heg(N, 2.0, 1.0, precision=V.PREC_MIXED, reinvert_interval=64)
-    assert per_walker(s) == 3 * N * 8 + 2 * n * n * 4 + n * n * 8 + 16 * 8 + 8 + 16 + 12 + 8 + 8 * N
+    assert per_walker(s) == (3 * N * 8 + 2 * n * n * 4 + n * n * 8 + 16 * 8 + 8 + 16 + 12 + 8 + 8 * N +
+                             2 * n * n * 8)
     s64 = V.System.heg(N, 2.0, 1.0, precision=V.PREC_FP64)
-    assert per_walker(s64) - per_walker(s) == 2 * n * n * 4
+    assert per_walker(s64) - per_walker(s) == 2 * n * n * 4 - 2 * n * n * 8
     for N, p in ((162, 96), (246, 128), (294, 160)):
         k = V.System.heg(N, 2.0, 1.0, precision=V.PREC_MIXED, reinvert_interval=64)
-        k0 = V.System.heg(N, 2.0, 1.0, precision=V.PREC_MIXED, reinvert_interval=0)
-        assert per_walker(k) - per_walker(k0) == p * p * 4
+        k64 = V.System.heg(N, 2.0, 1.0, precision=V.PREC_FP64, reinvert_interval=64)
+        n = N // 2
+        assert per_walker(k) - per_walker(k64) == p * p * 4 + 2 * n * n * 8 - 2 * n * n * 4

@@ -98,3 +98,19 @@ def test_server_admission_by_footprint(vmc):
     with pytest.raises(vmc.VmcError):
         srv.submit(s, 1024, 2, 6, 9)  # larger than the whole budget
     srv.close()
+
+
+def test_server_reports_a_failed_sweep(vmc):
+    """A request whose measured sweeps fail (block size 0 is rejected by the
+    sweep) reports that error and the sweeps it did, not a result; a good
+    request served beside it is unaffected."""
+    s = vmc.System.heg(14, 1.0, 1.0)
+    srv = vmc.Server(1 << 34, max_active=4)
+    bad = srv.submit(s, 32, 3, 5, 21, block_size=0)
+    good = srv.submit(s, 32, 3, 5, 22, block_size=1)
+    res = collect(srv, 2)
+    srv.close()
+    assert res[bad]["status"] == vmc.ERR_INVALID
+    assert res[bad]["moves"] == 32 * 14 * 3  # the warmup sweeps only
+    assert res[good]["status"] == 0
+    assert res[good]["moves"] == 32 * 14 * 8

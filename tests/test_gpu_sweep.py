@@ -14,6 +14,11 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
+# The mixed-precision local energy against the FP64 oracle at the same
+# configuration: within 1e-6 of max(|E_L|, N x 1 mHa) (it is evaluated in FP64
+# from an inverse refined to FP64 accuracy).
+MIXED_E_L = 1e-6
+
 
 def oracle_system(O, s):
     return O.System(s.n_up, s.n_dn, s.box_length, s.a_same, s.a_anti, s.b, s.alpha,
@@ -199,14 +204,12 @@ def test_mixed_reinversion_bounds_the_drift(vmc, oracle):
     for w in range(0, W, 8):
         wk = oracle.Walker(osys, pos[w])
         e, kin, _ = wk.local_energy()
-        scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
-        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale, (w, st[w, vmc.STAT["LAST_E"]], e)
+        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= MIXED_E_L * max(abs(e), 1e-3 * s.N), (w, st[w, vmc.STAT["LAST_E"]], e)
 
 
 def test_kernel_timing_counts_every_launch(vmc):
-    """The per-kernel event timer sees one sweep, one re-inversion and one
-    local-energy launch per measured sweep of the register path, and
-    forgets them once read."""
+    """The per-kernel event timer sees each sweep, re-inversion and
+    local-energy launch of the register path, and forgets them once read."""
     s = vmc.System.heg(54, 2.0, 1.0, precision=1, reinvert_interval=64)
     ens = vmc.Ensemble(s, 32, 9, step0=0.6)
     ens.set_kernel(2)
@@ -215,7 +218,9 @@ def test_kernel_timing_counts_every_launch(vmc):
     ens.sweep(3, measure=True)
     ens.sweep(2, measure=False)
     t = ens.timing()
-    assert t["sweep"][1] == 5 and t["reinvert"][1] == 5 and t["measure"][1] == 3
+    # (a mixed measured sweep's local energy refreshes every inverse itself:
+    # only the two unmeasured sweeps launch the re-inversion)
+    assert t["sweep"][1] == 5 and t["reinvert"][1] == 2 and t["measure"][1] == 3
     assert all(v[0] > 0 for v in t.values())
     assert all(v[1] == 0 for v in ens.timing().values())
 
@@ -240,8 +245,7 @@ def test_n114_reinversion_interval_drift_and_energy(vmc, oracle, K):
     for w in range(0, W, 4):
         wk = oracle.Walker(osys, pos[w])
         e, kin, _ = wk.local_energy()
-        scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
-        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale, (w, st[w, vmc.STAT["LAST_E"]], e)
+        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= MIXED_E_L * max(abs(e), 1e-3 * s.N), (w, st[w, vmc.STAT["LAST_E"]], e)
 
 
 @pytest.mark.parametrize("threads", [128, 256])
@@ -283,8 +287,7 @@ def test_generic_multi_wavefront_mixed_with_reinversion(vmc, oracle):
     for w in range(4):
         wk = oracle.Walker(osys, pos[w])
         e, kin, _ = wk.local_energy()
-        scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
-        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale
+        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= MIXED_E_L * max(abs(e), 1e-3 * s.N)
 
 
 RRW_SYSTEMS = [
@@ -342,8 +345,7 @@ def test_multiwave_register_kernel_mixed_reinversion(vmc, oracle):
     for w in range(0, W, 8):
         wk = oracle.Walker(osys, pos[w])
         e, kin, _ = wk.local_energy()
-        scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
-        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale
+        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= MIXED_E_L * max(abs(e), 1e-3 * s.N)
 
 
 @pytest.mark.parametrize("N", [246, 294])
@@ -366,8 +368,7 @@ def test_multiwave_register_kernel_large_spins_mixed(vmc, oracle, N):
     for w in range(W):
         wk = oracle.Walker(osys, pos[w])
         e, kin, _ = wk.local_energy()
-        scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
-        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-5 * scale, (w, st[w, vmc.STAT["LAST_E"]], e)
+        assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= MIXED_E_L * max(abs(e), 1e-3 * s.N), (w, st[w, vmc.STAT["LAST_E"]], e)
 
 
 NS_SYSTEMS = [
@@ -386,14 +387,13 @@ NS_SYSTEMS = [
 @pytest.mark.parametrize("name,factory,kernel", NS_SYSTEMS, ids=[s[0] for s in NS_SYSTEMS])
 def test_matrix_core_reinversion_reaches_fp32_rounding(vmc, oracle, name, factory, kernel):
     """FP32 inverses of 33 to 147 electrons per spin are re-inverted by
-    Newton-Schulz steps on the FP64 matrix cores (operands in LDS up to 96 per
-    spin, E and the residual through the caches above).  With K = 16 every walker
-    re-inverts after every sweep, and the local energy is measured right
-    after that, so its error against the FP64 oracle shows the quality of the
-    re-inverted inverse: within 1e-6 of the sum of the magnitudes of its
-    parts (the FP32 working set's own rounding is about 1e-7 there), ten
-    times inside the mixed-precision budget.  A wrong or unconverged inverse
-    (drift 1e-4 or more) misses this bound."""
+    Newton-Schulz steps on the FP64 matrix cores (operands in LDS up to 80 per
+    spin, E and the residual through the caches above).  Every measured sweep
+    re-inverts every walker, its last step's FP64 result feeding the FP64
+    local energy, so the local energy's error against the FP64 oracle shows
+    the quality of the re-inverted inverse: within 1e-6 of max(|E_L|,
+    N x 1 mHa).  A wrong or unconverged inverse (drift 1e-4 or more) misses
+    this bound."""
     s = factory(vmc)
     W = 8
     ens = vmc.Ensemble(s, W, 4242, step0=0.3 * s.box_length / 6)
@@ -411,10 +411,9 @@ def test_matrix_core_reinversion_reaches_fp32_rounding(vmc, oracle, name, factor
     for w in range(W):
         wk = oracle.Walker(osys, pos[w])
         e, kin, _ = wk.local_energy()
-        scale = abs(kin) + abs(wk.vreal()) + abs(wk.vrecip()) + 1e-3 * s.N
-        worst = max(worst, abs(st[w, vmc.STAT["LAST_E"]] - e) / scale)
-    print(f"{name}: E_L error / parts max {worst:.2e}, drift max {d.max():.2e}")
-    assert worst <= 1e-6, worst
+        worst = max(worst, abs(st[w, vmc.STAT["LAST_E"]] - e) / max(abs(e), 1e-3 * s.N))
+    print(f"{name}: E_L relative error max {worst:.2e}, drift max {d.max():.2e}")
+    assert worst <= MIXED_E_L, worst
 
 
 @pytest.mark.parametrize("name,factory", [("heg_N54_rs2", lambda V: V.System.heg(54, 2.0, 1.0)),

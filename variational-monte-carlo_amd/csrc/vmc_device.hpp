@@ -59,6 +59,43 @@ struct DevState {
     double *jrow;     // [W][N] Jastrow row sums U_i kept by the register sweeps between sweeps
     unsigned *due;    // [2] walkers listed in due_list, finished workgroups of the re-inversion
     int *due_list;    // [W] walkers a register sweep found due for re-inversion
+    // [W][ninv] FP64 inverses for the FP64 local energy of a mixed-precision
+    // measured sweep (multi-wavefront and generic kernels).  Set only in the
+    // state handed to that sweep's re-inversion, which then re-inverts every
+    // walker and writes its FP64 result here; null otherwise.
+    double *inv64;
+};
+
+// Caller-supplied moves of one sweep (vmc_ensemble_sweep_given): electron e
+// of walker w is proposed at prop[(w N + e) 3 + c] and accepted when
+// u[w N + e] < R^2 e^{2 dJ}; trace[(w N + e) kTraceCount + k] receives the
+// move's ratio, Jastrow change and decision.  All null in the Philox sweeps.
+struct SweepGiven {
+    const double *prop;
+    const double *u;
+    double *trace;
+};
+constexpr int kTraceCount = 3;  // VMC_MV_COUNT
+
+// Per-walker parts of the local energy written by a measurement launched
+// for vmc_ensemble_measure (statistics untouched): kinetic, V_real, V_recip,
+// E_L, log|det| of each determinant, J.
+constexpr int kPartCount = 7;  // VMC_PART_COUNT
+
+// log|det| of the two determinants tracked through the accepted moves'
+// ratios: the product of |R| kept as mantissa and binary exponent (no log
+// on the move's path), folded into log|det| at the end of the sweep.
+struct LogDetTrack {
+    double m[2] = {1.0, 1.0};
+    int ex[2] = {0, 0};
+    __device__ void add(int h, double R) {
+        int e;
+        m[h] = frexp(m[h] * fabs(R), &e);
+        ex[h] += e;
+    }
+    __device__ double fold(int h, double ld) const {
+        return ld + log(m[h]) + (double)ex[h] * 0.69314718055994531;
+    }
 };
 
 // Internal sweep flag (beside VMC_SWEEP_*): the register sweep lists the

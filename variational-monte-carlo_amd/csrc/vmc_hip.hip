@@ -101,10 +101,12 @@ __global__ __launch_bounds__(kWave) void k_init(DevSys s, DevState st, int draw)
 // proposal (tag MOVE, counter {p, walker, sweep}), ratio and Jastrow change, Metropolis test with a
 // Philox uniform (tag ACCP), Sherman-Morrison on accept; after each sweep the
 // optional step adaptation and local-energy measurement.
+// With given.prop set, one sweep takes the caller's moves (absolute
+// proposals and accept uniforms) and records each move in given.trace.
 template <typename Real>
 __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kInvLds,
                              int n_sweeps, unsigned long long sweep0, int flags, int block_size,
-                             unsigned char *smem) {
+                             unsigned char *smem, SweepGiven given = SweepGiven{}) {
     const int lane = threadIdx.x;
     const int N = s.N;
     Real *inv_hbm = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
@@ -134,7 +136,14 @@ __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kI
 
     for (int sw = 0; sw < n_sweeps; ++sw) {
         const uint32_t sweep = (uint32_t)(sweep0 + (unsigned long long)sw);
-        for (int q = lane; q < N; q += (int)blockDim.x) {
+        for (int q = lane; q < N && given.prop; q += (int)blockDim.x) {
+            const size_t g = (size_t)w * N + q;
+            v.draw[q] = given.prop[3 * g];
+            v.draw[N + q] = given.prop[3 * g + 1];
+            v.draw[2 * N + q] = given.prop[3 * g + 2];
+            v.draw[3 * N + q] = given.u[g];
+        }
+        for (int q = lane; q < N && !given.prop; q += (int)blockDim.x) {
             const uint4 r = philox4x32_10(make_uint4((uint32_t)q, (uint32_t)w, sweep, kTagMove),
                                           s.key0, s.key1);
             const uint4 ra = philox4x32_10(make_uint4((uint32_t)q, (uint32_t)w, sweep, kTagAccept),
@@ -163,15 +172,28 @@ __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kI
         int acc = 0;
         for (int t = 0; t < N; ++t) {
             const int p = sweep_visit(s.n_up, s.n_dn, t);
-            double x = v.px[p] + sigma * v.draw[p];
-            double y = v.py[p] + sigma * v.draw[N + p];
-            double z = v.pz[p] + sigma * v.draw[2 * N + p];
-            x -= s.L * floor(x * s.inv_L);
-            y -= s.L * floor(y * s.inv_L);
-            z -= s.L * floor(z * s.inv_L);
+            double x, y, z;
+            if (given.prop) {
+                x = v.draw[p];
+                y = v.draw[N + p];
+                z = v.draw[2 * N + p];
+            } else {
+                x = v.px[p] + sigma * v.draw[p];
+                y = v.py[p] + sigma * v.draw[N + p];
+                z = v.pz[p] + sigma * v.draw[2 * N + p];
+                x -= s.L * floor(x * s.inv_L);
+                y -= s.L * floor(y * s.inv_L);
+                z -= s.L * floor(z * s.inv_L);
+            }
             const Proposal pr = propose<Real>(s, v, p, x, y, z, lane);
             const double prob = pr.ratio * pr.ratio * exp(2.0 * pr.dj);
             const int accept = wave_uniform((int)(v.draw[3 * N + p] < prob));
+            if (given.trace && threadIdx.x == 0) {
+                double *q = given.trace + ((size_t)w * N + p) * kTraceCount;
+                q[0] = pr.ratio;
+                q[1] = pr.dj;
+                q[2] = (double)accept;
+            }
             if (accept) {
                 accept_move<Real>(s, v, p, x, y, z, pr, lane);
                 ++acc;
@@ -189,27 +211,7 @@ __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kI
         if (flags & VMC_SWEEP_MEASURE) {
             jastrow_rows<Real>(s, v, lane);
             const Measured m = measure<Real>(s, v, lane, nullptr, nullptr);
-            const double e = m.e_local;
-            stat[0] += 1.0;
-            stat[1] += e;
-            stat[2] += e * e;
-            stat[3] += m.kinetic;
-            stat[4] += m.vreal + m.vrecip + s.e_const;
-            stat[5] = e;
-            stat[13] = m.kinetic;
-            stat[14] = m.vreal + m.vrecip + s.e_const;
-            // Blocking over measured sweeps (blocking_analysis.cpp:17-39).
-            stat[8] += e;
-            stat[9] += 1.0;
-            if (stat[9] >= (double)block_size) {
-                const double bm = stat[8] / (double)block_size;
-                stat[10] += 1.0;
-                const double delta = bm - stat[11];
-                stat[11] += delta / stat[10];
-                stat[12] += delta * (bm - stat[11]);
-                stat[8] = 0.0;
-                stat[9] = 0.0;
-            }
+            accumulate_measure(stat, m, s.e_const, block_size);
         }
     }
     if (kInvLds)
@@ -239,6 +241,69 @@ __global__ __launch_bounds__(256) void k_sweep(DevSys s, DevState st, int n_swee
                                                  int block_size) {
     extern __shared__ __align__(16) unsigned char smem[];
     sweep_walker<Real>(s, st, blockIdx.x, kInvLds, n_sweeps, sweep0, flags, block_size, smem);
+}
+
+// One sweep with the caller's moves, each recorded.
+template <typename Real, bool kInvLds>
+__global__ __launch_bounds__(256) void k_sweep_given(DevSys s, DevState st, SweepGiven given) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    sweep_walker<Real>(s, st, blockIdx.x, kInvLds, 1, 0, 0, 1, smem, given);
+}
+
+// The local energy of every walker by the generic routines (the measurement
+// of k_sweep): its parts into parts[w][kPartCount] (statistics untouched),
+// or with parts null into the statistics.  The FP64 local energy of a mixed
+// measured sweep runs this in FP64 with st.inv the sweep's FP64 inverses.
+template <typename Real>
+__device__ void measure_walker(const DevSys &s, const DevState &st, int w, bool kInvLds, double *parts,
+                               int block_size, unsigned char *smem) {
+    const int lane = threadIdx.x;
+    const int N = s.N;
+    Real *inv_hbm = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
+    double *A = st.scratch + (size_t)w * s.nmax * s.nmax;
+    WalkerView<Real> v = carve<Real>(smem, s, kInvLds, inv_hbm, A);
+    if (kInvLds)
+        for (int e = lane; e < s.ninv; e += (int)blockDim.x) v.inv[e] = inv_hbm[e];
+    const double *gp = st.pos + (size_t)w * 3 * N;
+    for (int i = lane; i < N; i += (int)blockDim.x) {
+        v.px[i] = gp[i];
+        v.py[i] = gp[N + i];
+        v.pz[i] = gp[2 * N + i];
+    }
+    wave_sync();
+    const Measured m = measure<Real>(s, v, lane, nullptr, nullptr);
+    if (!parts) {
+        if (lane == 0) accumulate_measure(st.stats + (size_t)w * kStatCount, m, s.e_const, block_size);
+        return;
+    }
+    jastrow_rows<Real>(s, v, lane);
+    double u = 0.0;
+    for (int i = lane; i < N; i += (int)blockDim.x) u += v.U[i];
+    const double J = 0.5 * block_sum(u);
+    if (lane == 0) {
+        double *q = parts + (size_t)w * kPartCount;
+        q[0] = m.kinetic;
+        q[1] = m.vreal;
+        q[2] = m.vrecip;
+        q[3] = m.e_local;
+        q[4] = st.logdet[2 * w + 0];
+        q[5] = s.ndet == 2 ? st.logdet[2 * w + 1] : 0.0;
+        q[6] = J;
+    }
+}
+
+template <typename Real, bool kInvLds>
+__global__ __launch_bounds__(256) void k_measure_parts(DevSys s, DevState st, double *parts, int block_size) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    measure_walker<Real>(s, st, blockIdx.x, kInvLds, parts, block_size, smem);
+}
+
+// The FP64 local energy of a mixed batch's generic group after its measured
+// sweep: one workgroup per walker of the group, st.inv the FP64 inverses.
+__global__ __launch_bounds__(256) void k_measure_batch64(const BatchItem *items, int n_items, int block_size) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
+    measure_walker<double>(it.s, it.st, blockIdx.x - it.walker_begin, false, nullptr, block_size, smem);
 }
 
 // One sweep of every walker of every system of a batch in one persistent
@@ -297,7 +362,8 @@ __host__ __device__ inline size_t reinv_lds_bytes(int nmax, bool a_in_lds) {
 template <typename Real>
 __device__ void reinv_walker(const DevSys &s, const DevState &st, int w, bool a_in_lds,
                              unsigned char *smem) {
-    if (s.reinv_k <= 0 || st.since[w] < s.reinv_k) return;
+    // st.inv64 set: every walker, with the FP64 result kept there.
+    if (!st.inv64 && (s.reinv_k <= 0 || st.since[w] < s.reinv_k)) return;
     const int t = threadIdx.x, nt = blockDim.x;
     size_t off = 0;
     double *A = a_in_lds ? reinterpret_cast<double *>(smem) : st.scratch + (size_t)w * s.nmax * s.nmax;
@@ -350,6 +416,8 @@ __device__ void reinv_walker(const DevSys &s, const DevState &st, int w, bool a_
             break;
         }
         for (int e = t; e < n * n; e += nt) Y[e] = (Real)A[e];
+        if (st.inv64)
+            for (int e = t; e < n * n; e += nt) st.inv64[(size_t)w * s.ninv + s.inv_off[d] + e] = A[e];
         ld[d] = logdet;
         __syncthreads();
     }
@@ -675,8 +743,13 @@ struct vmc_ensemble {
     int inv_placement = 0;  // generic kernel: 0 LDS when it fits, 1 LDS, 2 HBM
     int walker_threads = 64;  // generic kernel: threads (1, 2 or 4 wavefronts) per walker
     bool in_batch = false;  // owned by a vmc_batch: swept and destroyed through it
+    // Mixed precision: [W][ninv] FP64 inverses that a measured sweep of the
+    // multi-wavefront or generic kernels re-inverts into for its FP64 local
+    // energy (DevState::inv64 of that sweep's re-inversion).
+    double *inv64 = nullptr;
 
     ~vmc_ensemble() {
+        if (inv64) (void)hipFree(inv64);
         if (st.pos) (void)hipFree(st.pos);
         if (st.inv) (void)hipFree(st.inv);
         if (st.sigma) (void)hipFree(st.sigma);
@@ -728,8 +801,10 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
         const unsigned long long s0 = (unsigned long long)e->sweeps_done;
         const ReinvLaunch rl{static_cast<int>(sizeof(Real))};
         const bool ok = sizeof(Real) == 4
-                            ? rrw_launch_f32(d, e->st, grid, n_sweeps, s0, flags, block_size, stream, e->timer, rl)
-                            : rrw_launch_f64(d, e->st, grid, n_sweeps, s0, flags, block_size, stream, e->timer, rl);
+                            ? rrw_launch_f32(d, e->st, grid, n_sweeps, s0, flags, block_size, stream, e->timer, rl,
+                                             e->inv64)
+                            : rrw_launch_f64(d, e->st, grid, n_sweeps, s0, flags, block_size, stream, e->timer, rl,
+                                             nullptr);
         if (!ok) return fail(VMC_ERR_INVALID, "no register-resident instantiation for this system");
         VMC_HIP_TRY(hipGetLastError());
         return VMC_OK;
@@ -740,9 +815,17 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
     const size_t lds = inv_lds ? lds_in : lds_for(d, sizeof(Real), false);
     if (lds > kMaxLdsBytes) return fail(VMC_ERR_INVALID, "system too large for one walker per wavefront");
     // With periodic re-inversion, one sweep per launch, each followed by the
-    // FP64 re-inversion of the walkers that came due.
-    const bool reinv = d.reinv_k > 0;
+    // FP64 re-inversion of the walkers that came due.  In mixed precision a
+    // measured sweep re-inverts every walker with its FP64 result in inv64
+    // and takes the local energy from that in FP64 (k_measure_parts).
+    const bool exact = sizeof(Real) == 4 && (flags & VMC_SWEEP_MEASURE) && e->inv64;
+    const bool reinv = d.reinv_k > 0 || exact;
     const int per_launch = reinv ? 1 : n_sweeps;
+    const int sweep_flags = exact ? flags & ~VMC_SWEEP_MEASURE : flags;
+    DevState st_refresh = e->st, st64 = e->st;
+    st_refresh.inv64 = e->inv64;
+    st64.inv = e->inv64;
+    const size_t lds64 = lds_for(d, 8, false);
     for (int i = 0; i < n_sweeps; i += per_launch) {
         const int n = n_sweeps - i < per_launch ? n_sweeps - i : per_launch;
         const unsigned long long s0 = (unsigned long long)e->sweeps_done + (unsigned long long)i;
@@ -750,15 +833,21 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
         bool timed = e->timer.begin(stream, ev);
         if (inv_lds)
             hipLaunchKernelGGL((k_sweep<Real, true>), dim3(grid), dim3(e->walker_threads), lds, stream, d, e->st, n,
-                               s0, flags, block_size);
+                               s0, sweep_flags, block_size);
         else
             hipLaunchKernelGGL((k_sweep<Real, false>), dim3(grid), dim3(e->walker_threads), lds, stream, d, e->st, n,
-                               s0, flags, block_size);
+                               s0, sweep_flags, block_size);
         if (timed) e->timer.end(stream, ev, kTimeSweep);
         if (reinv) {
             timed = e->timer.begin(stream, ev);
-            reinv_generic_launch(d, e->st, grid, stream, static_cast<int>(sizeof(Real)));
+            reinv_generic_launch(d, exact ? st_refresh : e->st, grid, stream, static_cast<int>(sizeof(Real)));
             if (timed) e->timer.end(stream, ev, kTimeReinv);
+        }
+        if (exact) {
+            timed = e->timer.begin(stream, ev);
+            hipLaunchKernelGGL((k_measure_parts<double, false>), dim3(grid), dim3(e->walker_threads), lds64, stream,
+                               d, st64, static_cast<double *>(nullptr), block_size);
+            if (timed) e->timer.end(stream, ev, kTimeMeasure);
         }
     }
     VMC_HIP_TRY(hipGetLastError());
@@ -808,9 +897,8 @@ int vmc_ensemble_footprint(const vmc_system *sys, int64_t n_walkers, int64_t *by
     const int64_t rb = sys->precision == VMC_PREC_MIXED ? 4 : 8;
     const int64_t per_walker = 3 * N * 8 + ninv * rb + nmax * nmax * 8 + kStatCount * 8 +
                                8 /* sigma */ + 16 /* logdet */ + 12 /* since, flags, due list */ + 8 /* drift */ +
-                               (sys->reinvert_interval > 0
-                                    ? 4 * ns_resid_floats(static_cast<int>(nmax), static_cast<int>(rb))
-                                    : 0) +
+                               4 * ns_resid_floats(static_cast<int>(nmax), static_cast<int>(rb)) +
+                               (rb == 4 ? ninv * 8 : 0) /* FP64 inverses of a measured sweep */ +
                                8 * N /* Jastrow row sums */;
     const int64_t tables = 2 * nmax * 16 + static_cast<int64_t>(ng) * (16 + 8) + 8 /* due counters */;
     *bytes = n_walkers * per_walker + tables;
@@ -845,8 +933,14 @@ int vmc_ensemble_create(const vmc_system *sys, int64_t n_walkers, uint64_t seed,
         delete e;
         return rc;
     }
-    if (const int64_t nr = ns_resid_floats(d.nmax, e->tables.real_bytes); nr > 0 && d.reinv_k > 0 &&
-                                                                          (rc = dev_alloc(&st.resid, W * static_cast<size_t>(nr)))) {
+    // (every mixed ensemble re-inverts: periodically when K > 0, and on each
+    // measured sweep of the multi-wavefront and generic kernels)
+    if (const int64_t nr = ns_resid_floats(d.nmax, e->tables.real_bytes);
+        nr > 0 && (rc = dev_alloc(&st.resid, W * static_cast<size_t>(nr)))) {
+        delete e;
+        return rc;
+    }
+    if (e->tables.real_bytes == 4 && (rc = dev_alloc(&e->inv64, W * d.ninv))) {
         delete e;
         return rc;
     }
@@ -937,6 +1031,129 @@ int vmc_ensemble_sweep(vmc_ensemble *e, int32_t n_sweeps, int32_t flags, int32_t
                                        : launch_sweep<float>(e, n_sweeps, flags, block_size, s);
     if (rc) return rc;
     e->sweeps_done += static_cast<uint64_t>(n_sweeps);
+    return VMC_OK;
+}
+
+int vmc_ensemble_sweep_given(vmc_ensemble *e, const double *proposals, const double *uniforms, double *trace,
+                             void *stream) {
+    if (!e || !proposals || !uniforms || !trace) return fail(VMC_ERR_INVALID, "NULL argument");
+    if (!e->positions_ready) return fail(VMC_ERR_STATE, "positions not initialised");
+    if (e->in_batch) return fail(VMC_ERR_STATE, "ensemble belongs to a batch; sweep the batch");
+    const DevSys &d = e->tables.dev;
+    const size_t n = static_cast<size_t>(e->W) * static_cast<size_t>(d.N);
+    for (size_t i = 0; i < 3 * n; ++i)
+        if (!(proposals[i] >= 0.0 && proposals[i] < d.L)) return fail(VMC_ERR_INVALID, "proposal outside [0, L)");
+    for (size_t i = 0; i < n; ++i)
+        if (uniforms[i] != uniforms[i]) return fail(VMC_ERR_INVALID, "accept uniform is NaN");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double *d_prop = nullptr, *d_u = nullptr, *d_tr = nullptr;
+    struct Guard {
+        std::vector<void *> p;
+        ~Guard() { for (void *q : p) if (q) (void)hipFree(q); }
+    } guard;
+    int rc;
+    if ((rc = dev_alloc(&d_prop, 3 * n)) || (rc = dev_alloc(&d_u, n)) || (rc = dev_alloc(&d_tr, n * kTraceCount))) {
+        guard.p = {d_prop, d_u, d_tr};
+        return rc;
+    }
+    guard.p = {d_prop, d_u, d_tr};
+    VMC_HIP_TRY(hipMemcpyAsync(d_prop, proposals, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    VMC_HIP_TRY(hipMemcpyAsync(d_u, uniforms, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    const SweepGiven g{d_prop, d_u, d_tr};
+    const unsigned grid = static_cast<unsigned>(e->W);
+    const int rb = e->tables.real_bytes;
+    bool done = false;
+    if (e->kernel != VMC_KERNEL_GENERIC && rr_supported(d)) {
+        done = rb == 4 ? rr_given_f32(d, e->st, grid, g, s) : rr_given_f64(d, e->st, grid, g, s);
+    } else if (e->kernel != VMC_KERNEL_GENERIC && rrw_supported(d, rb)) {
+        const ReinvLaunch rl{rb};
+        done = rb == 4 ? rrw_given_f32(d, e->st, grid, g, s, rl) : rrw_given_f64(d, e->st, grid, g, s, rl);
+    }
+    if (!done) {
+        const size_t lds_in = lds_for(d, rb, true);
+        const bool inv_lds = lds_in <= kMaxLdsBytes && e->inv_placement != 2;
+        const size_t lds = inv_lds ? lds_in : lds_for(d, rb, false);
+        const dim3 block(static_cast<unsigned>(e->walker_threads));
+        if (rb == 8) {
+            if (inv_lds) hipLaunchKernelGGL((k_sweep_given<double, true>), dim3(grid), block, lds, s, d, e->st, g);
+            else hipLaunchKernelGGL((k_sweep_given<double, false>), dim3(grid), block, lds, s, d, e->st, g);
+        } else {
+            if (inv_lds) hipLaunchKernelGGL((k_sweep_given<float, true>), dim3(grid), block, lds, s, d, e->st, g);
+            else hipLaunchKernelGGL((k_sweep_given<float, false>), dim3(grid), block, lds, s, d, e->st, g);
+        }
+        if (d.reinv_k > 0) reinv_generic_launch(d, e->st, grid, s, rb);
+    }
+    VMC_HIP_TRY(hipGetLastError());
+    VMC_HIP_TRY(hipMemcpyAsync(trace, d_tr, sizeof(double) * n * kTraceCount, hipMemcpyDeviceToHost, s));
+    VMC_HIP_TRY(hipStreamSynchronize(s));
+    return VMC_OK;
+}
+
+int vmc_ensemble_measure(vmc_ensemble *e, double *parts, void *stream) {
+    if (!e || !parts) return fail(VMC_ERR_INVALID, "NULL argument");
+    if (!e->positions_ready) return fail(VMC_ERR_STATE, "positions not initialised");
+    const DevSys &d = e->tables.dev;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double *d_parts = nullptr;
+    const size_t n = static_cast<size_t>(e->W) * kPartCount;
+    int rc;
+    if ((rc = dev_alloc(&d_parts, n))) return rc;
+    struct Guard {
+        double *p;
+        ~Guard() { (void)hipFree(p); }
+    } guard{d_parts};
+    const unsigned grid = static_cast<unsigned>(e->W);
+    const int rb = e->tables.real_bytes;
+    bool done = false;
+    if (e->kernel != VMC_KERNEL_GENERIC && rr_supported(d))
+        done = rb == 4 ? rr_parts_f32(d, e->st, grid, d_parts, s) : rr_parts_f64(d, e->st, grid, d_parts, s);
+    else if (e->kernel != VMC_KERNEL_GENERIC && rrw_supported(d, rb))
+        done = rb == 4 ? rrw_parts_exact_f32(d, e->st, grid, d_parts, e->inv64, s, ReinvLaunch{rb})
+                       : rrw_parts_f64(d, e->st, grid, d_parts, s);
+    if (!done && rb == 4) {
+        // Mixed precision, generic kernel: FP64 re-inversion of every walker,
+        // then the FP64 local energy from it, as a measured sweep does.
+        DevState st_refresh = e->st, st64 = e->st;
+        st_refresh.inv64 = e->inv64;
+        st64.inv = e->inv64;
+        reinv_generic_launch(d, st_refresh, grid, s, rb);
+        hipLaunchKernelGGL((k_measure_parts<double, false>), dim3(grid), dim3(static_cast<unsigned>(e->walker_threads)),
+                           lds_for(d, 8, false), s, d, st64, d_parts, 1);
+        done = true;
+    }
+    if (!done) {
+        const size_t lds_in = lds_for(d, rb, true);
+        const bool inv_lds = lds_in <= kMaxLdsBytes && e->inv_placement != 2;
+        const size_t lds = inv_lds ? lds_in : lds_for(d, rb, false);
+        const dim3 block(static_cast<unsigned>(e->walker_threads));
+        if (rb == 8) {
+            if (inv_lds) hipLaunchKernelGGL((k_measure_parts<double, true>), dim3(grid), block, lds, s, d, e->st, d_parts, 1);
+            else hipLaunchKernelGGL((k_measure_parts<double, false>), dim3(grid), block, lds, s, d, e->st, d_parts, 1);
+        } else {
+            if (inv_lds) hipLaunchKernelGGL((k_measure_parts<float, true>), dim3(grid), block, lds, s, d, e->st, d_parts, 1);
+            else hipLaunchKernelGGL((k_measure_parts<float, false>), dim3(grid), block, lds, s, d, e->st, d_parts, 1);
+        }
+    }
+    VMC_HIP_TRY(hipGetLastError());
+    VMC_HIP_TRY(hipMemcpyAsync(parts, d_parts, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    VMC_HIP_TRY(hipStreamSynchronize(s));
+    return VMC_OK;
+}
+
+int vmc_ensemble_get_inverse(vmc_ensemble *e, double *inv, void *stream) {
+    if (!e || !inv) return fail(VMC_ERR_INVALID, "NULL argument");
+    const DevSys &d = e->tables.dev;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t n = static_cast<size_t>(e->W) * static_cast<size_t>(d.ninv);
+    if (e->tables.real_bytes == 8) {
+        VMC_HIP_TRY(hipMemcpyAsync(inv, e->st.inv, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        VMC_HIP_TRY(hipStreamSynchronize(s));
+        return VMC_OK;
+    }
+    std::vector<float> tmp(n);
+    VMC_HIP_TRY(hipMemcpyAsync(tmp.data(), e->st.inv, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    VMC_HIP_TRY(hipStreamSynchronize(s));
+    for (size_t i = 0; i < n; ++i) inv[i] = static_cast<double>(tmp[i]);
     return VMC_OK;
 }
 
@@ -1125,6 +1342,34 @@ int vmc_frozen_eval(const vmc_system *sys, int32_t n_configs, const double *pos,
     }
     VMC_HIP_TRY(hipGetLastError());
     VMC_HIP_TRY(hipMemcpyAsync(scalars, d_sc, sizeof(double) * C * VMC_FZ_COUNT, hipMemcpyDeviceToHost, s));
+    if (t.real_bytes == 4) {
+        // Mixed precision measures in FP64 (as the sweeps' local energy
+        // does): log|det|, J and the energies before and after the move, the
+        // derivatives and S_G come from the FP64 evaluation of the same
+        // configurations; ratio, delta J, the inverses and the incremental
+        // Ewald updates stay the FP32 per-move results.
+        double *d_sc64 = nullptr, *d_inv64 = nullptr;
+        if ((rc = dev_alloc(&d_sc64, C * VMC_FZ_COUNT)) || (rc = dev_alloc(&d_inv64, C * d.ninv))) {
+            guard.p.push_back(d_sc64);
+            return rc;
+        }
+        guard.p.push_back(d_sc64);
+        guard.p.push_back(d_inv64);
+        const bool lds64_in = lds_for(d, 8, true) <= kMaxLdsBytes;
+        const size_t lds64 = lds_for(d, 8, lds64_in);
+        if (lds64_in)
+            hipLaunchKernelGGL((k_frozen<double, true>), grid, block, lds64, s, d, n_configs, d_pos, d_mv, d_new, d_sc64, d_gl, nullptr, nullptr, d_sk, d_inv64, d_A, d_skc);
+        else
+            hipLaunchKernelGGL((k_frozen<double, false>), grid, block, lds64, s, d, n_configs, d_pos, d_mv, d_new, d_sc64, d_gl, nullptr, nullptr, d_sk, d_inv64, d_A, d_skc);
+        VMC_HIP_TRY(hipGetLastError());
+        std::vector<double> sc64(C * VMC_FZ_COUNT);
+        VMC_HIP_TRY(hipMemcpyAsync(sc64.data(), d_sc64, sizeof(double) * C * VMC_FZ_COUNT, hipMemcpyDeviceToHost, s));
+        VMC_HIP_TRY(hipStreamSynchronize(s));
+        for (size_t c = 0; c < C; ++c)
+            for (int k : {VMC_FZ_LOGDET0, VMC_FZ_LOGDET1, VMC_FZ_JASTROW, VMC_FZ_KINETIC, VMC_FZ_VREAL, VMC_FZ_VRECIP,
+                          VMC_FZ_ELOCAL, VMC_FZ_ELOCAL_AFTER, VMC_FZ_KINETIC_AFTER})
+                scalars[c * VMC_FZ_COUNT + k] = sc64[c * VMC_FZ_COUNT + k];
+    }
     if (grad_lap) VMC_HIP_TRY(hipMemcpyAsync(grad_lap, d_gl, sizeof(double) * C * 4 * d.N, hipMemcpyDeviceToHost, s));
     if (inv_before) VMC_HIP_TRY(hipMemcpyAsync(inv_before, d_ib, sizeof(double) * C * d.ninv, hipMemcpyDeviceToHost, s));
     if (inv_after) VMC_HIP_TRY(hipMemcpyAsync(inv_after, d_ia, sizeof(double) * C * d.ninv, hipMemcpyDeviceToHost, s));
@@ -1160,11 +1405,25 @@ struct BatchGroup {
         int n_items, total;
     };
     std::vector<Reinv> reinv_sets;
+    // Mixed precision, multi-wavefront and generic groups: the items with
+    // st.inv64 set (a measured sweep's re-inversion of every walker, FP64
+    // result kept) and with st.inv = inv64 (that sweep's FP64 local energy);
+    // generic: the re-inversion sets over every member, and the FP64 local
+    // energy's LDS.
+    BatchItem *items_refresh = nullptr, *items_m64 = nullptr;
+    std::vector<Reinv> refresh_sets;
+    size_t lds64 = 0;
 
     void release() {
         for (auto &r : reinv_sets)
             if (r.items) (void)hipFree(r.items);
         reinv_sets.clear();
+        for (auto &r : refresh_sets)
+            if (r.items) (void)hipFree(r.items);
+        refresh_sets.clear();
+        if (items_refresh) (void)hipFree(items_refresh);
+        if (items_m64) (void)hipFree(items_m64);
+        items_refresh = items_m64 = nullptr;
         if (items) (void)hipFree(items);
         if (queue) (void)hipFree(queue);
         if (lane) (void)hipStreamDestroy(lane);
@@ -1279,16 +1538,56 @@ int batch_build_groups(vmc_batch *b) {
         int rc;
         if ((rc = dev_alloc(&g.items, items.size()))) return rc;
         VMC_HIP_TRY(hipMemcpy(g.items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice));
+        if (b->real_bytes == 4 && g.family != 1) {
+            std::vector<BatchItem> ir = items, i64 = items;
+            for (size_t k = 0; k < items.size(); ++k) {
+                const vmc_ensemble *e = b->members[static_cast<size_t>(g.members[k])];
+                ir[k].st.inv64 = e->inv64;
+                i64[k].st.inv = e->inv64;
+                if (g.family == 0) g.lds64 = std::max(g.lds64, lds_for(items[k].s, 8, false));
+            }
+            if ((rc = dev_alloc(&g.items_refresh, ir.size())) || (rc = dev_alloc(&g.items_m64, i64.size()))) return rc;
+            VMC_HIP_TRY(hipMemcpy(g.items_refresh, ir.data(), sizeof(BatchItem) * ir.size(), hipMemcpyHostToDevice));
+            VMC_HIP_TRY(hipMemcpy(g.items_m64, i64.data(), sizeof(BatchItem) * i64.size(), hipMemcpyHostToDevice));
+            if (g.family == 0) {
+                std::vector<std::pair<int, int>> orders;
+                for (const auto &it : ir) {
+                    const std::pair<int, int> key{it.s.nmax, it.s.ndet};
+                    if (std::find(orders.begin(), orders.end(), key) == orders.end()) orders.push_back(key);
+                }
+                for (const auto &[nm, nd] : orders) {
+                    std::vector<BatchItem> sub;
+                    int64_t t = 0;
+                    for (const auto &it : ir) {
+                        if (it.s.nmax != nm || it.s.ndet != nd) continue;
+                        BatchItem x = it;
+                        const int w = it.walker_end - it.walker_begin;
+                        x.walker_begin = static_cast<int>(t);
+                        t += w;
+                        x.walker_end = static_cast<int>(t);
+                        sub.push_back(x);
+                    }
+                    BatchGroup::Reinv r{sub[0].s, nullptr, static_cast<int>(sub.size()), static_cast<int>(t)};
+                    if ((rc = dev_alloc(&r.items, sub.size()))) return rc;
+                    g.refresh_sets.push_back(r);
+                    VMC_HIP_TRY(hipMemcpy(r.items, sub.data(), sizeof(BatchItem) * sub.size(), hipMemcpyHostToDevice));
+                }
+            }
+        }
         if (g.family == 0) {
-            std::vector<int> orders;
-            for (const auto &it : items)
-                if (it.s.reinv_k > 0 && std::find(orders.begin(), orders.end(), it.s.nmax) == orders.end())
-                    orders.push_back(it.s.nmax);
-            for (int nm : orders) {
+            // One re-inversion set per (order, determinant count): a launch
+            // takes its determinant count from its set's first member.
+            std::vector<std::pair<int, int>> orders;
+            for (const auto &it : items) {
+                const std::pair<int, int> key{it.s.nmax, it.s.ndet};
+                if (it.s.reinv_k > 0 && std::find(orders.begin(), orders.end(), key) == orders.end())
+                    orders.push_back(key);
+            }
+            for (const auto &[nm, nd] : orders) {
                 std::vector<BatchItem> sub;
                 int64_t t = 0;
                 for (const auto &it : items) {
-                    if (it.s.reinv_k <= 0 || it.s.nmax != nm) continue;
+                    if (it.s.reinv_k <= 0 || it.s.nmax != nm || it.s.ndet != nd) continue;
                     BatchItem x = it;
                     const int w = it.walker_end - it.walker_begin;
                     x.walker_begin = static_cast<int>(t);
@@ -1328,12 +1627,19 @@ int batch_group_step(vmc_batch *b, const BatchGroup &g, int flags, int block_siz
     const int n = static_cast<int>(g.members.size());
     constexpr bool f32 = sizeof(Real) == 4;
     bool ok = true;
+    // Mixed precision: a measured sweep re-inverts every walker with an FP64
+    // result and takes the FP64 local energy from it (as alone).
+    const bool exact = f32 && (flags & VMC_SWEEP_MEASURE) && g.items_m64;
     if (g.family == 0) {
         hipLaunchKernelGGL((k_sweep_batch<Real>), dim3(static_cast<unsigned>(g.grid)), dim3(b->threads), g.lds,
-                           stream, g.items, n, g.total, g.queue, b->ctr, flags, block_size);
-        for (const auto &r : g.reinv_sets)
+                           stream, g.items, n, g.total, g.queue, b->ctr, exact ? flags & ~VMC_SWEEP_MEASURE : flags,
+                           block_size);
+        for (const auto &r : exact ? g.refresh_sets : g.reinv_sets)
             reinv_batch_launch(r.shape, r.items, r.n_items, static_cast<unsigned>(r.total), stream,
                                static_cast<int>(sizeof(Real)));
+        if (exact)
+            hipLaunchKernelGGL(k_measure_batch64, dim3(static_cast<unsigned>(g.total)), dim3(b->threads), g.lds64,
+                               stream, g.items_m64, n, block_size);
     } else if (g.family == 1) {
         ok = f32 ? rr_batch_step_f32(g.shape, g.items, n, static_cast<unsigned>(g.total), b->ctr, flags, block_size,
                                      g.reinv, stream)
@@ -1341,9 +1647,9 @@ int batch_group_step(vmc_batch *b, const BatchGroup &g, int flags, int block_siz
                                      g.reinv, stream);
     } else {
         ok = f32 ? rrw_batch_step_f32(g.shape, g.items, n, static_cast<unsigned>(g.total), b->ctr, flags,
-                                      block_size, g.reinv, stream)
+                                      block_size, g.reinv, stream, g.items_refresh, g.items_m64)
                  : rrw_batch_step_f64(g.shape, g.items, n, static_cast<unsigned>(g.total), b->ctr, flags,
-                                      block_size, g.reinv, stream);
+                                      block_size, g.reinv, stream, nullptr, nullptr);
     }
     if (!ok) return fail(VMC_ERR_INVALID, "no register-resident instantiation for a batch group");
     return VMC_OK;

@@ -37,6 +37,10 @@ constexpr int kNSSteps = 6;
 // FP32 inverse (max|I - E Y| of a correctly rounded inverse is a few 1e-7):
 // from d < 5e-4 one step reaches that floor.
 constexpr double kNSDone = 5e-4;
+// With an FP64 output (DevState::inv64: the FP64 local energy of a mixed
+// measured sweep) the steps go on until a step starts from d < 1e-6, whose
+// FP64 result Y + Y R has a residual of about n d^2 < 2e-10.
+constexpr double kNSExact = 1e-6;
 
 // R = I - Z takes E's storage once Z = E Y is in registers (E is rebuilt
 // for a further step, rarely needed), so a workgroup holds E and Y only:
@@ -145,7 +149,8 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
     using S = NSShared<NT>;
     using T = NSTiles<NT>;
     constexpr int P = S::P, PE = S::PE, PF = S::PF;
-    if (s.reinv_k <= 0 || st.since[w] < s.reinv_k) return;
+    const bool force = st.inv64 != nullptr;  // every walker, FP64 output
+    if (!force && (s.reinv_k <= 0 || st.since[w] < s.reinv_k)) return;
     __shared__ S sh;
     float *R = reinterpret_cast<float *>(sh.E);
     const int t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
@@ -156,6 +161,7 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
     for (int d = d_only; d <= d_only; ++d) {
         const int n = s.n[d];
         float *Yg = inv + s.inv_off[d];
+        double *Y64 = force ? st.inv64 + (size_t)w * s.ninv + s.inv_off[d] : nullptr;
         ns_build<NT>(s, gp, d, sh.E, PE, true);
         for (int e = t; e < P * P; e += kNSThreads) {
             const int i = e / P, j = e - (e / P) * P;
@@ -191,19 +197,24 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
             if (it == 0) d0 = dm;
             if (!(dm < 0.5)) break;
             if (dm < 1e-15) {
+                if (Y64)
+                    for (int e = t; e < n * n; e += kNSThreads) Y64[e] = (double)sh.Y[(e / n) * PF + e % n];
                 ok = true;
                 break;
             }
             ns_v4d y[T::kCount];
             ns_mm<NT, float, PF, float, PF>(sh.Y, R, y, wv, lane);
             float yn[T::kCount][4];
+            const bool exact = Y64 && dm < kNSExact;  // this step's FP64 result is the output
 #pragma unroll
             for (int i = 0; i < T::kCount; ++i) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int r = 16 * T::row(i, wv) + 4 * q + hl;
                     const int c = 16 * T::col(i, wv) + ll;
-                    yn[i][q] = T::valid(i, wv) ? (float)((double)sh.Y[r * PF + c] + y[i][q]) : 0.0f;
+                    const double v = (double)sh.Y[r * PF + c] + y[i][q];
+                    yn[i][q] = T::valid(i, wv) ? (float)v : 0.0f;
+                    if (exact && T::valid(i, wv) && r < n && c < n) Y64[r * n + c] = v;
                 }
             }
             __syncthreads();
@@ -214,7 +225,7 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
                 for (int q = 0; q < 4; ++q)
                     sh.Y[(16 * T::row(i, wv) + 4 * q + hl) * PF + 16 * T::col(i, wv) + ll] = yn[i][q];
             }
-            if (dm < kNSDone) {
+            if (Y64 ? exact : dm < kNSDone) {
                 ok = true;
                 __syncthreads();
                 break;
@@ -242,13 +253,17 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
             }
             ld[d] = l;
             for (int e = t; e < n * n; e += kNSThreads) Yg[e] = (float)sh.E[e];
+            if (Y64)
+                for (int e = t; e < n * n; e += kNSThreads) Y64[e] = sh.E[e];
         }
         __syncthreads();
     }
     // One workgroup per determinant: the walker's next sweep clears its move
-    // count; the drift is an atomic maximum (a non-negative double, or +inf /
-    // NaN, orders as its bit pattern).
+    // count (a forced re-inversion clears it here); the drift is an atomic
+    // maximum (a non-negative double, or +inf / NaN, orders as its bit
+    // pattern).
     if (t == 0) {
+        if (force) st.since[w] = 0;
         if (d_only == 0) st.stats[(size_t)w * kStatCount + 15] += 1.0;
         if (singular) atomicOr(&st.flags[w], kFlagSingular);
         else st.logdet[2 * w + d_only] = ld[d_only];
@@ -311,7 +326,8 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
     using S = NSGShared<NT>;
     constexpr int P = S::P, PF = S::PF, H = NT / 2;
     // (an ensemble that re-inverts at these sizes always holds the residual buffer)
-    if (s.reinv_k <= 0 || st.since[w] < s.reinv_k || !st.resid) return;
+    const bool force = st.inv64 != nullptr;  // every walker, FP64 output
+    if ((!force && (s.reinv_k <= 0 || st.since[w] < s.reinv_k)) || !st.resid) return;
     __shared__ S sh;
     const int t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
     const int r0 = (wv >> 1) * H, c0 = (wv & 1) * H;
@@ -324,6 +340,7 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
     for (int d = 0; d < s.ndet; ++d) {
         const int n = s.n[d];
         float *Yg = inv + s.inv_off[d];
+        double *Y64 = force ? st.inv64 + (size_t)w * s.ninv + s.inv_off[d] : nullptr;
         ns_build<NT>(s, gp, d, E, n, false);
         for (int e = t; e < P * P; e += kNSThreads) {
             const int i = e / P, j = e - (e / P) * P;
@@ -365,6 +382,8 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
             if (it == 0) d0 = dm;
             if (!(dm < 0.5)) break;
             if (dm < 1e-15) {
+                if (Y64)
+                    for (int e = t; e < n * n; e += kNSThreads) Y64[e] = (double)sh.Y[(e / n) * PF + e % n];
                 ok = true;
                 break;
             }
@@ -373,18 +392,23 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
             float yn[H * H][4];
             int yl = rl * PF + cl;
             asm volatile("" : "+v"(yl));
+            const bool exact = Y64 && dm < kNSExact;  // this step's FP64 result is the output
 #pragma unroll
             for (int i = 0; i < H * H; ++i)
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    yn[i][q] = (float)((double)sh.Y[yl + (16 * (i / H) + 4 * q) * PF + 16 * (i % H)] + y[i][q]);
+                for (int q = 0; q < 4; ++q) {
+                    const double v = (double)sh.Y[yl + (16 * (i / H) + 4 * q) * PF + 16 * (i % H)] + y[i][q];
+                    yn[i][q] = (float)v;
+                    const int r = rl + 16 * (i / H) + 4 * q, c = cl + 16 * (i % H);
+                    if (exact && r < n && c < n) Y64[r * n + c] = v;
+                }
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < H * H; ++i)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) sh.Y[yl + (16 * (i / H) + 4 * q) * PF + 16 * (i % H)] = yn[i][q];
             __syncthreads();
-            if (dm < kNSDone) {
+            if (Y64 ? exact : dm < kNSDone) {
                 ok = true;
                 break;
             }
@@ -409,6 +433,8 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
             }
             ld[d] = l;
             for (int e = t; e < n * n; e += kNSThreads) Yg[e] = (float)E[e];
+            if (Y64)
+                for (int e = t; e < n * n; e += kNSThreads) Y64[e] = E[e];
         }
         __syncthreads();
     }

@@ -11,4 +11,11 @@ bool rr_batch_step_f32(const DevSys &shape, const BatchItem *items, int n_items,
                        const unsigned long long *sweep_ctr, int flags, int bs, bool reinv, hipStream_t stream) {
     return rr_batch_step<float>(shape, items, n_items, grid, sweep_ctr, flags, bs, reinv, stream);
 }
+bool rr_given_f32(const DevSys &d, const DevState &st, unsigned grid, const SweepGiven &given,
+                  hipStream_t stream) {
+    return rr_given<float>(d, st, grid, given, stream);
+}
+bool rr_parts_f32(const DevSys &d, const DevState &st, unsigned grid, double *parts, hipStream_t stream) {
+    return rr_parts<float>(d, st, grid, parts, stream);
+}
 }  // namespace vmc

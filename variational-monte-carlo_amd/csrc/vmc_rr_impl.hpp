@@ -10,13 +10,16 @@ namespace vmc {
 
 // One launch of the sweep kernel per sweep, each followed by the FP64
 // re-inversion of the walkers that reached K accepted moves (when K > 0) and,
-// with MEASURE, the local energy.
+// with MEASURE, the local energy.  In mixed precision the local energy is
+// the FP64 one of k_measure_rr_mixed, which refines and refreshes every
+// walker's inverse itself: a measured sweep takes no separate re-inversion.
 template <typename Real, int NU, int ND, int GPL>
 void rr_launch_g(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,
                  unsigned long long s0, int flags, int bs, int variant, hipStream_t stream,
                  KernelTimer &timer) {
+    constexpr bool kMixed = sizeof(Real) == 4;
     const bool measure = flags & VMC_SWEEP_MEASURE;
-    const bool reinv = d.reinv_k > 0;
+    const bool reinv = d.reinv_k > 0 && !(kMixed && measure);
     // The sweep lists the due walkers; the re-inversion runs on at most
     // kListGrid workgroups (eight per CU) over that list.
     constexpr unsigned kListGrid = 2048;
@@ -43,8 +46,12 @@ void rr_launch_g(const DevSys &d, const DevState &st, unsigned grid, int n_sweep
         }
         if (measure) {
             timed = timer.begin(stream, ev);
-            hipLaunchKernelGGL((k_measure_rr<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0,
-                               stream, d, st, bs);
+            if constexpr (kMixed)
+                hipLaunchKernelGGL((k_measure_rr_mixed<NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream, d, st,
+                                   bs);
+            else
+                hipLaunchKernelGGL((k_measure_rr<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0,
+                                   stream, d, st, bs);
             if (timed) timer.end(stream, ev, kTimeMeasure);
         }
     }
@@ -76,18 +83,57 @@ bool rr_launch(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,
     });
 }
 
+// One sweep with the caller's moves (each recorded), followed like a sweep by
+// the re-inversion of the walkers it made due.
+template <typename Real>
+bool rr_given(const DevSys &d, const DevState &st, unsigned grid, const SweepGiven &given,
+              hipStream_t stream) {
+    return rr_dispatch<Real>(d, [&]<int NU, int ND, int GPL>() {
+        const bool reinv = d.reinv_k > 0;
+        constexpr unsigned kListGrid = 2048;
+        const unsigned items = grid * (ND > 0 ? 2 : 1);
+        hipLaunchKernelGGL((k_sweep_rr_given<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream, d, st,
+                           reinv ? kSweepListDue : 0, given);
+        if (reinv) {
+            hipLaunchKernelGGL((k_reinv_rr_list<Real, NU, ND, GPL>), dim3(items < kListGrid ? items : kListGrid),
+                               dim3(kWave), 0, stream, d, st);
+            hipLaunchKernelGGL(k_due_reset<Real>, dim3(1), dim3(1), 0, stream, st);
+        }
+    });
+}
+
+template <typename Real>
+bool rr_parts(const DevSys &d, const DevState &st, unsigned grid, double *parts, hipStream_t stream) {
+    return rr_dispatch<Real>(d, [&]<int NU, int ND, int GPL>() {
+        if constexpr (sizeof(Real) == 4)
+            hipLaunchKernelGGL((k_measure_rr_mixed_parts<NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream, d, st,
+                               parts);
+        else
+            hipLaunchKernelGGL((k_measure_rr_parts<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream, d, st,
+                               parts);
+    });
+}
+
 template <typename Real>
 bool rr_batch_step(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
                    const unsigned long long *ctr, int flags, int bs, bool reinv, hipStream_t stream) {
     return rr_dispatch<Real>(shape, [&]<int NU, int ND, int GPL>() {
+        constexpr bool kMixed = sizeof(Real) == 4;
+        const bool measure = flags & VMC_SWEEP_MEASURE;
         hipLaunchKernelGGL((k_sweep_rr_b<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream, items,
                            n_items, ctr, flags);
-        if (reinv)
+        // (the mixed local energy refreshes every inverse itself)
+        if (reinv && !(kMixed && measure))
             hipLaunchKernelGGL((k_reinv_rr_b<Real, NU, ND, GPL>), dim3(grid * (ND > 0 ? 2 : 1)), dim3(kWave), 0,
                                stream, items, n_items);
-        if (flags & VMC_SWEEP_MEASURE)
-            hipLaunchKernelGGL((k_measure_rr_b<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream,
-                               items, n_items, bs);
+        if (measure) {
+            if constexpr (kMixed)
+                hipLaunchKernelGGL((k_measure_rr_mixed_b<NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream,
+                                   items, n_items, bs);
+            else
+                hipLaunchKernelGGL((k_measure_rr_b<Real, NU, ND, GPL>), dim3(grid), dim3(kWave), 0, stream,
+                                   items, n_items, bs);
+        }
     });
 }
 

@@ -99,12 +99,22 @@ struct ReinvLaunch {
     }
 };
 
+// inv64 (mixed precision): the ensemble's FP64 inverse buffer; measured
+// sweeps then re-invert every walker into it and measure in FP64.
 bool rrw_launch_f32(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,
                     unsigned long long s0, int flags, int bs, hipStream_t stream, KernelTimer &timer,
-                    const ReinvLaunch &reinv);
+                    const ReinvLaunch &reinv, double *inv64);
 bool rrw_launch_f64(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,
                     unsigned long long s0, int flags, int bs, hipStream_t stream, KernelTimer &timer,
-                    const ReinvLaunch &reinv);
+                    const ReinvLaunch &reinv, double *inv64);
+// The FP64 multi-wavefront local energy from FP64 inverses (st64.inv): into
+// the statistics, or with `parts` into parts[w][kPartCount].
+bool rrw_measure64(const DevSys &d, const DevState &st64, unsigned grid, int bs, double *parts,
+                   hipStream_t stream);
+// Mixed-precision parts: FP64 re-inversion of every walker into inv64, then
+// rrw_measure64.
+bool rrw_parts_exact_f32(const DevSys &d, const DevState &st, unsigned grid, double *parts, double *inv64,
+                         hipStream_t stream, const ReinvLaunch &reinv);
 
 // One sweep of a batch group of register-resident systems of one size
 // (`shape` = any member's tables, `grid` = the group's walkers): the sweep,
@@ -115,10 +125,34 @@ bool rr_batch_step_f32(const DevSys &shape, const BatchItem *items, int n_items,
                        const unsigned long long *sweep_ctr, int flags, int bs, bool reinv, hipStream_t stream);
 bool rr_batch_step_f64(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
                        const unsigned long long *sweep_ctr, int flags, int bs, bool reinv, hipStream_t stream);
+// items_refresh / items_m64 (mixed precision): the group's items with
+// st.inv64 set and with st.inv = inv64; a measured sweep then re-inverts
+// every walker into inv64 and measures in FP64 (rrw_measure64_batch).
 bool rrw_batch_step_f32(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
-                        const unsigned long long *sweep_ctr, int flags, int bs, bool reinv, hipStream_t stream);
+                        const unsigned long long *sweep_ctr, int flags, int bs, bool reinv, hipStream_t stream,
+                        const BatchItem *items_refresh, const BatchItem *items_m64);
 bool rrw_batch_step_f64(const DevSys &shape, const BatchItem *items, int n_items, unsigned grid,
-                        const unsigned long long *sweep_ctr, int flags, int bs, bool reinv, hipStream_t stream);
+                        const unsigned long long *sweep_ctr, int flags, int bs, bool reinv, hipStream_t stream,
+                        const BatchItem *items_refresh, const BatchItem *items_m64);
+bool rrw_measure64_batch(const DevSys &shape, const BatchItem *items_m64, int n_items, unsigned grid, int bs,
+                         hipStream_t stream);
+
+// One sweep of the register kernels with the caller's moves, each recorded
+// (then the re-inversion, as after a sweep); and the per-walker parts of the
+// local energy by the measurement kernel, statistics untouched.  False when
+// the size has no instantiation.
+bool rr_given_f32(const DevSys &d, const DevState &st, unsigned grid, const SweepGiven &given,
+                  hipStream_t stream);
+bool rr_given_f64(const DevSys &d, const DevState &st, unsigned grid, const SweepGiven &given,
+                  hipStream_t stream);
+bool rr_parts_f32(const DevSys &d, const DevState &st, unsigned grid, double *parts, hipStream_t stream);
+bool rr_parts_f64(const DevSys &d, const DevState &st, unsigned grid, double *parts, hipStream_t stream);
+bool rrw_given_f32(const DevSys &d, const DevState &st, unsigned grid, const SweepGiven &given,
+                   hipStream_t stream, const ReinvLaunch &reinv);
+bool rrw_given_f64(const DevSys &d, const DevState &st, unsigned grid, const SweepGiven &given,
+                   hipStream_t stream, const ReinvLaunch &reinv);
+bool rrw_parts_f32(const DevSys &d, const DevState &st, unsigned grid, double *parts, hipStream_t stream);
+bool rrw_parts_f64(const DevSys &d, const DevState &st, unsigned grid, double *parts, hipStream_t stream);
 
 // Launches n_sweeps sweeps (and, with VMC_SWEEP_MEASURE, one measurement per
 // sweep).  variant 0: no occupancy bound; 1: at most 128 VGPRs.

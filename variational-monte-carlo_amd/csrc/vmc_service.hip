@@ -174,6 +174,8 @@ struct vmc_server {
         vmc_ensemble *e = nullptr;
         hipStream_t stream = nullptr;
         int warm_left = 0, meas_left = 0;
+        int64_t sweeps_done = 0;  // sweeps issued without error
+        int32_t status = VMC_OK;  // the first error a sweep returned
         std::deque<hipEvent_t> inflight;  // one event per issued turn
         double t_start = 0.0;
     };
@@ -202,13 +204,13 @@ void vmc_server::finish(Active &a, std::vector<vmc_result> &out) {
     vmc_result r{};
     r.id = a.p.id;
     r.walkers = a.p.req.n_walkers;
-    r.status = VMC_OK;
+    r.status = a.status;
     std::vector<double> st(static_cast<size_t>(a.p.req.n_walkers) * VMC_STAT_COUNT);
-    if (hipStreamSynchronize(a.stream) != hipSuccess) r.status = VMC_ERR_DEVICE;
-    else r.status = vmc_ensemble_read_stats(a.e, st.data(), a.stream);
+    if (hipStreamSynchronize(a.stream) != hipSuccess && r.status == VMC_OK) r.status = VMC_ERR_DEVICE;
+    if (r.status == VMC_OK) r.status = vmc_ensemble_read_stats(a.e, st.data(), a.stream);
     if (r.status == VMC_OK) aggregate(st, a.p.req.n_walkers, &r.mean_energy, &r.standard_error, &r.acceptance_rate);
     const int N = a.p.req.system.n_up + a.p.req.system.n_dn;
-    r.moves = static_cast<double>(a.p.req.n_walkers) * N * (a.p.req.warmup_sweeps + a.p.req.measure_sweeps);
+    r.moves = static_cast<double>(a.p.req.n_walkers) * N * static_cast<double>(a.sweeps_done);
     const double t = now_s();
     r.queued_s = a.t_start - a.p.t_submit;
     r.latency_s = t - a.p.t_submit;
@@ -282,20 +284,26 @@ void vmc_server::run() {
             }
             if (a.warm_left + a.meas_left == 0 || a.inflight.size() >= 2) continue;
             int rc;
+            int n_issued;
             if (a.warm_left > 0) {
                 const int n = std::min(a.warm_left, sweeps_per_turn);
+                n_issued = n;
                 rc = vmc_ensemble_sweep(a.e, n, VMC_SWEEP_ADAPT, 1, a.stream);
                 a.warm_left -= n;
                 if (!rc && a.warm_left == 0) rc = vmc_ensemble_reset_stats(a.e, a.stream);
             } else {
                 const int n = std::min(a.meas_left, sweeps_per_turn);
-                rc = vmc_ensemble_sweep(a.e, n, VMC_SWEEP_MEASURE, std::max(1, a.p.req.block_size), a.stream);
+                n_issued = n;
+                rc = vmc_ensemble_sweep(a.e, n, VMC_SWEEP_MEASURE, a.p.req.block_size, a.stream);
                 a.meas_left -= n;
             }
             if (rc) {
+                // The request ends here and reports the error, not a result.
+                a.status = rc;
                 a.warm_left = a.meas_left = 0;
                 continue;
             }
+            a.sweeps_done += n_issued;
             hipEvent_t ev = nullptr;
             if (!free_events.empty()) {
                 ev = free_events.back();

@@ -222,6 +222,11 @@ struct RRShared {
     Real row[32];                  // inverse row of the moved electron
     double gjcol[32];
     int perm[32];
+    // The run prop | pph | tab as one staging area (inverse blocks, the
+    // structure-factor tables of the measurement).
+    static constexpr size_t kStageBytes =
+        sizeof(double) * 4 * kWave + sizeof(Real) * 6 * kWave + sizeof(Real) * kRRChunk * 3 * (kRRMaxPow + 1) * 2;
+    __device__ void *stage() { return &prop[0][0]; }
 };
 
 // Structure factor S(G) = sum_j e^{i G.r_j} (energy_tracking.cpp:72-91) by
@@ -578,7 +583,8 @@ struct RRKernel {
     // last step's pairs are met from both ends: each keeps its own side, and
     // their energy counts half.  Terms are summed in the working precision
     // within a chunk of kPairChunk steps, chunk sums in FP64.
-    __device__ static void pair_terms(const RRConst<Real> &c, const Sh &sh,
+    template <typename ShT>
+    __device__ static void pair_terms(const RRConst<Real> &c, const ShT &sh,
                                       const RRLane<Real, NMAX> &L, int lane, bool has_e, int h,
                                       double g[3], double &lap, double &vreal) {
         g[0] = g[1] = g[2] = 0.0;
@@ -662,7 +668,8 @@ struct RRKernel {
     }
 
     // Jastrow row sum U_i of the lane's electron (jastrow_pade.cpp:7-55 rows).
-    __device__ static double jastrow_row(const RRConst<Real> &c, const Sh &sh,
+    template <typename ShT>
+    __device__ static double jastrow_row(const RRConst<Real> &c, const ShT &sh,
                                          const RRLane<Real, NMAX> &L, int lane, bool has_e, int h) {
         double U = 0.0;
         for (int j = 0; j < N; ++j) {
@@ -681,7 +688,8 @@ struct RRKernel {
 
     // Local energy after a sweep; also refreshes U (drift-free) and returns
     // kinetic / potential parts.
-    __device__ static Measured measure(const DevSys &s, const RRConst<Real> &cst, Sh &sh,
+    template <typename ShT>
+    __device__ static Measured measure(const DevSys &s, const RRConst<Real> &cst, ShT &sh,
                                        RRLane<Real, NMAX> &L, int lane, bool has_e, int h) {
         double g[3], lap, vreal;
         pair_terms(cst, sh, L, lane, has_e, h, g, lap, vreal);
@@ -729,18 +737,21 @@ struct RRKernel {
         // Both are complete here: their inputs (the inverse rows among them)
         // are not carried through the structure-factor loop.
         asm volatile("" : "+v"(m.kinetic), "+v"(m.vreal));
-        m.vrecip = s.pref * recip(s, sh, L, lane, has_e);
+        // (an uncontracted product: callers that add it to other terms round
+        // it the same way in every kernel they are inlined into)
+        m.vrecip = __dmul_rn(s.pref, recip(s, sh, L, lane, has_e));
         m.e_local = m.kinetic + m.vreal + m.vrecip + s.e_const;
         return m;
     }
 
     // sum_G w_G |S_G|^2 with S_G rebuilt from per-electron axis-power tables
     // staged in LDS (energy_tracking.cpp:72-91,146-191).
-    __device__ static double recip(const DevSys &s, Sh &sh, const RRLane<Real, NMAX> &L, int lane,
+    template <typename ShT>
+    __device__ static double recip(const DevSys &s, ShT &sh, const RRLane<Real, NMAX> &L, int lane,
                                    bool has_e) {
-        static_assert(sizeof(sh.prop) + sizeof(sh.pph) + sizeof(sh.tab) >= sizeof(Real) * sf::table_reals<Real>(),
+        static_assert(ShT::kStageBytes >= sizeof(Real) * sf::table_reals<Real>(),
                       "structure-factor tables exceed the stage run");
-        auto *tab = reinterpret_cast<sf::Pair<Real> *>(&sh.prop[0][0]);
+        auto *tab = reinterpret_cast<sf::Pair<Real> *>(sh.stage());
         // Electron e sits in lane e (up) or 32 + e - NU (down).
         const auto pos_of = [&sh](int ax, int e) { return sh.pos[ax][e < NU ? e : 32 + e - NU]; };
         return rr::wave_total(sf::energy<Real, N, GPL, kWave>(s, tab, lane, pos_of));
@@ -822,7 +833,7 @@ struct RRKernel {
     // both outcomes.  Same decisions and updates as two calls of move.
     // Returns the number of accepted moves.
     __device__ static int move_pair(const RRConst<Real> &c, Sh &sh, RRLane<Real, NMAX> &L, int lane,
-                                    bool has_e, int h, int r, int i) {
+                                    bool has_e, int h, int r, int i, LogDetTrack &ldt, double *rec) {
         static_assert(ND == NU, "paired moves need equal spins");
         constexpr int NP = NU;
         const int lu = i, ld = 32 + i;
@@ -868,17 +879,30 @@ struct RRKernel {
         const double jn_d0 = (double)rr::wave_total(un_d);    // up electron at its old position
         const Real ud_old = rr::readlane(un_d, lu);
         // Up move.
-        const int acc_u = accept_test(Ru, jn_u - rr::readlane(L.U, lu), sh.prop[3][lu]);
+        const double dj_u = jn_u - rr::readlane(L.U, lu);
+        const int acc_u = accept_test(Ru, dj_u, sh.prop[3][lu]);
         if (acc_u) {
             apply_move(c, sh, L, lane, has_e, h, r, 0, i, Ru, sk, un_u, jn_u, xu, yu, zu);
             if (lane == lu) un_d = ud_new;
+            ldt.add(0, Ru);
         }
         // Down move, after the up move's outcome.
         const double jn_d = acc_u ? jn_d0 + ((double)ud_new - (double)ud_old) : jn_d0;
-        const int acc_d = accept_test(Rd, jn_d - rr::readlane(L.U, ld), sh.prop[3][ld]);
+        const double dj_d = jn_d - rr::readlane(L.U, ld);
+        const int acc_d = accept_test(Rd, dj_d, sh.prop[3][ld]);
         if (acc_d) {
             __syncthreads();  // the up move's reads of the broadcast row are done
             apply_move(c, sh, L, lane, has_e, h, r, 1, i, Rd, sk, un_d, jn_d, xd, yd, zd);
+            ldt.add(1, Rd);
+        }
+        if (rec && lane == 0) {
+            double *ru = rec + kTraceCount * i, *rd = rec + kTraceCount * (NU + i);
+            ru[0] = Ru;
+            ru[1] = dj_u;
+            ru[2] = (double)acc_u;
+            rd[0] = Rd;
+            rd[1] = dj_d;
+            rd[2] = (double)acc_d;
         }
         return acc_u + acc_d;
     }
@@ -886,7 +910,7 @@ struct RRKernel {
     // One move of electron (HP, rp).  Returns 1 if accepted.
     template <int HP>
     __device__ static int move(const RRConst<Real> &c, Sh &sh, RRLane<Real, NMAX> &L, int lane,
-                               bool has_e, int h, int r, int rp) {
+                               bool has_e, int h, int r, int rp, LogDetTrack &ldt, double *rec) {
         constexpr int NP = HP ? ND : NU;
         const int lp = HP * 32 + rp;
         const double xn = sh.prop[0][lp], yn = sh.prop[1][lp], zn = sh.prop[2][lp];
@@ -945,6 +969,12 @@ struct RRKernel {
         }
         // Every input above is wave-uniform; say so, so the branches are scalar.
         const int accepted = wave_uniform(acc);
+        if (rec && lane == 0) {
+            double *q = rec + kTraceCount * (HP ? NU + rp : rp);
+            q[0] = R;
+            q[1] = dj;
+            q[2] = (double)accepted;
+        }
         // Sherman-Morrison (simulation.cpp:133-145, slater_plane_wave.cpp:
         // 367-423) as one FMA per element and lane: rows k != p of spin HP
         // lose (s_k / R) row_p; row p itself (row_p is its own registers)
@@ -965,6 +995,7 @@ struct RRKernel {
             for (int m = 0; m < NP; ++m) L.inv[m] -= sh.row[m] * f;
         }
         if (!accepted) return 0;
+        ldt.add(HP, R);
         const double xo = rr::readlane(L.x, lp), yo = rr::readlane(L.y, lp), zo = rr::readlane(L.z, lp);
         if (has_e && lane != lp) {
             const Real dx = c.mi_r(xo, L.x);
@@ -985,6 +1016,20 @@ struct RRKernel {
         return 1;
     }
 };
+
+// The proposal q (in [0, L)), its axis phasors and the accept uniform ua
+// into the lane's LDS slots.
+template <typename Real>
+__device__ __forceinline__ void rr_stage_proposal(const DevSys &s, const double (&q)[3], double ua, int lane,
+                                                  double *prop, Real *pph) {
+    for (int k = 0; k < 3; ++k) {
+        prop[k * kWave + lane] = q[k];
+        const rr::Cx<Real> ph = rr::axis_phasor<Real>(q[k], s.inv_L);
+        pph[(2 * k) * kWave + lane] = ph.re;
+        pph[(2 * k + 1) * kWave + lane] = ph.im;
+    }
+    prop[3 * kWave + lane] = ua;
+}
 
 // Philox draws (tags MOVE and ACCP, counter {electron, walker, sweep}),
 // Box-Muller or uniform-cube displacement, wrap into [0, L) and the axis
@@ -1014,20 +1059,17 @@ __device__ __forceinline__ void rr_proposal(const DevSys &s, double sigma, doubl
     const double x = x0 + sigma * dx, y = y0 + sigma * dy, z = z0 + sigma * dz;
     const double q[3] = {x - s.L * floor(x * s.inv_L), y - s.L * floor(y * s.inv_L),
                          z - s.L * floor(z * s.inv_L)};
-    for (int k = 0; k < 3; ++k) {
-        prop[k * kWave + lane] = q[k];
-        const rr::Cx<Real> ph = rr::axis_phasor<Real>(q[k], s.inv_L);
-        pph[(2 * k) * kWave + lane] = ph.re;
-        pph[(2 * k + 1) * kWave + lane] = ph.im;
-    }
-    prop[3 * kWave + lane] = u01(ra.x);
+    rr_stage_proposal<Real>(s, q, u01(ra.x), lane, prop, pph);
 }
 
 __device__ __attribute__((noinline)) double rr_exp_f64(double x) { return exp(x); }
 
-template <typename Real, int NU, int ND, int GPL>
+// kGiven: the moves come from the caller (SweepGiven) instead of Philox and
+// each move's ratio, Jastrow change and decision are recorded.
+template <typename Real, int NU, int ND, int GPL, bool kGiven = false>
 __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &st, int w,
-                                              unsigned long long sweep0, int flags) {
+                                              unsigned long long sweep0, int flags,
+                                              SweepGiven given = SweepGiven{}) {
     using K = RRKernel<Real, NU, ND, GPL>;
     constexpr int NMAX = K::NMAX;
     constexpr int N = NU + ND;
@@ -1075,18 +1117,29 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
     // so its position is current until its own move); staged in LDS for the
     // move that reads it.  One sweep per launch keeps this straight-line code
     // out of the move loop, so its transcendental constants die before it.
-    if (has_e) rr_proposal<Real>(s, sh.misc[0], L.x, L.y, L.z, e, w, sweep, lane, &sh.prop[0][0], &sh.pph[0][0]);
+    double *rec = nullptr;
+    if constexpr (kGiven) {
+        if (has_e) {
+            const size_t q = (size_t)w * N + e;
+            const double p3[3] = {given.prop[3 * q], given.prop[3 * q + 1], given.prop[3 * q + 2]};
+            rr_stage_proposal<Real>(s, p3, given.u[q], lane, &sh.prop[0][0], &sh.pph[0][0]);
+        }
+        rec = given.trace + (size_t)w * N * kTraceCount;
+    } else {
+        if (has_e) rr_proposal<Real>(s, sh.misc[0], L.x, L.y, L.z, e, w, sweep, lane, &sh.prop[0][0], &sh.pph[0][0]);
+    }
     __syncthreads();
     // Sweep order (sweep_visit): with two determinants (ND == NU) the spins
     // alternate, up p then down p.
     int acc = 0;
+    LogDetTrack ldt;
     if constexpr (ND > 0) {
         static_assert(ND == NU, "two determinants of equal size");
 #pragma unroll 1
-        for (int p = 0; p < NU; ++p) acc += K::move_pair(cst, sh, L, lane, has_e, h, r, p);
+        for (int p = 0; p < NU; ++p) acc += K::move_pair(cst, sh, L, lane, has_e, h, r, p, ldt, rec);
     } else {
 #pragma unroll 1
-        for (int p = 0; p < NU; ++p) acc += K::template move<0>(cst, sh, L, lane, has_e, h, r, p);
+        for (int p = 0; p < NU; ++p) acc += K::template move<0>(cst, sh, L, lane, has_e, h, r, p, ldt, rec);
     }
     const int n_acc = acc;
     since += acc;
@@ -1118,8 +1171,8 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
             const unsigned i = atomicAdd(&st.due[0], 1u);
             if (i < (unsigned)st.W) st.due_list[i] = w;
         }
-        st.logdet[2 * w + 0] = sh.misc[1];
-        st.logdet[2 * w + 1] = sh.misc[2];
+        st.logdet[2 * w + 0] = ldt.fold(0, sh.misc[1]);
+        if (ND > 0) st.logdet[2 * w + 1] = ldt.fold(1, sh.misc[2]);
     }
     __syncthreads();
     if (lane < kStatCount) st.stats[(size_t)w * kStatCount + lane] = sh.stat[lane];
@@ -1148,6 +1201,14 @@ template <typename Real, int NU, int ND, int GPL>
 __global__ __launch_bounds__(kWave, 4) void k_sweep_rr_o4(DevSys s, DevState st,
                                                           unsigned long long sweep0, int flags) {
     sweep_rr_body<Real, NU, ND, GPL>(s, st, blockIdx.x, sweep0, flags);
+}
+
+// The same sweep with the caller's moves, recording each move (parity checks
+// of the timed move code).
+template <typename Real, int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave, (rr_sweep_waves<Real, NU, ND>())) void k_sweep_rr_given(
+    DevSys s, DevState st, int flags, SweepGiven given) {
+    sweep_rr_body<Real, NU, ND, GPL, true>(s, st, blockIdx.x, 0, flags, given);
 }
 
 typedef double rr_v4d __attribute__((ext_vector_type(4)));
@@ -1385,9 +1446,11 @@ __device__ __forceinline__ void reinv_rr_body(const DevSys &s, const DevState &s
 // After a sweep: the local energy of every walker with the same lane layout
 // as the sweep, feeding the FP64 accumulators and the blocking state
 // (blocking_analysis.cpp:17-39).
+// With `parts` the walker's kinetic, V_real, V_recip, E_L, log|det| and J go
+// there ([w][kPartCount]) and the statistics are left alone.
 template <typename Real, int NU, int ND, int GPL>
 __device__ __forceinline__ void measure_rr_body(const DevSys &s, const DevState &st, int w,
-                                                int block_size) {
+                                                int block_size, double *parts = nullptr) {
     using K = RRKernel<Real, NU, ND, GPL>;
     constexpr int NMAX = K::NMAX;
     constexpr int N = NU + ND;
@@ -1410,32 +1473,246 @@ __device__ __forceinline__ void measure_rr_body(const DevSys &s, const DevState 
     sh.pos[2][lane] = L.z;
     __syncthreads();
     const Measured m = K::measure(s, cst, sh, L, lane, has_e, h);
+    if (parts) {
+        const double J = 0.5 * rr::wave_total(has_e ? K::jastrow_row(cst, sh, L, lane, has_e, h) : 0.0);
+        if (lane == 0) {
+            double *q = parts + (size_t)w * kPartCount;
+            q[0] = m.kinetic;
+            q[1] = m.vreal;
+            q[2] = m.vrecip;
+            q[3] = m.e_local;
+            q[4] = st.logdet[2 * w + 0];
+            q[5] = ND > 0 ? st.logdet[2 * w + 1] : 0.0;
+            q[6] = J;
+        }
+        return;
+    }
     // The re-inversion that ran before this launch has read the list.
     if (w == 0 && lane == 0 && st.due) st.due[0] = 0;
-    if (lane == 0) {
-        double *t = st.stats + (size_t)w * kStatCount;
-        const double en = m.e_local;
-        const double pot = m.vreal + m.vrecip + s.e_const;
-        t[0] += 1.0;
-        t[1] += en;
-        t[2] += en * en;
-        t[3] += m.kinetic;
-        t[4] += pot;
-        t[5] = en;
-        t[13] = m.kinetic;
-        t[14] = pot;
-        t[8] += en;
-        t[9] += 1.0;
-        if (t[9] >= (double)block_size) {
-            const double bm = t[8] / (double)block_size;
-            t[10] += 1.0;
-            const double delta = bm - t[11];
-            t[11] += delta / t[10];
-            t[12] += delta * (bm - t[11]);
-            t[8] = 0.0;
-            t[9] = 0.0;
+    if (lane == 0) accumulate_measure(st.stats + (size_t)w * kStatCount, m, s.e_const, block_size);
+}
+
+// ------------------------------------------- FP64 local energy, mixed path
+// A mixed-precision walker's local energy is evaluated in FP64 from its FP32
+// inverse refined to FP64 accuracy first, in the same wavefront: with
+// Y = X^T the FP32 inverse (X = invT, lane k of spin h holding row x_k) and
+// D the FP64 Slater matrix (D[i][j] = phi_j(r_i)), a Newton-Schulz step is
+//   E = I - D Y,  Y <- Y + Y E,  i.e.  x_k <- x_k + sum_i E[i][k] x_i,
+// which leaves a residual of about E^2.  Lane k forms column k of E itself
+// (E[i][k] = delta_ik - d_i . x_k, FP64 from the D rows staged in LDS and its
+// own row) and then its correction from the X rows staged in LDS, in FP32
+// (E is below 1e-4 here, so the correction's FP32 rounding sits under 1e-11
+// of the row).  A second step follows when n max|E|^2 exceeds 1e-10; a
+// residual of 0.5 or more (or a non-finite one) rebuilds the inverse by FP64
+// Gauss-Jordan instead (also refreshing log|det|).
+// LDS of the mixed-precision local energy: positions, and one staging area
+// that holds the refinement's D rows (FP64) or X rows (FP32) and, after it,
+// the structure-factor tables.
+template <int NU, int ND>
+struct RRMixedShared {
+    static constexpr int NMAX = NU > ND ? NU : ND;
+    static constexpr int P = (NMAX + 1) / 2 * 2;  // row pitch in doubles (16-byte rows)
+    static constexpr int kRefineDoubles = 2 * NMAX * P;
+    static constexpr int kTableDoubles = sf::table_reals<double>();
+    double pos[3][kWave];
+    alignas(16) double buf[kRefineDoubles > kTableDoubles ? kRefineDoubles : kTableDoubles];
+    float et[2 * NMAX][NMAX | 1];  // each row lane's column of E (FP32; odd pitch)
+    double gjcol[32];
+    int perm[32];
+    static constexpr size_t kStageBytes = sizeof(buf);
+    __device__ void *stage() { return buf; }
+    __device__ double *rows(int h) { return buf + h * NMAX * P; }
+};
+
+constexpr double kRefineTarget = 1e-10;  // n max|E|^2 after the last step
+
+// Refines lane (h, r)'s row of the FP32 inverse, held widened in L.inv, to
+// FP64 accuracy in place.  Returns the residual max|I - D Y| of the FP32
+// inverse (the drift; +inf when it was rebuilt from scratch, -1 when a
+// determinant is singular).  ld[h] receives a rebuilt determinant's
+// log|det|.
+template <int NU, int ND, int GPL>
+__device__ double rr_refine(const DevSys &s, RRMixedShared<NU, ND> &sh, RRLane<double, (NU > ND ? NU : ND)> &L,
+                            int lane, int h, int r, bool has_e, double (&ld)[2]) {
+    using K = RRKernel<double, NU, ND, GPL>;
+    using Sh = RRMixedShared<NU, ND>;
+    constexpr int NMAX = K::NMAX, P = Sh::P;
+    constexpr int NP = NU;  // equal spins or polarised: every row lane has NU rows
+    const int nh = h ? ND : NU;
+    constexpr auto orb = rr::shell_orbitals<NMAX>();
+    // D row of the lane's electron (FP64 axis phasors) into its LDS row.
+    const auto put_d_row = [&](double *dst) {
+        // The position made opaque: otherwise the row, invariant across the
+        // refinement steps, is hoisted out of their loop and held in
+        // registers throughout.
+        double px = L.x, py = L.y, pz = L.z;
+        asm volatile("" : "+v"(px), "+v"(py), "+v"(pz));
+        rr::Cx<double> a[3], a2[3];
+        a[0] = rr::axis_phasor<double>(px, s.inv_L);
+        a[1] = rr::axis_phasor<double>(py, s.inv_L);
+        a[2] = rr::axis_phasor<double>(pz, s.inv_L);
+        K::squares(a, a2);
+#pragma unroll
+        for (int m = 0; m < NMAX; ++m) {
+            const rr::Cx<double> ph = rr::orbital_phasor<double>(orb[m].x, orb[m].y, orb[m].z, a, a2);
+            dst[m] = orb[m].t ? ph.im : ph.re;
+        }
+    };
+    double drift0 = 0.0;
+    bool rebuild = false;
+#pragma unroll 1
+    for (int it = 0; it < 3; ++it) {
+        if (has_e) put_d_row(sh.rows(h) + r * P);
+        __syncthreads();
+        // Column r of E into the lane's LDS row (a rolled loop: unrolled, the
+        // rows' reads are interleaved and spill).
+        double dm = 0.0;
+        bool finite = true;
+#pragma unroll 1
+        for (int i = 0; i < NMAX; ++i) {
+            const double *Di = sh.rows(h) + i * P;
+            double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+            for (int j = 0; j < NMAX; j += 2) {
+                a0 = fma(Di[j], L.inv[j], a0);
+                if (j + 1 < NMAX) a1 = fma(Di[j + 1], L.inv[j + 1], a1);
+            }
+            const double ev = (i == r ? 1.0 : 0.0) - (a0 + a1);
+            if (has_e) sh.et[h * NMAX + r][i] = (float)ev;
+            if (has_e && i < nh) {
+                dm = fabs(ev) > dm ? fabs(ev) : dm;
+                finite = finite && (ev == ev);
+            }
+        }
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) dm = fmax(dm, __shfl_xor(dm, off, kWave));
+        dm = wave_uniform(dm);
+        const int all_finite = __syncthreads_and(finite ? 1 : 0);
+        if (it == 0) drift0 = dm;
+        if (!all_finite || !(dm < 0.5)) {
+            rebuild = true;
+            break;
+        }
+        // X rows in the D rows' place: the current rows rounded to FP32 (the
+        // correction Y E needs Y only to the accuracy of its small factor E).
+        float *X = reinterpret_cast<float *>(sh.rows(h));
+        if (has_e) {
+#pragma unroll
+            for (int m = 0; m < NMAX; ++m) X[r * 2 * P + m] = (float)L.inv[m];
+        }
+        __syncthreads();
+        float c[NMAX];
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) c[j] = 0.0f;
+#pragma unroll 1
+        for (int i = 0; i < NMAX; ++i) {
+            const float *Xi = X + i * 2 * P;
+            const float e = has_e ? sh.et[h * NMAX + r][i] : 0.0f;
+#pragma unroll
+            for (int j = 0; j < NMAX; ++j) c[j] = fmaf(e, Xi[j], c[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) L.inv[j] += (double)c[j];
+        __syncthreads();
+        if ((double)NP * dm * dm <= kRefineTarget) break;
+    }
+    if (rebuild) {
+        // FP64 Gauss-Jordan of each determinant from scratch, A = D^T.
+#pragma unroll 1
+        for (int hp = 0; hp < (ND > 0 ? 2 : 1); ++hp) {
+            double *A = sh.buf;
+            if (h == hp && has_e) {
+                double d[NMAX];
+                put_d_row(d);
+#pragma unroll
+                for (int m = 0; m < NMAX; ++m) A[m * NP + r] = d[m];
+            }
+            __syncthreads();
+            const double l = rr_gauss_jordan(A, sh.gjcol, sh.perm, NP, lane);
+            if (!(l > -INFINITY)) return -1.0;
+            ld[hp] = l;
+            if (h == hp && has_e) {
+#pragma unroll
+                for (int m = 0; m < NMAX; ++m) L.inv[m] = A[r * NP + m];
+            }
+            __syncthreads();
+        }
+        return INFINITY;
+    }
+    return drift0;
+}
+
+// Mixed-precision local energy (k_measure_rr for FP32 inverses): the FP32
+// inverse refined to FP64 (rr_refine), then the FP64 measurement.  The
+// refined inverse, rounded to FP32, replaces the stored one (this is the
+// walker's periodic re-inversion: move count cleared, drift recorded) unless
+// only the parts are asked for.
+template <int NU, int ND, int GPL>
+__device__ __forceinline__ void measure_rr_refined_body(const DevSys &s, const DevState &st, int w,
+                                                        int block_size, double *parts = nullptr) {
+    using K = RRKernel<double, NU, ND, GPL>;
+    constexpr int NMAX = K::NMAX;
+    constexpr int N = NU + ND;
+    __shared__ RRMixedShared<NU, ND> sh;
+    const RRConst<double> cst(s);
+    const int lane = threadIdx.x;
+    const int h = lane >> 5, r = lane & 31;
+    const int nh = h ? ND : NU;
+    const bool has_e = r < nh;
+    const int e = h ? NU + r : r;
+    RRLane<double, NMAX> L;
+    {
+        float x[NMAX];
+        const float *inv_hbm = reinterpret_cast<const float *>(st.inv) + (size_t)w * s.ninv;
+        rr_inverse_in<float, NU, ND, NMAX>(inv_hbm, reinterpret_cast<float *>(sh.buf), x, lane, h, r);
+#pragma unroll
+        for (int m = 0; m < NMAX; ++m) L.inv[m] = (double)x[m];
+    }
+    const double *gp = st.pos + (size_t)w * 3 * N;
+    L.x = has_e ? gp[e] : 0.0;
+    L.y = has_e ? gp[N + e] : 0.0;
+    L.z = has_e ? gp[2 * N + e] : 0.0;
+    sh.pos[0][lane] = L.x;
+    sh.pos[1][lane] = L.y;
+    sh.pos[2][lane] = L.z;
+    double ld[2] = {st.logdet[2 * w + 0], st.logdet[2 * w + 1]};
+    const double drift = rr_refine<NU, ND, GPL>(s, sh, L, lane, h, r, has_e, ld);
+    if (!parts) {
+        float xo[NMAX];
+#pragma unroll
+        for (int m = 0; m < NMAX; ++m) xo[m] = (float)L.inv[m];
+        float *inv_hbm = reinterpret_cast<float *>(st.inv) + (size_t)w * s.ninv;
+        rr_inverse_out<float, NU, ND, NMAX>(inv_hbm, reinterpret_cast<float *>(sh.buf), xo, lane, h, r);
+        if (lane == 0) {
+            st.since[w] = 0;
+            st.stats[(size_t)w * kStatCount + 15] += 1.0;
+            if (drift < 0.0) {
+                st.flags[w] |= kFlagSingular;
+            } else {
+                st.logdet[2 * w + 0] = ld[0];
+                st.logdet[2 * w + 1] = ld[1];
+                if (drift > st.drift[w] || drift != drift) st.drift[w] = drift;
+            }
         }
     }
+    __syncthreads();
+    const Measured m = K::measure(s, cst, sh, L, lane, has_e, h);
+    if (parts) {
+        const double J = 0.5 * rr::wave_total(has_e ? K::jastrow_row(cst, sh, L, lane, has_e, h) : 0.0);
+        if (lane == 0) {
+            double *q = parts + (size_t)w * kPartCount;
+            q[0] = m.kinetic;
+            q[1] = m.vreal;
+            q[2] = m.vrecip;
+            q[3] = m.e_local;
+            q[4] = st.logdet[2 * w + 0];
+            q[5] = ND > 0 ? st.logdet[2 * w + 1] : 0.0;
+            q[6] = J;
+        }
+        return;
+    }
+    if (w == 0 && lane == 0 && st.due) st.due[0] = 0;
+    if (lane == 0) accumulate_measure(st.stats + (size_t)w * kStatCount, m, s.e_const, block_size);
 }
 
 // The re-inversion over the walkers the preceding sweep listed (sweep
@@ -1471,6 +1748,31 @@ template <typename Real, int NU, int ND, int GPL>
 __global__ __launch_bounds__(kWave, (rr_measure_waves<Real, NU + ND, GPL>())) void k_measure_rr(DevSys s, DevState st,
                                                                                int block_size) {
     measure_rr_body<Real, NU, ND, GPL>(s, st, blockIdx.x, block_size);
+}
+
+// The local energy's parts per walker, statistics untouched.
+template <typename Real, int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave) void k_measure_rr_parts(DevSys s, DevState st, double *parts) {
+    measure_rr_body<Real, NU, ND, GPL>(s, st, blockIdx.x, 1, parts);
+}
+
+// Mixed precision: the FP64 local energy from the refined FP32 inverse
+// (which it also refreshes), its parts-only form, and the batch form.
+template <int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave, ND > 0 ? 2 : 1) void k_measure_rr_mixed(DevSys s, DevState st, int block_size) {
+    measure_rr_refined_body<NU, ND, GPL>(s, st, blockIdx.x, block_size);
+}
+
+template <int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave, ND > 0 ? 2 : 1) void k_measure_rr_mixed_parts(DevSys s, DevState st, double *parts) {
+    measure_rr_refined_body<NU, ND, GPL>(s, st, blockIdx.x, 1, parts);
+}
+
+template <int NU, int ND, int GPL>
+__global__ __launch_bounds__(kWave, ND > 0 ? 2 : 1) void k_measure_rr_mixed_b(const BatchItem *__restrict__ items, int n_items,
+                                                              int block_size) {
+    const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
+    measure_rr_refined_body<NU, ND, GPL>(it.s, it.st, blockIdx.x - it.walker_begin, block_size);
 }
 
 // Batch forms: one launch covers the walkers of every system of a batch group

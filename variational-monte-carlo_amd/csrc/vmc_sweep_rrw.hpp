@@ -46,7 +46,7 @@ struct RRWShared {
     double mv[2][8];     // (up) mover: R, U, x, y, z (by move parity)
     double mv2[2][8];    // down mover of a paired move
     double stat[kStatCount];
-    double misc[4];      // sigma
+    double misc[4];      // sigma, log|det| up, log|det| down
 };
 
 template <typename Real, int NU, int ND, int WPS>
@@ -194,7 +194,8 @@ struct RRW {
     // the down electron, formed as the threads form them).  Same decisions
     // and updates as two calls of move.  Returns the accepted moves.
     __device__ static int move_pair(const RRConst<Real> &c, Sh &sh, const Thread &T, Real (&inv)[NMAX],
-                                    double &x, double &y, double &z, double &U, int ob, int i, int par) {
+                                    double &x, double &y, double &z, double &U, int ob, int i, int par,
+                                    LogDetTrack &ldt, double *rec) {
         static_assert(ND == NU, "paired moves need equal spins");
         const int pu = i, pd = NU + i;
         const double xu = sh.prop[0][pu], yu = sh.prop[1][pu], zu = sh.prop[2][pu];
@@ -270,9 +271,11 @@ struct RRW {
         // Phase C, up move.
         const double Ru = sh.mv[par][0];
         const double uxo = sh.mv[par][2], uyo = sh.mv[par][3], uzo = sh.mv[par][4];
-        const int acc_u = accept_test(Ru, jn_u - sh.mv[par][1], sh.prop[3][pu]);
+        const double dj_u = jn_u - sh.mv[par][1];
+        const int acc_u = accept_test(Ru, dj_u, sh.prop[3][pu]);
         double Ud = sh.mv2[par][1];
         if (acc_u) {
+            ldt.add(0, Ru);
             if (T.h == 0) {
                 const Real inv_ratio = wrcp<Real>((Real)Ru);
                 Real f = (T.r == i) ? (Real)1 - inv_ratio : sk * inv_ratio;
@@ -303,8 +306,19 @@ struct RRW {
         }
         // Phase C, down move.
         const double Rd = sh.mv2[par][0];
-        const int acc_d = accept_test(Rd, jn_d - Ud, sh.prop[3][pd]);
+        const double dj_d = jn_d - Ud;
+        const int acc_d = accept_test(Rd, dj_d, sh.prop[3][pd]);
+        if (rec && threadIdx.x == 0) {
+            double *ru = rec + kTraceCount * pu, *rd = rec + kTraceCount * pd;
+            ru[0] = Ru;
+            ru[1] = dj_u;
+            ru[2] = (double)acc_u;
+            rd[0] = Rd;
+            rd[1] = dj_d;
+            rd[2] = (double)acc_d;
+        }
         if (acc_d) {
+            ldt.add(1, Rd);
             if (T.h == 1) {
                 const Real inv_ratio = wrcp<Real>((Real)Rd);
                 Real f = (T.r == i) ? (Real)1 - inv_ratio : sk * inv_ratio;
@@ -331,7 +345,8 @@ struct RRW {
     // One move of electron p, the sweep's move number par (mod 2) selecting
     // the LDS buffers phase C reads.  Returns 1 if accepted (uniform).
     __device__ static int move(const RRConst<Real> &c, Sh &sh, const Thread &T, Real (&inv)[NMAX],
-                               double &x, double &y, double &z, double &U, int ob, int p, int par) {
+                               double &x, double &y, double &z, double &U, int ob, int p, int par,
+                               LogDetTrack &ldt, double *rec) {
         const int hp = (ND > 0 && p >= NU) ? 1 : 0;
         const int rp = hp ? p - NU : p;
         const int NP = hp ? ND : NU;
@@ -418,7 +433,15 @@ struct RRW {
         if (!(approx == approx)) acc = 0;
         else if (fabs(ua - approx) > 1e-5 * approx || !(fabs(dj) < 40.0)) acc = ua < approx;
         else acc = ua < r2 * rr_exp_f64(2.0 * dj);
-        if (!wave_uniform(acc)) return 0;
+        acc = wave_uniform(acc);
+        if (rec && threadIdx.x == 0) {
+            double *q = rec + kTraceCount * p;
+            q[0] = R;
+            q[1] = dj;
+            q[2] = (double)acc;
+        }
+        if (!acc) return 0;
+        ldt.add(hp, R);
         if (T.h == hp) {
             const Real inv_ratio = wrcp<Real>((Real)R);
             Real f = (T.r == rp) ? (Real)1 - inv_ratio : sk * inv_ratio;
@@ -448,6 +471,20 @@ struct RRW {
     }
 };
 
+// Proposal q (in [0, L)) of electron e, its axis phasors and accept uniform
+// ua into the electron's slots.
+template <typename Real>
+__device__ __forceinline__ void rrw_stage_proposal(const DevSys &s, const double (&q)[3], double ua, int e, int N,
+                                                   double *prop, Real *pph) {
+    for (int k = 0; k < 3; ++k) {
+        prop[k * N + e] = q[k];
+        const rr::Cx<Real> ph = rr::axis_phasor<Real>(q[k], s.inv_L);
+        pph[(2 * k) * N + e] = ph.re;
+        pph[(2 * k + 1) * N + e] = ph.im;
+    }
+    prop[3 * N + e] = ua;
+}
+
 // Philox draws of one electron for this sweep into its proposal slots.
 template <typename Real>
 __device__ __forceinline__ void rrw_proposal(const DevSys &s, double sigma, double x0, double y0,
@@ -473,18 +510,15 @@ __device__ __forceinline__ void rrw_proposal(const DevSys &s, double sigma, doub
     const double x = x0 + sigma * dx, y = y0 + sigma * dy, z = z0 + sigma * dz;
     const double q[3] = {x - s.L * floor(x * s.inv_L), y - s.L * floor(y * s.inv_L),
                          z - s.L * floor(z * s.inv_L)};
-    for (int k = 0; k < 3; ++k) {
-        prop[k * N + e] = q[k];
-        const rr::Cx<Real> ph = rr::axis_phasor<Real>(q[k], s.inv_L);
-        pph[(2 * k) * N + e] = ph.re;
-        pph[(2 * k + 1) * N + e] = ph.im;
-    }
-    prop[3 * N + e] = u01(ra.x);
+    rrw_stage_proposal<Real>(s, q, u01(ra.x), e, N, prop, pph);
 }
 
-template <typename Real, int NU, int ND, int WPS>
+// kGiven: the moves come from the caller (SweepGiven) instead of Philox and
+// each move's ratio, Jastrow change and decision are recorded.
+template <typename Real, int NU, int ND, int WPS, bool kGiven = false>
 __device__ __forceinline__ void sweep_rrw_body(const DevSys &s, const DevState &st, int w,
-                                               unsigned long long sweep0, int flags) {
+                                               unsigned long long sweep0, int flags,
+                                               SweepGiven given = SweepGiven{}) {
     using K = RRW<Real, NU, ND, WPS>;
     constexpr int N = K::N;
     constexpr int NMAX = K::NMAX;
@@ -495,7 +529,11 @@ __device__ __forceinline__ void sweep_rrw_body(const DevSys &s, const DevState &
     double x, y, z;
     K::load(s, st, w, sh, T, inv, x, y, z);
     if (threadIdx.x < kStatCount) sh.stat[threadIdx.x] = st.stats[(size_t)w * kStatCount + threadIdx.x];
-    if (threadIdx.x == 0) sh.misc[0] = st.sigma[w];
+    if (threadIdx.x == 0) {
+        sh.misc[0] = st.sigma[w];
+        sh.misc[1] = st.logdet[2 * w + 0];
+        sh.misc[2] = st.logdet[2 * w + 1];
+    }
     __syncthreads();
     const int wflags = st.flags[w];
     double U;
@@ -504,25 +542,37 @@ __device__ __forceinline__ void sweep_rrw_body(const DevSys &s, const DevState &
     else
         U = T.has_e ? st.jrow[(size_t)w * N + T.e] : 0.0;
     const int ob = K::packed_orbital(T.r);
-    if (T.has_e)
-        rrw_proposal<Real>(s, sh.misc[0], x, y, z, T.e, w, (uint32_t)sweep0, N, &sh.prop[0][0], &sh.pph[0][0]);
+    double *rec = nullptr;
+    if constexpr (kGiven) {
+        if (T.has_e) {
+            const size_t q = (size_t)w * N + T.e;
+            const double p3[3] = {given.prop[3 * q], given.prop[3 * q + 1], given.prop[3 * q + 2]};
+            rrw_stage_proposal<Real>(s, p3, given.u[q], T.e, N, &sh.prop[0][0], &sh.pph[0][0]);
+        }
+        rec = given.trace + (size_t)w * N * kTraceCount;
+    } else {
+        if (T.has_e)
+            rrw_proposal<Real>(s, sh.misc[0], x, y, z, T.e, w, (uint32_t)sweep0, N, &sh.prop[0][0], &sh.pph[0][0]);
+    }
     __syncthreads();
     int acc = 0;
+    LogDetTrack ldt;
     if constexpr (K::kPaired) {
 #pragma unroll 1
-        for (int i = 0; i < NU; ++i) acc += K::move_pair(cst, sh, T, inv, x, y, z, U, ob, i, i & 1);
+        for (int i = 0; i < NU; ++i) acc += K::move_pair(cst, sh, T, inv, x, y, z, U, ob, i, i & 1, ldt, rec);
     } else if constexpr (ND > 0 && sizeof(Real) == 8) {
         // The sweep order up i, down i as two calls with constant spin and
         // buffer parity (the move's addressing then folds; in FP32 at 81 and
         // 123 per spin the doubled loop body measured slower).
 #pragma unroll 1
         for (int i = 0; i < NU; ++i) {
-            acc += K::move(cst, sh, T, inv, x, y, z, U, ob, i, 0);
-            acc += K::move(cst, sh, T, inv, x, y, z, U, ob, NU + i, 1);
+            acc += K::move(cst, sh, T, inv, x, y, z, U, ob, i, 0, ldt, rec);
+            acc += K::move(cst, sh, T, inv, x, y, z, U, ob, NU + i, 1, ldt, rec);
         }
     } else {
 #pragma unroll 1
-        for (int t = 0; t < N; ++t) acc += K::move(cst, sh, T, inv, x, y, z, U, ob, sweep_visit(NU, ND, t), t & 1);
+        for (int t = 0; t < N; ++t)
+            acc += K::move(cst, sh, T, inv, x, y, z, U, ob, sweep_visit(NU, ND, t), t & 1, ldt, rec);
     }
     __syncthreads();
     if (flags & VMC_SWEEP_ADAPT) {
@@ -551,6 +601,8 @@ __device__ __forceinline__ void sweep_rrw_body(const DevSys &s, const DevState &
         // (a count at K was re-inverted after the last sweep)
         st.since[w] = (s.reinv_k > 0 && st.since[w] >= s.reinv_k ? 0 : st.since[w]) + acc;
         st.flags[w] = wflags | kFlagJrowValid;
+        st.logdet[2 * w + 0] = ldt.fold(0, sh.misc[1]);
+        if (ND > 0) st.logdet[2 * w + 1] = ldt.fold(1, sh.misc[2]);
     }
     __syncthreads();
     if (threadIdx.x < kStatCount) st.stats[(size_t)w * kStatCount + threadIdx.x] = sh.stat[threadIdx.x];
@@ -561,9 +613,11 @@ __device__ __forceinline__ void sweep_rrw_body(const DevSys &s, const DevState &
 // 425-499): per thread its electron's pair terms and Slater terms; the
 // structure factor from per-electron axis-power tables staged in LDS, the
 // workgroup's threads owning G vectors.
+// With `parts` the walker's kinetic, V_real, V_recip, E_L, log|det| and J go
+// there ([w][kPartCount]) and the statistics are left alone.
 template <typename Real, int NU, int ND, int WPS>
 __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState &st, int w,
-                                                 int block_size) {
+                                                 int block_size, double *parts = nullptr) {
     using K = RRW<Real, NU, ND, WPS>;
     constexpr int N = K::N;
     constexpr int NMAX = K::NMAX;
@@ -708,7 +762,21 @@ __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState
     const auto pos_of = [](int ax, int e) { return sh.pos[ax][e]; };
     const double acc = sf::energy<Real, N, GPT, NT>(s, reinterpret_cast<sf::Pair<Real> *>(&sh.prop[0][0]),
                                                     (int)threadIdx.x, pos_of);
-    const double vrecip = s.pref * K::block_total(acc, sh.red);
+    const double vrecip = __dmul_rn(s.pref, K::block_total(acc, sh.red));  // (uncontracted, as in every kernel)
+    if (parts) {
+        const double J = 0.5 * K::block_total(K::jastrow_row(c, sh, T, x, y, z), sh.red);
+        if (threadIdx.x == 0) {
+            double *q = parts + (size_t)w * kPartCount;
+            q[0] = kinetic;
+            q[1] = vr;
+            q[2] = vrecip;
+            q[3] = kinetic + vr + vrecip + s.e_const;
+            q[4] = st.logdet[2 * w + 0];
+            q[5] = ND > 0 ? st.logdet[2 * w + 1] : 0.0;
+            q[6] = J;
+        }
+        return;
+    }
     if (threadIdx.x == 0) {
         double *t = st.stats + (size_t)w * kStatCount;
         const double en = kinetic + vr + vrecip + s.e_const;
@@ -763,6 +831,21 @@ template <typename Real, int NU, int ND, int WPS>
 __global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64, (rrw_measure_waves<Real, NU, ND, WPS>())) void k_measure_rrw(DevSys s, DevState st,
                                                                             int block_size) {
     measure_rrw_body<Real, NU, ND, WPS>(s, st, blockIdx.x, block_size);
+}
+
+// The sweep with the caller's moves, each recorded (parity checks of the
+// timed move code).
+template <typename Real, int NU, int ND, int WPS>
+__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64, (rrw_sweep_waves<Real, NU, ND, WPS>())) void k_sweep_rrw_given(
+    DevSys s, DevState st, int flags, SweepGiven given) {
+    sweep_rrw_body<Real, NU, ND, WPS, true>(s, st, blockIdx.x, 0, flags, given);
+}
+
+// The local energy's parts per walker, statistics untouched.
+template <typename Real, int NU, int ND, int WPS>
+__global__ __launch_bounds__((ND > 0 ? 2 : 1) * WPS * 64) void k_measure_rrw_parts(DevSys s, DevState st,
+                                                                                  double *parts) {
+    measure_rrw_body<Real, NU, ND, WPS>(s, st, blockIdx.x, 1, parts);
 }
 
 // Batch forms over a group of systems of one size (as k_sweep_rr_b).

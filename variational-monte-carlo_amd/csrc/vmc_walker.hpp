@@ -218,6 +218,33 @@ struct Measured {
     double kinetic, vreal, vrecip, e_local;
 };
 
+// One measured local energy into a walker's FP64 statistics t[kStatCount]
+// and its blocking state: block means of block_size samples, Welford
+// over the block means (blocking_analysis.cpp:17-39).
+__device__ inline void accumulate_measure(double *t, const Measured &m, double e_const, int block_size) {
+    const double en = m.e_local;
+    const double pot = m.vreal + m.vrecip + e_const;
+    t[0] += 1.0;
+    t[1] += en;
+    t[2] += en * en;
+    t[3] += m.kinetic;
+    t[4] += pot;
+    t[5] = en;
+    t[13] = m.kinetic;
+    t[14] = pot;
+    t[8] += en;
+    t[9] += 1.0;
+    if (t[9] >= (double)block_size) {
+        const double bm = t[8] / (double)block_size;
+        t[10] += 1.0;
+        const double delta = bm - t[11];
+        t[11] += delta / t[10];
+        t[12] += delta * (bm - t[11]);
+        t[8] = 0.0;
+        t[9] = 0.0;
+    }
+}
+
 // Local energy of the current configuration (energy_tracking.cpp:253-382):
 // full Jastrow and Slater derivatives (jastrow_pade.cpp:57-138,
 // slater_plane_wave.cpp:425-499), real-space Ewald over all pairs and the
@@ -307,7 +334,9 @@ __device__ Measured measure(const DevSys &s, const WalkerView<Real> &v, int lane
             sk_out[2 * g + 1] = sim;
         }
     }
-    m.vrecip = s.pref * block_sum(vq);
+    // (an uncontracted product: the sums callers form with it round the same
+    // way in every kernel this is inlined into, batch and alone)
+    m.vrecip = __dmul_rn(s.pref, block_sum(vq));
     m.e_local = m.kinetic + m.vreal + m.vrecip + s.e_const;
     return m;
 }

@@ -22,6 +22,7 @@ HEADER_PATH = os.path.join(REPO_DIR, "include", "vmc_hip.h")
 VMC_OK = 0
 ERRORS = {1: "invalid argument", 2: "device error", 3: "singular", 4: "out of memory",
           5: "bad state"}
+ERR_INVALID, ERR_DEVICE, ERR_SINGULAR, ERR_MEMORY, ERR_STATE = 1, 2, 3, 4, 5
 
 PREC_FP64, PREC_MIXED = 0, 1
 MOVE_UNIFORM, MOVE_GAUSSIAN = 0, 1
@@ -37,6 +38,10 @@ FZ = dict(LOGDET0=0, LOGDET1=1, JASTROW=2, KINETIC=3, VREAL=4, VRECIP=5, ELOCAL=
           DELTA_J=8, VREAL_INC=9, VRECIP_INC=10, ELOCAL_AFTER=11, KINETIC_AFTER=12,
           JASTROW_AFTER=13, SINGULAR=14)
 FZ_COUNT = 15
+MV = dict(RATIO=0, DELTA_J=1, ACCEPTED=2)
+MV_COUNT = 3
+PART = dict(KINETIC=0, VREAL=1, VRECIP=2, ELOCAL=3, LOGDET0=4, LOGDET1=5, JASTROW=6)
+PART_COUNT = 7
 
 
 class VmcError(RuntimeError):
@@ -179,6 +184,9 @@ def lib():
             "vmc_ensemble_set_positions": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_get_positions": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_sweep": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _VP]),
+            "vmc_ensemble_sweep_given": (C.c_int, [_VP, _dp, _dp, _dp, _VP]),
+            "vmc_ensemble_measure": (C.c_int, [_VP, _dp, _VP]),
+            "vmc_ensemble_get_inverse": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_reset_stats": (C.c_int, [_VP, _VP]),
             "vmc_ensemble_set_kernel": (C.c_int, [_VP, C.c_int32]),
             "vmc_ensemble_get_kernel": (C.c_int, [_VP, _ip]),
@@ -294,6 +302,30 @@ class Ensemble:
     def sweep(self, n_sweeps=1, measure=False, adapt=False, block_size=1):
         flags = (SWEEP_MEASURE if measure else 0) | (SWEEP_ADAPT if adapt else 0)
         _check(lib().vmc_ensemble_sweep(self._h, n_sweeps, flags, block_size, self.stream))
+
+    def sweep_given(self, proposals, uniforms):
+        """One sweep with the caller's moves (vmc_ensemble_sweep_given):
+        proposals [W][N][3], uniforms [W][N]; returns the trace [W][N][MV_COUNT]
+        (ratio, Jastrow change, accepted) indexed by electron."""
+        p = np.ascontiguousarray(proposals, dtype=np.float64).reshape(self.W, self.N, 3)
+        u = np.ascontiguousarray(uniforms, dtype=np.float64).reshape(self.W, self.N)
+        tr = np.zeros((self.W, self.N, MV_COUNT))
+        _check(lib().vmc_ensemble_sweep_given(self._h, _ptr(p), _ptr(u), _ptr(tr), self.stream))
+        return tr
+
+    def measure(self):
+        """Per-walker local energy parts (vmc_ensemble_measure) as a dict of
+        arrays keyed like PART."""
+        a = np.zeros((self.W, PART_COUNT))
+        _check(lib().vmc_ensemble_measure(self._h, _ptr(a), self.stream))
+        return {k: a[:, i].copy() for k, i in PART.items()}
+
+    def inverse(self):
+        """Transposed inverses [W][n_up^2 + n_dn^2] as doubles."""
+        (_, _), _, ninv = system_sizes(self.system)
+        a = np.zeros((self.W, ninv))
+        _check(lib().vmc_ensemble_get_inverse(self._h, _ptr(a), self.stream))
+        return a
 
     def set_kernel(self, kernel):
         _check(lib().vmc_ensemble_set_kernel(self._h, kernel))
