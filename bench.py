@@ -23,12 +23,27 @@ sys.path.insert(0, os.path.join(ROOT, "variational-monte-carlo_amd"))
 
 N_ELECTRONS, RS, JB, WALKERS = 54, 2.0, 1.0, 4096
 SEED = 20251202
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_measure_rr_N54_W4096_traffic.md")
+PROFILES = os.path.join(ROOT, "profiles")
 
 # MI355X published dense peaks.
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_LDS_TBPS = 256 * 128 * 2.4e9 / 1e12  # 256 CUs x 128 B/clk x 2.4 GHz
+
+
+def refine_flops_per_walker(n, ndet=2):
+    """Algorithmic flops of the FP64 refinement of a walker's FP32 inverse in
+    the mixed local energy (one Newton-Schulz step per determinant: the
+    residual E = I - D Y and the correction Y E, 2 n^3 each)."""
+    return ndet * 4 * n ** 3
+
+
+def sweep_side_flops_per_move(n, nk, N, p):
+    """Sweep-side work per proposed move of the reference algorithm on this
+    workload (add/mul = 1, FMA = 2): orbital phases 6 n_k, ratio 2 n, new
+    Jastrow pairs 40 (N-1), and on accept (rate p) the Sherman-Morrison update
+    4 n^2."""
+    return 6 * nk + 2 * n + 40 * (N - 1) + p * 4 * n * n
 
 
 def measure_flops_per_walker(n, N, ng):
@@ -71,16 +86,67 @@ def path_work_per_move(n, nk, N, ng, p):
     return flops, lds
 
 
-def read_traffic():
-    """HBM bytes per local-energy kernel launch from the committed rocprofv3 --pmc
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from its committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(os.path.join(PROFILES, f"{kernel}_N54_W4096_traffic.md")) as f:
             text = f.read()
     except OSError:
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
     return float(m.group(1)) if m else None
+
+
+# Kernels of the headline step, their arithmetic and their algorithmic work.
+KERNELS = {
+    "mixed": {"sweep": ("k_sweep_rr", "f32", "sweep-side work of the reference per proposed move "
+                        "(6 n_k + 2 n + 40 (N-1) + p 4 n^2) x 4096 x 54 moves"),
+              "measure": ("k_measure_rr_mixed", "f64", "FP64 refinement of the FP32 inverse (8 n^3) and "
+                          "the local energy (24 N(N-1) + 14 N n + 14 N N_G + 4 N_G) per walker x 4096"),
+              "reinvert": ("k_reinv_rr_list", "f64", "Newton-Schulz step 8 n^3 per re-inverted walker")},
+    "fp64": {"sweep": ("k_sweep_rr", "f64", "sweep-side work of the reference per proposed move x 4096 x 54"),
+             "measure": ("k_measure_rr", "f64", "local energy per walker x 4096"),
+             "reinvert": ("k_reinv_rr_list", "f64", "")},
+}
+
+
+def roofline_block(head, head_name):
+    """Roofline entries of the step's kernels from their HIP-event times:
+    the dominant one (largest time per step) is the roofline, the others
+    follow as secondary entries.  Every kernel here is bound by VALU issue
+    and latency (no MFMA, HBM traffic near its algorithmic bytes); `peak` is
+    the published dense figure of the kernel's arithmetic (FP32 157.3 TFLOP/s
+    counts packed FP32, which this build does not issue: non-packed FP32
+    issues at the FP64 rate, 78.6 TFLOP/s, given as peak_nonpacked)."""
+    ents = {}
+    for kind in ("sweep", "measure", "reinvert"):
+        if f"{kind}_ms_avg" not in head:
+            continue
+        name, arith, what = KERNELS[head_name][kind]
+        peak = PEAK_FP32_TFLOPS if arith == "f32" else PEAK_FP64_TFLOPS
+        achieved = head[f"{kind}_tflops"]
+        per_step_ms = head[f"{kind}_ms_avg"] * head.get(f"{kind}_launches", 1) / max(head.get("sweep_launches", 1), 1)
+        ents[kind] = {"kernel": name, "dtype": arith, "bound": "valu", "achieved": achieved, "peak": peak,
+                      "unit": "TFLOP/s", "frac": achieved / peak, "peak_nonpacked": PEAK_FP64_TFLOPS,
+                      "frac_nonpacked": achieved / PEAK_FP64_TFLOPS, "launch_ms_avg": head[f"{kind}_ms_avg"],
+                      "ms_per_step": per_step_ms, "work": what}
+    dom = max(ents, key=lambda k: ents[k]["ms_per_step"])
+    line = dict(ents[dom])
+    line["traffic"] = read_traffic(line["kernel"])
+    line["secondary"] = {k: v for k, v in ents.items() if k != dom}
+    acc_rate = head["acceptance"]
+    f_move, lds_move = path_work_per_move(27, 14, 54, 257, acc_rate)
+    line["path"] = {
+        "what": "whole path at the measured electron-moves/s, work per move of the reference "
+                "algorithm on this workload (n=27, n_k=14, N=54, N_G=257, acceptance p)",
+        "flops_per_move": f_move, "achieved": f_move * head["moves_per_s"] / 1e12,
+        "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        "frac": f_move * head["moves_per_s"] / 1e12 / PEAK_FP64_TFLOPS,
+        "lds_bytes_per_move": lds_move, "lds_achieved_TBps": lds_move * head["moves_per_s"] / 1e12,
+        "lds_peak_TBps": PEAK_LDS_TBPS,
+        "lds_frac": lds_move * head["moves_per_s"] / 1e12 / PEAK_LDS_TBPS}
+    return line
 
 
 def pct(xs, q):
@@ -142,6 +208,8 @@ def gpu_run(V, torch, precision, steps, warmup, reinvert, kernel):
     if kt["measure"][1]:
         ms = kt["measure"][0] / kt["measure"][1]
         f = measure_flops_per_walker(n, N_ELECTRONS, ng)
+        if precision == V.PREC_MIXED:
+            f += refine_flops_per_walker(n)
         out.update(measure_ms_avg=ms, measure_launches=kt["measure"][1],
                    measure_tflops=f * WALKERS / (ms * 1e-3) / 1e12, measure_flops_per_walker=f)
     if kt["reinvert"][1]:
@@ -151,11 +219,34 @@ def gpu_run(V, torch, precision, steps, warmup, reinvert, kernel):
                    reinvert_tflops=f * WALKERS / (ms * 1e-3) / 1e12)
     if kt["sweep"][1]:
         sweep_ms = kt["sweep"][0] / kt["sweep"][1]
-        f = sweep_flops_per_move(n, N_ELECTRONS, acc)
+        f = sweep_side_flops_per_move(n, nk0, N_ELECTRONS, acc)
         out.update(sweep_ms_avg=sweep_ms, sweep_launches=kt["sweep"][1],
                    sweep_tflops=f * WALKERS * N_ELECTRONS / (sweep_ms * 1e-3) / 1e12,
                    sweep_flops_per_move=f)
     return out
+
+
+def physics_run(V, torch, precision, seed, warm=500, meas=5000, block=100):
+    """The configuration's own run: N=54, r_s=2, b=1, 4096 walkers, `warm`
+    adaptive equilibration sweeps then `meas` measured sweeps (E_L every
+    sweep, blocks of `block`); energy per electron with its blocking error
+    (walker aggregation of the reference's main.cpp:79-104)."""
+    s = V.System.heg(N_ELECTRONS, RS, JB, precision=precision, reinvert_interval=256 if precision else 0)
+    stream = torch.cuda.Stream()
+    ens = V.Ensemble(s, WALKERS, seed, step0=0.3 * RS, stream=stream.cuda_stream)
+    ens.init_positions()
+    t0 = time.perf_counter()
+    ens.sweep(warm, adapt=True)
+    ens.reset_stats()
+    ens.sweep(meas, measure=True, block_size=block)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = ens.stats()
+    mean, se = V.blocking_summary(st)
+    acc = float(st[:, V.STAT["ACCEPTED"]].sum() / st[:, V.STAT["PROPOSED"]].sum())
+    ens.close()
+    return dict(seed=seed, sweeps=f"{warm}+{meas}", block=block, e_per_electron=mean / N_ELECTRONS,
+                se_per_electron=None if se is None else se / N_ELECTRONS, acceptance=acc, wall_s=wall)
 
 
 def cpu_baseline(target_s):
@@ -362,8 +453,6 @@ def main():
                                       V.KERNEL_GENERIC)
     head_name = "mixed" if "mixed" in res else "fp64"
     head = res[head_name]
-    peak = PEAK_FP32_TFLOPS if head_name == "mixed" else PEAK_FP64_TFLOPS
-    traffic = read_traffic() if head_name == "mixed" else None
     line = {
         "metric": "electron-moves/s at N=54, 4096 walkers, FP32 vs FP64; p50/p99 solve latency",
         "value": head["moves_per_s"],
@@ -375,13 +464,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if head_name == "mixed" else "f64",
+        "dtype": "f32/f64" if head_name == "mixed" else "f64",
         "data": "synthetic: Philox uniform starts per walker, adaptive equilibration sweeps",
         "config": {"workload": "HEG VMC sweep N=54 (27+27), r_s=2, b=1, 4096 walkers, "
                                "alpha=5/L, |n|^2<=25 (257 G), rcut=L/2, Gaussian moves, "
                                "E_L every sweep",
-                   "precision": "mixed: FP32 working set, FP64 accumulators and re-inversion"
-                   if head_name == "mixed" else "fp64",
+                   "precision": "mixed: FP32 moves (orbitals, ratios, Sherman-Morrison, Jastrow "
+                                "change), FP64 local energy from an FP64-refined inverse, FP64 "
+                                "accumulators" if head_name == "mixed" else "fp64",
                    "reinvert_interval": a.reinvert if head_name == "mixed" else 0,
                    "kernel": "register" if head["kernel"] == V.KERNEL_REGISTER else "generic"},
         "value_with_kernel_events": head["moves_per_s_with_events"],
@@ -390,32 +480,7 @@ def main():
                                "HIP events on the launch stream"},
     }
     if "measure_ms_avg" in head:
-        line["roofline"] = {
-            "kernel": "k_measure_rr (local energy of every walker)",
-            "bound": "mfma", "engine": "VALU-issue bound; peak is the dense figure of the "
-                                       "working precision",
-            "achieved": head["measure_tflops"], "peak": peak, "unit": "TFLOP/s",
-            "frac": head["measure_tflops"] / peak, "traffic": traffic,
-            "launch_ms_avg": head["measure_ms_avg"],
-            "flops_per_walker": head["measure_flops_per_walker"],
-            "sweep_kernel": {"launch_ms_avg": head["sweep_ms_avg"],
-                             "achieved": head["sweep_tflops"], "frac": head["sweep_tflops"] / peak,
-                             "flops_per_move": head["sweep_flops_per_move"]}}
-        acc_rate = head["acceptance"]
-        f_move, lds_move = path_work_per_move(27, 14, 54, 257, acc_rate)
-        line["roofline"]["path"] = {
-            "what": "whole path at the measured electron-moves/s, work per move of the reference "
-                    "algorithm on this workload (n=27, n_k=14, N=54, N_G=257, acceptance p)",
-            "flops_per_move": f_move, "achieved": f_move * head["moves_per_s"] / 1e12,
-            "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": f_move * head["moves_per_s"] / 1e12 / PEAK_FP32_TFLOPS,
-            "lds_bytes_per_move": lds_move, "lds_achieved_TBps": lds_move * head["moves_per_s"] / 1e12,
-            "lds_peak_TBps": PEAK_LDS_TBPS,
-            "lds_frac": lds_move * head["moves_per_s"] / 1e12 / PEAK_LDS_TBPS}
-        if "reinvert_ms_avg" in head:
-            line["roofline"]["reinvert_kernel"] = {
-                "launch_ms_avg": head["reinvert_ms_avg"], "achieved": head["reinvert_tflops"],
-                "peak": PEAK_FP64_TFLOPS, "frac": head["reinvert_tflops"] / PEAK_FP64_TFLOPS}
+        line["roofline"] = roofline_block(head, head_name)
     for k in ("fp64", "fp64_generic"):
         if k in res and head_name != k:
             r = res[k]
@@ -426,6 +491,7 @@ def main():
                 line[k].update(measure_ms_avg=r["measure_ms_avg"], sweep_ms_avg=r["sweep_ms_avg"],
                                measure_frac_fp64_peak=r["measure_tflops"] / PEAK_FP64_TFLOPS)
     if "mixed" in res and "fp64" in res:
+        # The target ratio: against the fastest FP64 kernel (register rows).
         line["mixed_over_fp64"] = res["mixed"]["moves_per_s"] / res["fp64"]["moves_per_s"]
         line["mixed_over_fp64_generic"] = (res["mixed"]["moves_per_s"] /
                                            res["fp64_generic"]["moves_per_s"])
@@ -440,11 +506,20 @@ def main():
                                             "40 energy evaluations through the request server, mixed")
         line["batch"] = batch_run(V, torch)
         line["configs"] = configs_run(V, torch)
-    line["physics"] = {k: {"e_per_electron": v["e_per_electron"],
-                           "se_per_electron": v["se_per_electron"],
-                           "acceptance": v["acceptance"],
-                           "reinversions_per_walker_step": v["reinversions_per_walker_step"],
-                           "drift_max": v["drift_max"]} for k, v in res.items()}
+    # Physics at the configuration's stated length, independent seeds.
+    phys = {}
+    if "mixed" in res:
+        phys["mixed"] = physics_run(V, torch, V.PREC_MIXED, SEED + 1)
+    if "fp64" in res:
+        phys["fp64"] = physics_run(V, torch, V.PREC_FP64, SEED + 2)
+    if len(phys) == 2 and phys["mixed"]["se_per_electron"] and phys["fp64"]["se_per_electron"]:
+        d = phys["mixed"]["e_per_electron"] - phys["fp64"]["e_per_electron"]
+        phys["difference_in_sigma"] = abs(d) / (phys["mixed"]["se_per_electron"] ** 2 +
+                                                phys["fp64"]["se_per_electron"] ** 2) ** 0.5
+    line["physics"] = phys
+    line["timed_runs"] = {k: {"acceptance": v["acceptance"],
+                              "reinversions_per_walker_step": v["reinversions_per_walker_step"],
+                              "drift_max": v["drift_max"]} for k, v in res.items()}
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
         ref = reference_cpu(min(a.cpu_seconds, 5.0))

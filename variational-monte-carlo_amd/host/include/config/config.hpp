@@ -87,6 +87,23 @@ public:
     // Re-validates and re-derives after fields were changed in code.
     void refresh() { compute_derived(); }
 
+    // Validates without re-deriving: a caller that set the step counts and
+    // the step size itself keeps them (the reference's Simulation uses the
+    // Config's fields as given, src/simulation/simulation.cpp:13-19,160-199).
+    void validate() const {
+        if (num_particles < 1U) throw std::invalid_argument("[Config] Num_Particles must be >= 1");
+        if (block_size < 1U) throw std::invalid_argument("[Config] Block_Size must be >= 1");
+        if (!(box_length > 0.0) || !std::isfinite(box_length))
+            throw std::invalid_argument("[Config] Box_Length must be finite and > 0");
+        if (measure_sweeps < 1U) throw std::invalid_argument("[Config] Measure_Sweeps must be >= 1");
+        if (unpolarised && num_particles % 2U != 0U)
+            throw std::invalid_argument("[Config] Spin = unpolarised needs an even Num_Particles");
+        if (num_threads < 1U && num_walkers < 1U)
+            throw std::invalid_argument("[Config] Num_Threads or Num_Walkers must be >= 1");
+        if (!(step_size >= 0.0) || !std::isfinite(step_size))
+            throw std::invalid_argument("[Config] step size must be finite and >= 0");
+    }
+
 private:
     void compute_derived() {
         if (num_particles < 1U) throw std::invalid_argument("[Config] Num_Particles must be >= 1");

@@ -69,7 +69,7 @@ Simulation::MeasurementSummary summary_of(const double *st, std::size_t bs) {
 
 Simulation::Simulation(Config cfg, std::unique_ptr<OutputWriter> output_writer)
     : config_{std::move(cfg)}, output_writer_{std::move(output_writer)} {
-    config_.refresh();
+    config_.validate();
 }
 
 Simulation::~Simulation() {
