@@ -201,6 +201,17 @@ int vmc_ensemble_sweep_given(vmc_ensemble *e, const double *proposals, const dou
  * are left alone. */
 int vmc_ensemble_measure(vmc_ensemble *e, double *parts, void *stream);
 
+/* vmc_ensemble_measure without any allocation: no hipMalloc, hipFree or
+ * allocating library call.  No counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid it).  The per-walker parts are staged in a
+ * device array of the library, one per GPU, held until the process ends:
+ * at most VMC_NOALLOC_MAX_WALKERS walkers (VMC_ERR_INVALID above that).
+ * Calls that overlap on one GPU share that array and race; serialise them.
+ * parts [W][VMC_PART_COUNT] host buffer, as vmc_ensemble_measure. */
+#define VMC_NOALLOC_MAX_WALKERS 65536
+int vmc_ensemble_measure_noalloc(vmc_ensemble *e, double *parts, void *stream);
+
 /* The walkers' transposed inverses as doubles, [W][n_up^2 + n_dn^2]
  * (invT[k][j] = (D^-1)[j][k], slater_plane_wave.cpp:312-314; spin up
  * first); mixed-precision inverses are widened from FP32. */

@@ -34,6 +34,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numbers>
 #include <string>
 #include <vector>
@@ -1089,19 +1090,16 @@ int vmc_ensemble_sweep_given(vmc_ensemble *e, const double *proposals, const dou
     return VMC_OK;
 }
 
-int vmc_ensemble_measure(vmc_ensemble *e, double *parts, void *stream) {
-    if (!e || !parts) return fail(VMC_ERR_INVALID, "NULL argument");
-    if (!e->positions_ready) return fail(VMC_ERR_STATE, "positions not initialised");
+}  // extern "C"
+
+namespace {
+
+// The measurement kernels of vmc_ensemble_measure writing every walker's
+// parts into d_parts [W][kPartCount] on stream s.  Allocates nothing: the
+// mixed-precision FP64 inverses and the re-inversion workspaces belong to
+// the ensemble.
+int launch_parts(vmc_ensemble *e, double *d_parts, hipStream_t s) {
     const DevSys &d = e->tables.dev;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    double *d_parts = nullptr;
-    const size_t n = static_cast<size_t>(e->W) * kPartCount;
-    int rc;
-    if ((rc = dev_alloc(&d_parts, n))) return rc;
-    struct Guard {
-        double *p;
-        ~Guard() { (void)hipFree(p); }
-    } guard{d_parts};
     const unsigned grid = static_cast<unsigned>(e->W);
     const int rb = e->tables.real_bytes;
     bool done = false;
@@ -1135,7 +1133,70 @@ int vmc_ensemble_measure(vmc_ensemble *e, double *parts, void *stream) {
         }
     }
     VMC_HIP_TRY(hipGetLastError());
+    return VMC_OK;
+}
+
+// Staging of vmc_ensemble_measure_noalloc: the parts of up to
+// kNoAllocWalkers walkers, one copy per device for the life of the process.
+constexpr int64_t kNoAllocWalkers = VMC_NOALLOC_MAX_WALKERS;
+static __device__ __align__(256) double scratch[kNoAllocWalkers * kPartCount];
+
+// The device address of `scratch` on each device, looked up once per device
+// with that device current (the array's host-side name is no device address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(double **out) {
+    int dev = -1;
+    VMC_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(VMC_ERR_INVALID, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        VMC_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(VMC_ERR_DEVICE, "no device address for the measurement staging array");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return VMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vmc_ensemble_measure(vmc_ensemble *e, double *parts, void *stream) {
+    if (!e || !parts) return fail(VMC_ERR_INVALID, "NULL argument");
+    if (!e->positions_ready) return fail(VMC_ERR_STATE, "positions not initialised");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double *d_parts = nullptr;
+    const size_t n = static_cast<size_t>(e->W) * kPartCount;
+    int rc;
+    if ((rc = dev_alloc(&d_parts, n))) return rc;
+    struct Guard {
+        double *p;
+        ~Guard() { (void)hipFree(p); }
+    } guard{d_parts};
+    if ((rc = launch_parts(e, d_parts, s))) return rc;
     VMC_HIP_TRY(hipMemcpyAsync(parts, d_parts, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    VMC_HIP_TRY(hipStreamSynchronize(s));
+    return VMC_OK;
+}
+
+int vmc_ensemble_measure_noalloc(vmc_ensemble *e, double *parts, void *stream) {
+    if (!e || !parts) return fail(VMC_ERR_INVALID, "NULL argument");
+    if (e->W > kNoAllocWalkers)
+        return fail(VMC_ERR_INVALID, "vmc_ensemble_measure_noalloc takes at most %lld walkers, ensemble has %lld",
+                    static_cast<long long>(kNoAllocWalkers), static_cast<long long>(e->W));
+    if (!e->positions_ready) return fail(VMC_ERR_STATE, "positions not initialised");
+    double *d_parts = nullptr;
+    int rc;
+    if ((rc = scratch_for_current_device(&d_parts))) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((rc = launch_parts(e, d_parts, s))) return rc;
+    VMC_HIP_TRY(hipMemcpyAsync(parts, d_parts, sizeof(double) * static_cast<size_t>(e->W) * kPartCount,
+                               hipMemcpyDeviceToHost, s));
     VMC_HIP_TRY(hipStreamSynchronize(s));
     return VMC_OK;
 }

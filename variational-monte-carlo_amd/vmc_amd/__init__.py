@@ -42,6 +42,7 @@ MV = dict(RATIO=0, DELTA_J=1, ACCEPTED=2)
 MV_COUNT = 3
 PART = dict(KINETIC=0, VREAL=1, VRECIP=2, ELOCAL=3, LOGDET0=4, LOGDET1=5, JASTROW=6)
 PART_COUNT = 7
+NOALLOC_MAX_WALKERS = 65536  # VMC_NOALLOC_MAX_WALKERS
 
 
 class VmcError(RuntimeError):
@@ -186,6 +187,7 @@ def lib():
             "vmc_ensemble_sweep": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _VP]),
             "vmc_ensemble_sweep_given": (C.c_int, [_VP, _dp, _dp, _dp, _VP]),
             "vmc_ensemble_measure": (C.c_int, [_VP, _dp, _VP]),
+            "vmc_ensemble_measure_noalloc": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_get_inverse": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_reset_stats": (C.c_int, [_VP, _VP]),
             "vmc_ensemble_set_kernel": (C.c_int, [_VP, C.c_int32]),
@@ -318,6 +320,14 @@ class Ensemble:
         arrays keyed like PART."""
         a = np.zeros((self.W, PART_COUNT))
         _check(lib().vmc_ensemble_measure(self._h, _ptr(a), self.stream))
+        return {k: a[:, i].copy() for k, i in PART.items()}
+
+    def measure_noalloc(self):
+        """measure() through vmc_ensemble_measure_noalloc: the same parts,
+        staged in the library's per-device array instead of an allocation
+        (at most NOALLOC_MAX_WALKERS walkers)."""
+        a = np.zeros((self.W, PART_COUNT))
+        _check(lib().vmc_ensemble_measure_noalloc(self._h, _ptr(a), self.stream))
         return {k: a[:, i].copy() for k, i in PART.items()}
 
     def inverse(self):
