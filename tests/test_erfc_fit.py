@@ -1,9 +1,11 @@
-"""The FP32 erfc of the local energy's real-space Ewald term (erfc_fit in
-vmc_device.hpp) against scipy's erfc: the polynomial coefficients in the
-source are the fit that fit_erfc.py makes, and the FP32 evaluation of
-exp(-x^2) P(t) stays within 3e-7 relative of erfc on 0 <= x <= 2.5 (the
-exponential is taken in float64 here; the device's v_exp_f32 adds about one
-rounding)."""
+"""The FP32 and FP64 erfc of the local energy's real-space Ewald term
+(erfc_fit, erfc_fit64 in vmc_device.hpp) against erfc: the polynomial
+coefficients in the source are the fits that fit_erfc.py makes; the FP32
+evaluation of exp(-x^2) P(t) stays within 3e-7 relative of erfc on
+0 <= x <= 2.5 (the exponential is taken in float64 here; the device's
+v_exp_f32 adds about one rounding), and the FP64 evaluation of
+exp(-x^2) Q(s) within 1e-15 (40-digit erfc as the reference; the device's
+exp adds about one rounding)."""
 import importlib.util
 import pathlib
 import re
@@ -50,3 +52,36 @@ def test_fp32_evaluation_tracks_erfc():
     for cc in bad[-2::-1]:
         q = (q * t + cc).astype(np.float32)
     assert (np.abs(np.exp(-x64 * x64) * q - erfc(x64)) / erfc(x64)).max() > 1e-5
+
+
+def fit_module():
+    spec = importlib.util.spec_from_file_location("fit_erfc", CSRC / "fit_erfc.py")
+    fit = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(fit)
+    return fit
+
+
+def source_coefficients64():
+    text = (CSRC / "vmc_device.hpp").read_text()
+    body = text[text.index("__device__ inline double erfc_fit64(double x)"):]
+    body = body[body.index("double q ="): body.index("const double h")]
+    vals = [float(v) for v in re.findall(r"([-+]?\d\.\d+(?:e[-+]\d+)?)\)?;", body)]
+    return vals[::-1]
+
+
+def test_fp64_coefficients_are_the_fit():
+    fit = fit_module()
+    want = fit.coefficients64()
+    got = source_coefficients64()
+    assert len(got) == 15
+    assert got == want
+
+
+def test_fp64_evaluation_tracks_erfc():
+    fit = fit_module()
+    c = source_coefficients64()
+    assert fit.fp64_error(c, points=401) < 1e-15
+    # A coefficient off in its eighth digit already fails.
+    bad = list(c)
+    bad[9] *= 1 + 1e-7
+    assert fit.fp64_error(bad, points=401) > 1e-13

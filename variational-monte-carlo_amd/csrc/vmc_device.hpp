@@ -64,6 +64,10 @@ struct DevState {
     // state handed to that sweep's re-inversion, which then re-inverts every
     // walker and writes its FP64 result here; null otherwise.
     double *inv64;
+    // With inv64 set: 1 when only inv64 is written (the measurement entry
+    // point, which leaves the walkers' state alone), 0 when the re-inversion
+    // also replaces the stored inverse and its bookkeeping (a measured sweep).
+    int inv64_only = 0;
 };
 
 // Caller-supplied moves of one sweep (vmc_ensemble_sweep_given): electron e
@@ -340,6 +344,21 @@ template <>
 __device__ __forceinline__ double wrcp<double>(double x) { return 1.0 / x; }
 template <>
 __device__ __forceinline__ float wrcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+// FP64 reciprocal and inverse square root from the hardware approximations
+// (v_rcp_f64, v_rsq_f64) with two Newton steps each: within about an ulp,
+// without the scaling and special-case paths of IEEE division and sqrt.
+// rcp_nr needs a finite non-zero argument, rsqrt_nr a positive normal one.
+__device__ __forceinline__ double rcp_nr(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    const double hx = 0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(y, fma(-hx * y, y, 0.5), y);
+    return fma(y, fma(-hx * y, y, 0.5), y);
+}
 template <typename Real>
 __device__ __forceinline__ Real wexp(Real x);
 template <>
@@ -389,11 +408,42 @@ __device__ inline float erfc_fit(float x) {
     return fmaf(e, ye * kLn2, e) * p;
 }
 
+// FP64 erfc for 0 <= x <= kErfcFitMax: erfc(x) = exp(-x^2) Q(s),
+// s = (x - 3) / (x + 3), Q the degree-14 interpolant of erfcx at the
+// Chebyshev nodes of the s interval (fit_erfc.py: Q within 6e-16 relative of
+// erfcx, exp(-x^2) Q(s) within 6e-16 of erfc with a correctly rounded exp).
+// One branch-free path for every lane, where the library erfc splits the
+// range into intervals that the lanes of a wavefront take in turn.  The
+// rounding of x^2 enters as a first-order correction of the exponential.
+__device__ inline double erfc_fit64(double x) {
+    const double s = (x - 3.0) * rcp_nr(x + 3.0);
+    double q = -5.626491863172647e-07;
+    q = fma(q, s, -6.1853504882410555e-06);
+    q = fma(q, s, -2.5954587494259244e-05);
+    q = fma(q, s, -3.812811629936757e-05);
+    q = fma(q, s, 3.882381513054981e-05);
+    q = fma(q, s, 2.6665540044490982e-05);
+    q = fma(q, s, -0.0006073342239491165);
+    q = fma(q, s, 0.0007034695201026614);
+    q = fma(q, s, 0.004267800284827522);
+    q = fma(q, s, -0.024392808541876164);
+    q = fma(q, s, 0.07166578534896151);
+    q = fma(q, s, -0.15011594257305944);
+    q = fma(q, s, 0.24560380124505782);
+    q = fma(q, s, -0.32623356006410514);
+    q = fma(q, s, 0.17900115118097112);
+    const double h = x * x, lo = fma(x, x, -h);  // x^2 = h + lo
+    const double e = exp(-h);
+    return fma(e, -lo, e) * q;
+}
+
 template <typename Real>
 __device__ inline Real erfc_r(Real x, int kind) {
     if (kind == 0) return erfc_as7126<Real>(x);
     if constexpr (sizeof(Real) == 4) {
         if (x >= 0.0f && x <= kErfcFitMax) return erfc_fit(x);
+    } else {
+        if (x >= 0.0 && x <= (double)kErfcFitMax) return erfc_fit64(x);
     }
     return erfc(x);
 }

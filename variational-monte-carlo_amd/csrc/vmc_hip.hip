@@ -416,13 +416,13 @@ __device__ void reinv_walker(const DevSys &s, const DevState &st, int w, bool a_
             singular = true;
             break;
         }
-        for (int e = t; e < n * n; e += nt) Y[e] = (Real)A[e];
+        for (int e = t; e < n * n && !st.inv64_only; e += nt) Y[e] = (Real)A[e];
         if (st.inv64)
             for (int e = t; e < n * n; e += nt) st.inv64[(size_t)w * s.ninv + s.inv_off[d] + e] = A[e];
         ld[d] = logdet;
         __syncthreads();
     }
-    if (t == 0) {
+    if (t == 0 && !st.inv64_only) {
         st.since[w] = 0;
         st.stats[(size_t)w * kStatCount + 15] += 1.0;
         if (singular) {
@@ -1109,10 +1109,12 @@ int launch_parts(vmc_ensemble *e, double *d_parts, hipStream_t s) {
         done = rb == 4 ? rrw_parts_exact_f32(d, e->st, grid, d_parts, e->inv64, s, ReinvLaunch{rb})
                        : rrw_parts_f64(d, e->st, grid, d_parts, s);
     if (!done && rb == 4) {
-        // Mixed precision, generic kernel: FP64 re-inversion of every walker,
-        // then the FP64 local energy from it, as a measured sweep does.
+        // Mixed precision, generic kernel: FP64 re-inversion of every walker
+        // into inv64 (the walkers' state untouched), then the FP64 local
+        // energy from it, as a measured sweep takes it.
         DevState st_refresh = e->st, st64 = e->st;
         st_refresh.inv64 = e->inv64;
+        st_refresh.inv64_only = 1;
         st64.inv = e->inv64;
         reinv_generic_launch(d, st_refresh, grid, s, rb);
         hipLaunchKernelGGL((k_measure_parts<double, false>), dim3(grid), dim3(static_cast<unsigned>(e->walker_threads)),

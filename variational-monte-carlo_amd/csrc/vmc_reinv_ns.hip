@@ -38,9 +38,10 @@ constexpr int kNSSteps = 6;
 // from d < 5e-4 one step reaches that floor.
 constexpr double kNSDone = 5e-4;
 // With an FP64 output (DevState::inv64: the FP64 local energy of a mixed
-// measured sweep) the steps go on until a step starts from d < 1e-6, whose
-// FP64 result Y + Y R has a residual of about n d^2 < 2e-10.
-constexpr double kNSExact = 1e-6;
+// measured sweep) the steps go on until a step starts from a drift d with
+// n d^2 <= kRefineTarget (the register kernels' in-kernel refinement rule):
+// its FP64 result Y + Y R has a residual of at most n d^2.
+__device__ __forceinline__ bool ns_exact_from(int n, double dm) { return (double)n * dm * dm <= kRefineTarget; }
 
 // R = I - Z takes E's storage once Z = E Y is in registers (E is rebuilt
 // for a further step, rarely needed), so a workgroup holds E and Y only:
@@ -205,7 +206,7 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
             ns_v4d y[T::kCount];
             ns_mm<NT, float, PF, float, PF>(sh.Y, R, y, wv, lane);
             float yn[T::kCount][4];
-            const bool exact = Y64 && dm < kNSExact;  // this step's FP64 result is the output
+            const bool exact = Y64 && ns_exact_from(n, dm);  // this step's FP64 result is the output
 #pragma unroll
             for (int i = 0; i < T::kCount; ++i) {
 #pragma unroll
@@ -235,7 +236,7 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
         }
         drift = (d0 > drift || d0 != d0) ? d0 : drift;
         if (ok) {
-            for (int e = t; e < n * n; e += kNSThreads) {
+            for (int e = t; e < n * n && !st.inv64_only; e += kNSThreads) {
                 const int i = e / n, j = e - (e / n) * n;
                 Yg[e] = sh.Y[i * PF + j];
             }
@@ -252,7 +253,7 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
                 break;
             }
             ld[d] = l;
-            for (int e = t; e < n * n; e += kNSThreads) Yg[e] = (float)sh.E[e];
+            for (int e = t; e < n * n && !st.inv64_only; e += kNSThreads) Yg[e] = (float)sh.E[e];
             if (Y64)
                 for (int e = t; e < n * n; e += kNSThreads) Y64[e] = sh.E[e];
         }
@@ -262,7 +263,7 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
     // count (a forced re-inversion clears it here); the drift is an atomic
     // maximum (a non-negative double, or +inf / NaN, orders as its bit
     // pattern).
-    if (t == 0) {
+    if (t == 0 && !st.inv64_only) {
         if (force) st.since[w] = 0;
         if (d_only == 0) st.stats[(size_t)w * kStatCount + 15] += 1.0;
         if (singular) atomicOr(&st.flags[w], kFlagSingular);
@@ -392,7 +393,7 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
             float yn[H * H][4];
             int yl = rl * PF + cl;
             asm volatile("" : "+v"(yl));
-            const bool exact = Y64 && dm < kNSExact;  // this step's FP64 result is the output
+            const bool exact = Y64 && ns_exact_from(n, dm);  // this step's FP64 result is the output
 #pragma unroll
             for (int i = 0; i < H * H; ++i)
 #pragma unroll
@@ -415,7 +416,7 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
         }
         drift = (d0 > drift || d0 != d0) ? d0 : drift;
         if (ok) {
-            for (int e = t; e < n * n; e += kNSThreads) {
+            for (int e = t; e < n * n && !st.inv64_only; e += kNSThreads) {
                 const int i = e / n, j = e - (e / n) * n;
                 Yg[e] = sh.Y[i * PF + j];
             }
@@ -432,13 +433,13 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
                 break;
             }
             ld[d] = l;
-            for (int e = t; e < n * n; e += kNSThreads) Yg[e] = (float)E[e];
+            for (int e = t; e < n * n && !st.inv64_only; e += kNSThreads) Yg[e] = (float)E[e];
             if (Y64)
                 for (int e = t; e < n * n; e += kNSThreads) Y64[e] = E[e];
         }
         __syncthreads();
     }
-    if (t == 0) {
+    if (t == 0 && !st.inv64_only) {
         st.since[w] = 0;
         st.stats[(size_t)w * kStatCount + 15] += 1.0;
         if (singular) {

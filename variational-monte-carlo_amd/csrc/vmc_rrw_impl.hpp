@@ -140,14 +140,15 @@ bool rrw_parts(const DevSys &d, const DevState &st, unsigned grid, double *parts
 }
 
 // The mixed-precision parts: every walker re-inverted with its FP64 result
-// in inv64 (the stored FP32 inverse refreshed too), then the FP64 local
-// energy's parts from it.
+// in inv64 (the walkers' state, the stored FP32 inverse among it, left
+// alone), then the FP64 local energy's parts from it.
 template <typename Real>
 bool rrw_parts_exact(const DevSys &d, const DevState &st, unsigned grid, double *parts, double *inv64,
                      hipStream_t stream, const ReinvLaunch &reinv) {
     if (!rrw_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng, (int)sizeof(Real))) return false;
     DevState st_refresh = st, st64 = st;
     st_refresh.inv64 = inv64;
+    st_refresh.inv64_only = 1;
     st64.inv = inv64;
     reinv.launch(d, st_refresh, grid, stream);
     return rrw_measure64(d, st64, grid, 1, parts, stream);

@@ -480,6 +480,29 @@ struct RRConst {
         const Real inv_r = (r < (Real)1e-12) ? (Real)1 : wrcp<Real>(r);
         return erfc_r<Real>(alpha * r, erfc_kind) * inv_r;
     }
+    // The same with inv_r from pair_geom.
+    __device__ Real ewald_ri(Real r, Real inv_r) const {
+        if (use_cut && !(r < rcut)) return (Real)0;
+        return erfc_r<Real>(alpha * r, erfc_kind) * inv_r;
+    }
+    // Pair distance r from its square, 1/r (1 below r = 1e-12, the real-space
+    // Ewald guard; the Jastrow derivatives skip those pairs) and the Pade
+    // denominator 1 / (1 + b r).  FP64 takes both reciprocals from rsqrt_nr
+    // and rcp_nr; a pair closer than 1e-12 gets r = 0, which changes
+    // erfc(alpha r) by under 2 alpha 1e-12.
+    __device__ void pair_geom(Real r2, Real &r, Real &inv_r, Real &den) const {
+        if constexpr (sizeof(Real) == 8) {
+            const bool tiny = !(r2 >= 1e-24);
+            const double y = rsqrt_nr(tiny ? 1.0 : r2);
+            r = tiny ? 0.0 : r2 * y;
+            inv_r = tiny ? 1.0 : y;
+            den = rcp_nr(1.0 + b * r);
+        } else {
+            r = wsqrt<Real>(r2);
+            inv_r = (r < (Real)1e-12) ? (Real)1 : wrcp<Real>(r);
+            den = wrcp<Real>((Real)1 + b * r);
+        }
+    }
 };
 
 // In-LDS FP64 Gauss-Jordan inversion of the n x n matrix A (A = D^T on entry,
@@ -613,14 +636,13 @@ struct RRKernel {
                     const Real dx = c.mi_r(L.x, xj);
                     const Real dy = c.mi_r(L.y, yj);
                     const Real dz = c.mi_r(L.z, zj);
-                    const Real r = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+                    Real r, inv_r, den;
+                    c.pair_geom(dx * dx + dy * dy + dz * dz, r, inv_r, den);
                     const Real a = ((lj >> 5) == h) ? c.a_same : c.a_anti;
-                    const Real den = wrcp<Real>((Real)1 + c.b * r);
-                    const Real v = c.ewald(r);
+                    const Real v = c.ewald_ri(r, inv_r);
                     v_c += last_even ? (Real)0.5 * v : v;
                     if (r >= (Real)1e-12) {
                         const Real d1 = a * den * den;
-                        const Real inv_r = wrcp<Real>(r);
                         const Real gf = d1 * inv_r;
                         const Real l = d1 * (m2b * den + (Real)2 * inv_r);
                         gx_c += gf * dx;
@@ -1524,7 +1546,12 @@ struct RRMixedShared {
     __device__ double *rows(int h) { return buf + h * NMAX * P; }
 };
 
-constexpr double kRefineTarget = 1e-10;  // n max|E|^2 after the last step
+// A refinement step Y' = Y + Y E from the residual E = I - D Y leaves the
+// residual E^2, whose entries are at most n max|E|^2: the steps stop once
+// that bound is below 1e-8, which keeps the local energy's share of the
+// inverse error about two orders under 1e-6 of |E_L|.  After a sweep from
+// a refined inverse one step is the rule (max|E| of order 1e-6).
+constexpr double kRefineTarget = 1e-8;
 
 // Refines lane (h, r)'s row of the FP32 inverse, held widened in L.inv, to
 // FP64 accuracy in place.  Returns the residual max|I - D Y| of the FP32

@@ -672,15 +672,14 @@ __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState
                 const Real dx = c.mi_r(x, sh.pos[0][j]);
                 const Real dy = c.mi_r(y, sh.pos[1][j]);
                 const Real dz = c.mi_r(z, sh.pos[2][j]);
-                const Real r = wsqrt<Real>(dx * dx + dy * dy + dz * dz);
+                Real r, inv_r, den;
+                c.pair_geom(dx * dx + dy * dy + dz * dz, r, inv_r, den);
                 const Real a = ((ND > 0 && j >= NU) ? 1 : 0) == T.h ? c.a_same : c.a_anti;
-                const Real den = wrcp<Real>((Real)1 + c.b * r);
-                const Real v = c.ewald(r);
+                const Real v = c.ewald_ri(r, inv_r);
                 v_c += last_even ? (Real)0.5 * v : v;
                 Real pgx = 0, pgy = 0, pgz = 0, pl = 0;
                 if (r >= (Real)1e-12) {
                     const Real d1 = a * den * den;
-                    const Real inv_r = wrcp<Real>(r);
                     const Real gf = d1 * inv_r;
                     const Real l = d1 * (m2b * den + (Real)2 * inv_r);
                     gx_c += gf * dx;
