@@ -19,6 +19,10 @@ constexpr int kWave = 64;
 constexpr int kStatCount = 16;
 
 // ------------------------------------------------------------------ system
+// Steps of one structure-factor column segment (DevSys::sfseg): a fixed count
+// per thread, absent steps carrying zero weights.
+constexpr int kSfSegSteps = 4;
+
 struct DevSys {
     int N, n_up, n_dn, ndet;
     int n[2];        // electrons (= orbitals) per determinant
@@ -41,6 +45,11 @@ struct DevSys {
     const int4 *orb;     // [ndet][nmax] = {nx, ny, nz, type (0 cos, 1 sin)}
     const int4 *gvec;    // [ng] = {mx, my, mz, 0}
     const double *gw;    // [ng] weights 8 pi^2 / G^2 exp(-G^2 / 4 alpha^2)
+    // The same G vectors as column segments (vmc_tables.hpp sf_segments):
+    // [nseg] = {mx, my, a0, steps} and [nseg][kSfSegSteps][2] weights.
+    int nseg;
+    const int4 *sfseg;
+    const double *sfw;
 };
 
 // Per-ensemble device state (walker-major SoA in HBM).

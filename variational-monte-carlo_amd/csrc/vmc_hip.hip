@@ -614,12 +614,16 @@ struct SystemTables {
     int4 *d_orb = nullptr;
     int4 *d_gvec = nullptr;
     double *d_gw = nullptr;
+    int4 *d_sfseg = nullptr;
+    double *d_sfw = nullptr;
     int real_bytes = 8;
 
     ~SystemTables() {
         if (d_orb) (void)hipFree(d_orb);
         if (d_gvec) (void)hipFree(d_gvec);
         if (d_gw) (void)hipFree(d_gw);
+        if (d_sfseg) (void)hipFree(d_sfseg);
+        if (d_sfw) (void)hipFree(d_sfw);
     }
 };
 
@@ -712,6 +716,18 @@ int build_tables(const vmc_system *sys, uint64_t seed, SystemTables &t) {
     d.orb = t.d_orb;
     d.gvec = t.d_gvec;
     d.gw = t.d_gw;
+    const SfSegments sg = sf_segments(g);
+    std::vector<int4> seg(sg.seg.size());
+    for (size_t i = 0; i < seg.size(); ++i) seg[i] = make_int4(sg.seg[i][0], sg.seg[i][1], sg.seg[i][2], sg.seg[i][3]);
+    VMC_HIP_TRY(hipMalloc(&t.d_sfseg, sizeof(int4) * std::max<size_t>(seg.size(), 1)));
+    VMC_HIP_TRY(hipMalloc(&t.d_sfw, sizeof(double) * std::max<size_t>(sg.w.size(), 1)));
+    if (!seg.empty()) {
+        VMC_HIP_TRY(hipMemcpy(t.d_sfseg, seg.data(), sizeof(int4) * seg.size(), hipMemcpyHostToDevice));
+        VMC_HIP_TRY(hipMemcpy(t.d_sfw, sg.w.data(), sizeof(double) * sg.w.size(), hipMemcpyHostToDevice));
+    }
+    d.nseg = static_cast<int>(seg.size());
+    d.sfseg = t.d_sfseg;
+    d.sfw = t.d_sfw;
     return VMC_OK;
 }
 

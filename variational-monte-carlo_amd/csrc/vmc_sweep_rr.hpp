@@ -370,6 +370,74 @@ __device__ double energy_g(const DevSys &s, Pair<Real> *tab, int tid, PosOf pos_
     return acc;
 }
 
+// sum_G w_G |S_G|^2 by column segments (DevSys::sfseg; thread t takes
+// segments t, t + NT, ...): per staged electron two LDS reads give
+// X^mx Y^my, a third Z^a0, and Z^1 is a broadcast; the steps a0, a0 + 1, ...
+// multiply Z^a on, each adding X^mx Y^my Z^a to S(mx, my, a) and its
+// conjugate-in-z partner X^mx Y^my Z^-a to S(mx, my, -a) with four fused
+// accumulations each.  Against one G vector per slot this reads a third of
+// the LDS bytes per electron.  The thread's share is reduced by the caller.
+template <typename Real, int N, int NT, typename PosOf>
+__device__ double energy_cols(const DevSys &s, Pair<Real> *tab, int tid, PosOf pos_of) {
+    constexpr int P = kPitch;
+    constexpr int S = kSfSegSteps;
+    constexpr int kChunks = (N + kRRChunk - 1) / kRRChunk;
+    constexpr int oz1 = (kZ0 + 1) * P;
+    double acc = 0.0;
+#pragma unroll 1
+    for (int p0 = 0; p0 < s.nseg; p0 += NT) {
+        int sg = p0 + tid;
+        asm volatile("" : "+v"(sg));  // the segment reads stay in this pass
+        const bool act = sg < s.nseg;
+        const int4 G = act ? s.sfseg[sg] : make_int4(0, 0, 0, 0);
+        const int ox = G.x * P, oy = (kY0 + G.y) * P, oz = (kZ0 + G.z) * P;
+        Real pr[S], pi[S], mr[S], mi[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) pr[k] = pi[k] = mr[k] = mi[k] = (Real)0;
+#pragma unroll 1
+        for (int c = 0; c < kChunks; ++c) {
+            __syncthreads();
+            if (tid < 3 * kRRChunk) {
+                const int i = tid / 3, ax = tid - 3 * i, e = c * kRRChunk + i;
+                stage_axis(tab, i, ax, e < N, e < N ? pos_of(ax, e) : 0.0, s.inv_L);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < kRRChunk; ++i) {
+                const Pair<Real> X = tab[ox + i], Y = tab[oy + i], Z = tab[oz1 + i];
+                Pair<Real> Zp = tab[oz + i];
+                const Real xr = X.re * Y.re - X.im * Y.im, xi = X.re * Y.im + X.im * Y.re;
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    pr[k] = fma(xr, Zp.re, pr[k]);
+                    pr[k] = fma(-xi, Zp.im, pr[k]);
+                    pi[k] = fma(xr, Zp.im, pi[k]);
+                    pi[k] = fma(xi, Zp.re, pi[k]);
+                    mr[k] = fma(xr, Zp.re, mr[k]);
+                    mr[k] = fma(xi, Zp.im, mr[k]);
+                    mi[k] = fma(xi, Zp.re, mi[k]);
+                    mi[k] = fma(-xr, Zp.im, mi[k]);
+                    if (k + 1 < S) {
+                        const Real t = Zp.re * Z.re - Zp.im * Z.im;
+                        Zp.im = Zp.re * Z.im + Zp.im * Z.re;
+                        Zp.re = t;
+                    }
+                }
+                if (i % 2 == 1) __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        if (act) {
+            const double *w = s.sfw + (size_t)sg * 2 * S;
+#pragma unroll
+            for (int k = 0; k < S; ++k)
+                acc += w[2 * k] * ((double)pr[k] * (double)pr[k] + (double)pi[k] * (double)pi[k]) +
+                       w[2 * k + 1] * ((double)mr[k] * (double)mr[k] + (double)mi[k] * (double)mi[k]);
+        }
+    }
+    __syncthreads();
+    return acc;
+}
+
 // As energy_g with GPT slots per thread, and with fewer when the G set of the
 // Ewald sum with alpha = 5/L (at most 5 per wavefront lane) fits them.
 template <typename Real, int N, int GPT, int NT, typename PosOf>
@@ -460,14 +528,16 @@ struct RRConst {
     __device__ double mi(double d) const {
         return d + L * (double)(d <= -half_L) - L * (double)(d > half_L);
     }
-    // Minimum-image separation a - b in the working precision: FP64 is the
-    // reference's masked form (jastrow_pade.cpp:18-24); FP32 rounds the FP64
-    // difference once and subtracts the nearest multiple of L (three VALU
-    // operations, no compare-select hazards).  The two differ only at
-    // |a - b| = L/2 exactly, where both are minimum images of equal length.
+    // Minimum-image separation a - b in the working precision: the
+    // difference less the nearest multiple of L (three VALU operations, no
+    // compare-select hazards), in FP32 from the FP64 difference rounded once.
+    // Against the reference's masked form (jastrow_pade.cpp:18-24) it
+    // differs only where |a - b| rounds to L/2, where both are minimum images
+    // whose lengths agree to an ulp.
     __device__ Real mi_r(double a, double b) const {
         if constexpr (sizeof(Real) == 8) {
-            return (Real)mi(a - b);
+            const double d = a - b;
+            return fma(-L, rint(d * inv_L), d);
         } else {
             const Real d = (Real)(a - b);
             return __builtin_fmaf(-(Real)L, __builtin_rintf(d * (Real)inv_L), d);
@@ -616,7 +686,7 @@ struct RRKernel {
         const Real m2b = (Real)-2 * c.b;
         constexpr int K = N / 2;
         constexpr bool kEven = N % 2 == 0;
-        constexpr int kPairChunk = sizeof(Real) == 4 ? 8 : 1;
+        constexpr int kPairChunk = sizeof(Real) == 4 ? 8 : 2;
         const int e = h ? NU + (lane & 31) : (lane & 31);
         for (int k0 = 1; k0 <= K; k0 += kPairChunk) {
             Real gx_c = 0, gy_c = 0, gz_c = 0, l_c = 0, v_c = 0;
@@ -776,7 +846,7 @@ struct RRKernel {
         auto *tab = reinterpret_cast<sf::Pair<Real> *>(sh.stage());
         // Electron e sits in lane e (up) or 32 + e - NU (down).
         const auto pos_of = [&sh](int ax, int e) { return sh.pos[ax][e < NU ? e : 32 + e - NU]; };
-        return rr::wave_total(sf::energy<Real, N, GPL, kWave>(s, tab, lane, pos_of));
+        return rr::wave_total(sf::energy_cols<Real, N, kWave>(s, tab, lane, pos_of));
     }
 
     // Metropolis test u < R^2 exp(2 dJ) (simulation.cpp:125-131).  The
@@ -1598,13 +1668,10 @@ __device__ double rr_refine(const DevSys &s, RRMixedShared<NU, ND> &sh, RRLane<d
 #pragma unroll 1
         for (int i = 0; i < NMAX; ++i) {
             const double *Di = sh.rows(h) + i * P;
-            double a0 = 0.0, a1 = 0.0;
+            double a[4] = {0.0, 0.0, 0.0, 0.0};  // four independent chains
 #pragma unroll
-            for (int j = 0; j < NMAX; j += 2) {
-                a0 = fma(Di[j], L.inv[j], a0);
-                if (j + 1 < NMAX) a1 = fma(Di[j + 1], L.inv[j + 1], a1);
-            }
-            const double ev = (i == r ? 1.0 : 0.0) - (a0 + a1);
+            for (int j = 0; j < NMAX; ++j) a[j & 3] = fma(Di[j], L.inv[j], a[j & 3]);
+            const double ev = (i == r ? 1.0 : 0.0) - ((a[0] + a[1]) + (a[2] + a[3]));
             if (has_e) sh.et[h * NMAX + r][i] = (float)ev;
             if (has_e && i < nh) {
                 dm = fabs(ev) > dm ? fabs(ev) : dm;

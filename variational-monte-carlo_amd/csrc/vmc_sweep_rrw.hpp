@@ -622,7 +622,6 @@ __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState
     constexpr int N = K::N;
     constexpr int NMAX = K::NMAX;
     constexpr int NT = K::NT;
-    constexpr int GPT = (kRRMaxGPerLane * kWave + NT - 1) / NT;  // G slots per thread
     __shared__ typename K::Sh sh;
     const RRConst<Real> c(s);
     const typename K::Thread T;
@@ -759,7 +758,7 @@ __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState
                       sizeof(sh.prop) + sizeof(sh.pph) + sizeof(sh.tab) >= sizeof(Real) * sf::table_reals<Real>(),
                   "structure-factor tables exceed the run prop | pph | tab");
     const auto pos_of = [](int ax, int e) { return sh.pos[ax][e]; };
-    const double acc = sf::energy<Real, N, GPT, NT>(s, reinterpret_cast<sf::Pair<Real> *>(&sh.prop[0][0]),
+    const double acc = sf::energy_cols<Real, N, NT>(s, reinterpret_cast<sf::Pair<Real> *>(&sh.prop[0][0]),
                                                     (int)threadIdx.x, pos_of);
     const double vrecip = __dmul_rn(s.pref, K::block_total(acc, sh.red));  // (uncontracted, as in every kernel)
     if (parts) {

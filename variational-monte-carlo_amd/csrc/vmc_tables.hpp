@@ -6,9 +6,13 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <map>
 #include <numbers>
 #include <tuple>
+#include <utility>
 #include <vector>
+
+#include "vmc_device.hpp"
 
 namespace vmc {
 
@@ -86,6 +90,46 @@ inline GTable ewald_gvectors(double L, double alpha, int nmax2) {
                 t.w.push_back(8.0 * pi * pi / g2 * std::exp(-g2 / four_a2));
             }
     return t;
+}
+
+// The G vectors of a half-space set regrouped for the structure factor by
+// column (mx, my): a segment holds |mz| = a0 .. a0 + steps - 1 of one column,
+// steps <= kSfSegSteps and consecutive, each step covering (mx, my, +a) and
+// (mx, my, -a) where the set holds them.  seg = {mx, my, a0, steps}; w holds
+// per segment and step the weights of +a and -a (0 where absent; -a is
+// absent for a = 0 and for the column mx = my = 0, which the half-space set
+// holds for mz > 0 only).
+struct SfSegments {
+    std::vector<std::array<int, 4>> seg;
+    std::vector<double> w;  // [seg][kSfSegSteps][2]
+};
+
+inline SfSegments sf_segments(const GTable &g) {
+    // column -> a -> weights of +a, -a
+    std::map<std::pair<int, int>, std::map<int, std::array<double, 2>>> cols;
+    for (size_t i = 0; i < g.m.size(); ++i) {
+        const auto &m = g.m[i];
+        auto &slot = cols[{m[0], m[1]}][std::abs(m[2])];
+        slot[m[2] < 0 ? 1 : 0] = g.w[i];
+    }
+    SfSegments out;
+    for (const auto &[col, steps] : cols) {
+        auto it = steps.begin();
+        while (it != steps.end()) {
+            const int a0 = it->first;
+            std::array<double, 2 * kSfSegSteps> w{};
+            int n = 0;
+            while (it != steps.end() && n < kSfSegSteps && it->first == a0 + n) {
+                w[2 * n] = it->second[0];
+                w[2 * n + 1] = it->second[1];
+                ++n;
+                ++it;
+            }
+            out.seg.push_back({col.first, col.second, a0, n});
+            out.w.insert(out.w.end(), w.begin(), w.end());
+        }
+    }
+    return out;
 }
 
 }  // namespace vmc
