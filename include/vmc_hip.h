@@ -193,6 +193,17 @@ enum {
 int vmc_ensemble_sweep_given(vmc_ensemble *e, const double *proposals, const double *uniforms, double *trace,
                              void *stream);
 
+/* One sweep of every walker (as vmc_ensemble_sweep with n_sweeps = 1, on the
+ * generic kernel) with walker `walker`'s local energy evaluated after each of
+ * its moves: the reference's measurement cadence, E_L after every step
+ * (Simulation::measure, src/simulation/simulation.cpp:203-243), for the walker
+ * whose frames the host writes.  Per move t in sweep order (host buffers of
+ * N entries): e_local[t], moved[t] (the electron), accepted[t] (0 or 1), and
+ * when pos is not NULL the walker's positions after the move, pos[t][N][3].
+ * FP64 systems only (VMC_ERR_INVALID otherwise). */
+int vmc_ensemble_sweep_traced(vmc_ensemble *e, int32_t flags, int32_t block_size, int64_t walker,
+                              double *e_local, int32_t *moved, int32_t *accepted, double *pos, void *stream);
+
 /* The local energy and its parts, log|det| and J of every walker at its
  * current positions, evaluated by the local-energy kernel the ensemble's
  * measured sweeps use (WaveFunction::evaluate_derivatives +

@@ -1,7 +1,7 @@
 """The C++23 facade end to end on the GPU: the vmc_gpu command line (the
 reference's src/main.cpp flow) reads a config.cfg, runs the ensemble through
 Simulation::run() and streams walker 0 to output/vmc.bin in the reference's
-binary format.  Its energy is checked against a long run of the oracle's
+binary format, a frame per step.  Its energy is checked against a long run of the oracle's
 restatement of the reference's own Simulation (same physics: one polarised
 determinant, alpha = 6/L, A&S erfc, uniform moves), within the combined
 blocking error bars.
@@ -38,9 +38,10 @@ def test_cli_reference_physics_matches_reference_simulation(oracle, tmp_path):
                                    f"Block_Size = 70\nMaster_Seed = 5\n", "--no-optimize")
     b = (tmp_path / "output" / "vmc.bin").read_bytes()
     assert struct.unpack_from("<QdQ", b, 0) == (N, L, N * sweeps)
+    # FP64 streams a frame per step (the reference's cadence).
     frame = 4 + 3 * N
-    assert len(b) == 24 + sweeps * frame * 8
-    frames = np.frombuffer(b, dtype="<f8", offset=24).reshape(sweeps, frame)
+    assert len(b) == 24 + N * sweeps * frame * 8
+    frames = np.frombuffer(b, dtype="<f8", offset=24).reshape(N * sweeps, frame)
     pos = frames[:, 4:]
     assert np.all(pos >= 0.0) and np.all(pos < L)
     assert np.all((frames[:, 3] > 0.0) & (frames[:, 3] <= 1.0))

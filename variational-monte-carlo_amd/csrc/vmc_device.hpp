@@ -88,6 +88,18 @@ struct SweepGiven {
     const double *u;
     double *trace;
 };
+
+// The reference's measurement cadence for one walker (vmc_ensemble_sweep_traced):
+// after each of walker `walker`'s moves, in sweep order t, its local energy
+// e_local[t], the moved electron moved[t], the decision accepted[t] and, with
+// pos set, its positions pos[t][N][3].  walker < 0: no trace.
+struct StepTrace {
+    long long walker;
+    double *e_local;
+    int *moved;
+    int *accepted;
+    double *pos;
+};
 constexpr int kTraceCount = 3;  // VMC_MV_COUNT
 
 // Per-walker parts of the local energy written by a measurement launched

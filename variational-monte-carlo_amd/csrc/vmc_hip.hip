@@ -107,7 +107,8 @@ __global__ __launch_bounds__(kWave) void k_init(DevSys s, DevState st, int draw)
 template <typename Real>
 __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kInvLds,
                              int n_sweeps, unsigned long long sweep0, int flags, int block_size,
-                             unsigned char *smem, SweepGiven given = SweepGiven{}) {
+                             unsigned char *smem, SweepGiven given = SweepGiven{},
+                             StepTrace trace = StepTrace{-1, nullptr, nullptr, nullptr, nullptr}) {
     const int lane = threadIdx.x;
     const int N = s.N;
     Real *inv_hbm = reinterpret_cast<Real *>(st.inv) + (size_t)w * s.ninv;
@@ -202,6 +203,25 @@ __device__ void sweep_walker(const DevSys &s, const DevState &st, int w, bool kI
                 if (d == 0) ld0 += log(fabs(pr.ratio)); else ld1 += log(fabs(pr.ratio));
                 ++since;  // the FP64 re-inversion follows the sweep (k_reinv_generic)
             }
+            if ((long long)w == trace.walker) {
+                // E_L after every step (simulation.cpp:212-221); measure reads
+                // the positions and the inverse only, so the chain goes on as
+                // it would untraced.
+                wave_sync();
+                const Measured m = measure<Real>(s, v, lane, nullptr, nullptr);
+                if (lane == 0) {
+                    trace.e_local[t] = m.e_local;
+                    trace.moved[t] = p;
+                    trace.accepted[t] = accept;
+                }
+                for (int i = lane; i < N && trace.pos; i += (int)blockDim.x) {
+                    double *q = trace.pos + ((size_t)t * N + i) * 3;
+                    q[0] = v.px[i];
+                    q[1] = v.py[i];
+                    q[2] = v.pz[i];
+                }
+                wave_sync();
+            }
         }
         stat[6] += (double)acc;
         stat[7] += (double)N;
@@ -242,6 +262,14 @@ __global__ __launch_bounds__(256) void k_sweep(DevSys s, DevState st, int n_swee
                                                  int block_size) {
     extern __shared__ __align__(16) unsigned char smem[];
     sweep_walker<Real>(s, st, blockIdx.x, kInvLds, n_sweeps, sweep0, flags, block_size, smem);
+}
+
+// One sweep with walker trace.walker's local energy after each of its moves.
+template <bool kInvLds>
+__global__ __launch_bounds__(256) void k_sweep_traced(DevSys s, DevState st, unsigned long long sweep0, int flags,
+                                                      int block_size, StepTrace trace) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    sweep_walker<double>(s, st, blockIdx.x, kInvLds, 1, sweep0, flags, block_size, smem, SweepGiven{}, trace);
 }
 
 // One sweep with the caller's moves, each recorded.
@@ -1048,6 +1076,51 @@ int vmc_ensemble_sweep(vmc_ensemble *e, int32_t n_sweeps, int32_t flags, int32_t
                                        : launch_sweep<float>(e, n_sweeps, flags, block_size, s);
     if (rc) return rc;
     e->sweeps_done += static_cast<uint64_t>(n_sweeps);
+    return VMC_OK;
+}
+
+int vmc_ensemble_sweep_traced(vmc_ensemble *e, int32_t flags, int32_t block_size, int64_t walker,
+                              double *e_local, int32_t *moved, int32_t *accepted, double *pos, void *stream) {
+    if (!e || !e_local || !moved || !accepted) return fail(VMC_ERR_INVALID, "NULL argument");
+    if (!e->positions_ready) return fail(VMC_ERR_STATE, "positions not initialised");
+    if (e->in_batch) return fail(VMC_ERR_STATE, "ensemble belongs to a batch; sweep the batch");
+    if (e->tables.real_bytes != 8) return fail(VMC_ERR_INVALID, "per-move local energies need an FP64 system");
+    if (walker < 0 || walker >= e->W) return fail(VMC_ERR_INVALID, "walker %lld out of range", (long long)walker);
+    if (block_size < 1) return fail(VMC_ERR_INVALID, "block_size must be >= 1");
+    if (flags & ~(VMC_SWEEP_MEASURE | VMC_SWEEP_ADAPT)) return fail(VMC_ERR_INVALID, "unknown sweep flags");
+    const DevSys &d = e->tables.dev;
+    const size_t N = static_cast<size_t>(d.N);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double *d_e = nullptr, *d_pos = nullptr;
+    int *d_mv = nullptr, *d_acc = nullptr;
+    struct Guard {
+        std::vector<void *> p;
+        ~Guard() { for (void *q : p) if (q) (void)hipFree(q); }
+    } guard;
+    int rc;
+    if ((rc = dev_alloc(&d_e, N)) || (rc = dev_alloc(&d_mv, N)) || (rc = dev_alloc(&d_acc, N)) ||
+        (pos && (rc = dev_alloc(&d_pos, N * N * 3)))) {
+        guard.p = {d_e, d_mv, d_acc, d_pos};
+        return rc;
+    }
+    guard.p = {d_e, d_mv, d_acc, d_pos};
+    const StepTrace tr{static_cast<long long>(walker), d_e, d_mv, d_acc, d_pos};
+    const size_t lds_in = lds_for(d, 8, true);
+    const bool inv_lds = lds_in <= kMaxLdsBytes && e->inv_placement != 2;
+    const size_t lds = inv_lds ? lds_in : lds_for(d, 8, false);
+    if (lds > kMaxLdsBytes) return fail(VMC_ERR_INVALID, "system too large for one walker per wavefront");
+    const dim3 grid(static_cast<unsigned>(e->W)), block(static_cast<unsigned>(e->walker_threads));
+    const unsigned long long s0 = e->sweeps_done;
+    if (inv_lds) hipLaunchKernelGGL((k_sweep_traced<true>), grid, block, lds, s, d, e->st, s0, flags, block_size, tr);
+    else hipLaunchKernelGGL((k_sweep_traced<false>), grid, block, lds, s, d, e->st, s0, flags, block_size, tr);
+    if (d.reinv_k > 0) reinv_generic_launch(d, e->st, grid.x, s, 8);
+    VMC_HIP_TRY(hipGetLastError());
+    VMC_HIP_TRY(hipMemcpyAsync(e_local, d_e, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    VMC_HIP_TRY(hipMemcpyAsync(moved, d_mv, sizeof(int) * N, hipMemcpyDeviceToHost, s));
+    VMC_HIP_TRY(hipMemcpyAsync(accepted, d_acc, sizeof(int) * N, hipMemcpyDeviceToHost, s));
+    if (pos) VMC_HIP_TRY(hipMemcpyAsync(pos, d_pos, sizeof(double) * N * N * 3, hipMemcpyDeviceToHost, s));
+    VMC_HIP_TRY(hipStreamSynchronize(s));
+    e->sweeps_done += 1;
     return VMC_OK;
 }
 

@@ -41,6 +41,10 @@ public:
     bool gaussian_moves{false};     // Move = gaussian
     bool mixed_precision{false};    // Precision = mixed
     std::size_t reinvert_interval{0};
+    // Frames = auto | step | sweep: the streamed walker's frames after every
+    // step (the reference's cadence, FP64 only) or after every sweep; auto is
+    // step in FP64 and sweep in mixed precision.
+    int frame_cadence{0};
 
     std::size_t warmup_steps{};
     std::size_t measure_steps{};
@@ -72,6 +76,7 @@ public:
         take_choice(keys, "Ewald", "reference", "heg", cfg.heg_ewald);
         take_choice(keys, "Move", "uniform", "gaussian", cfg.gaussian_moves);
         take_choice(keys, "Precision", "fp64", "mixed", cfg.mixed_precision);
+        take_cadence(keys, cfg.frame_cadence);
         if (!keys.empty()) {
             std::cout << "[Config] Warning! unknown keys ignored:";
             for (auto const &kv : keys) std::cout << " " << kv.first;
@@ -83,6 +88,9 @@ public:
 
     // Walkers the ensemble runs.
     [[nodiscard]] std::size_t walkers() const { return num_walkers ? num_walkers : num_threads; }
+
+    // Frames after every step (else after every sweep).
+    [[nodiscard]] bool frames_per_step() const { return frame_cadence == 1 || (frame_cadence == 0 && !mixed_precision); }
 
     // Re-validates and re-derives after fields were changed in code.
     void refresh() { compute_derived(); }
@@ -102,6 +110,8 @@ public:
             throw std::invalid_argument("[Config] Num_Threads or Num_Walkers must be >= 1");
         if (!(step_size >= 0.0) || !std::isfinite(step_size))
             throw std::invalid_argument("[Config] step size must be finite and >= 0");
+        if (frame_cadence == 1 && mixed_precision)
+            throw std::invalid_argument("[Config] Frames = step needs Precision = fp64");
     }
 
 private:
@@ -153,6 +163,17 @@ private:
     static void take(Keys &keys, char const *key, T &out) {
         if (auto it{keys.find(key)}; it != keys.end()) {
             out = static_cast<T>(std::stoull(it->second));
+            keys.erase(it);
+        }
+    }
+    static void take_cadence(Keys &keys, int &out) {
+        if (auto it{keys.find("Frames")}; it != keys.end()) {
+            std::string v{it->second};
+            for (auto &ch : v) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+            if (v == "auto") out = 0;
+            else if (v == "step") out = 1;
+            else if (v == "sweep") out = 2;
+            else throw std::invalid_argument("[Config] Frames must be auto, step or sweep");
             keys.erase(it);
         }
     }

@@ -9,7 +9,10 @@
 // src/main.cpp:50-68), each electron is proposed once per sweep in a fixed
 // order (index order, or the two spins alternating when they are equal), and
 // the local energy is measured after every sweep.  Walker 0's frames stream
-// to the writer, the way the reference's thread 0 streams its own.
+// to the writer, the way the reference's thread 0 streams its own: after every
+// step with its local energy after that step and blocking over steps (the
+// reference's cadence, FP64; the ensemble then sweeps on the generic kernel),
+// or after every sweep (Config::frames_per_step).
 //
 // Errors keep the reference's types: std::invalid_argument for a bad
 // configuration, std::runtime_error when no non-singular start is found or
@@ -60,5 +63,7 @@ private:
     vmc_ensemble *ensemble_{nullptr};
     std::vector<MeasurementSummary> per_walker_;
     std::size_t accepted0_{0}, proposed0_{0};  // walker 0's counts (the streamed chain)
+    // Walker 0's summary from its per-step stream (frames after every step).
+    std::optional<MeasurementSummary> stream_summary_;
     double moves_per_second_{0.0};
 };

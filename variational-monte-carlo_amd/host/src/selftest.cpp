@@ -109,9 +109,6 @@ int wavefunction_cmd(char **argv) {
     const double lp = wf.evaluate_log_psi(p);
     wf.evaluate_derivatives(p);
     EnergyTracker et{L, static_cast<double>(n)};
-    et.initialize_structure_factors(p);
-    et.initialize_reciprocal_energy();
-    et.initialize_real_energy(p);
     const double e = et.eval_total_energy(p);
     std::cout << std::setprecision(std::numeric_limits<double>::max_digits10) << "log_psi=" << lp
               << "\ne_local=" << e << "\n";

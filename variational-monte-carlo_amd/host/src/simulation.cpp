@@ -6,8 +6,11 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstddef>
+#include <optional>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace {
 
@@ -93,6 +96,10 @@ void Simulation::create_ensemble() {
           "ensemble creation");
     check(vmc_ensemble_set_walker_threads(ensemble_, vmc_default_walker_threads(&sys)),
           "walker workgroup");
+    // Frames after every step come from the generic kernel's traced sweep;
+    // the whole run sweeps on that kernel so the streamed chain is one chain.
+    if (output_writer_ && config_.frames_per_step())
+        check(vmc_ensemble_set_kernel(ensemble_, VMC_KERNEL_GENERIC), "kernel selection");
 }
 
 Simulation::MeasurementSummary Simulation::run() {
@@ -121,7 +128,7 @@ Simulation::MeasurementSummary Simulation::run() {
               "warmup");
     const MeasurementSummary summary = measure();
     if (output_writer_) {
-        const auto &w0 = per_walker_.front();
+        const auto &w0 = stream_summary_ ? *stream_summary_ : per_walker_.front();
         output_writer_->write_done(DoneData{.total_accepted = accepted0_,
                                             .total_proposed = proposed0_,
                                             .final_acceptance_rate = w0.acceptance_rate,
@@ -137,7 +144,51 @@ Simulation::MeasurementSummary Simulation::measure() {
     const std::size_t W = config_.walkers();
     const std::size_t N = config_.num_particles;
     const auto t0 = std::chrono::steady_clock::now();
-    if (output_writer_) {
+    stream_summary_.reset();
+    std::size_t stream_accepted = 0, stream_steps = 0;
+    if (output_writer_ && config_.frames_per_step()) {
+        // Step by step for walker 0: its local energy after every move, the
+        // running mean and the blocking over steps of Block_Size
+        // (src/simulation/simulation.cpp:203-243).
+        std::vector<double> el(N), pos(N * N * 3);
+        std::vector<int32_t> mv(N), acc(N);
+        BlockingAnalysis blocking{config_.block_size};
+        double running = 0.0, final_mean = 0.0;
+        std::optional<double> final_se;
+        for (std::size_t i = 0; i < config_.measure_sweeps; ++i) {
+            check(vmc_ensemble_sweep_traced(ensemble_, VMC_SWEEP_MEASURE, bs, 0, el.data(), mv.data(), acc.data(),
+                                            pos.data(), nullptr),
+                  "traced sweep");
+            for (std::size_t t = 0; t < N; ++t) {
+                ++stream_steps;
+                stream_accepted += acc[t] != 0 ? 1U : 0U;
+                running += el[t];
+                blocking.add(el[t]);
+                const double mean = running / static_cast<double>(stream_steps);
+                final_mean = mean;
+                std::optional<double> se;
+                if (blocking.ready()) {
+                    const auto [bm, e] = blocking.mean_and_standard_error();
+                    final_mean = bm;
+                    se = e;
+                    final_se = e;
+                }
+                const auto first = pos.begin() + static_cast<std::ptrdiff_t>(t * N * 3);
+                output_writer_->write_frame(FrameData{
+                    .step = stream_steps,
+                    .accepted = stream_accepted,
+                    .proposed = stream_steps,
+                    .acceptance_rate = static_cast<double>(stream_accepted) / static_cast<double>(stream_steps),
+                    .local_energy = el[t],
+                    .mean_energy = mean,
+                    .standard_error = se,
+                    .positions = std::vector<double>(first, first + static_cast<std::ptrdiff_t>(N * 3))});
+            }
+        }
+        stream_summary_ = MeasurementSummary{
+            .mean_energy = final_mean, .standard_error = final_se,
+            .acceptance_rate = static_cast<double>(stream_accepted) / static_cast<double>(stream_steps)};
+    } else if (output_writer_) {
         // Sweep by sweep, streaming walker 0 (src/simulation/simulation.cpp:203-243).
         double st[VMC_STAT_COUNT];
         std::vector<double> pos(3 * N);
@@ -170,8 +221,8 @@ Simulation::MeasurementSummary Simulation::measure() {
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     moves_per_second_ = secs > 0 ? static_cast<double>(W * N * config_.measure_sweeps) / secs : 0.0;
 
-    accepted0_ = static_cast<std::size_t>(stats[VMC_STAT_ACCEPTED]);
-    proposed0_ = static_cast<std::size_t>(stats[VMC_STAT_PROPOSED]);
+    accepted0_ = stream_summary_ ? stream_accepted : static_cast<std::size_t>(stats[VMC_STAT_ACCEPTED]);
+    proposed0_ = stream_summary_ ? stream_steps : static_cast<std::size_t>(stats[VMC_STAT_PROPOSED]);
     per_walker_.clear();
     double e_sum = 0.0, var_sum = 0.0, acc_sum = 0.0;
     std::size_t with_se = 0;

@@ -187,6 +187,8 @@ def lib():
             "vmc_ensemble_sweep": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _VP]),
             "vmc_ensemble_sweep_given": (C.c_int, [_VP, _dp, _dp, _dp, _VP]),
             "vmc_ensemble_measure": (C.c_int, [_VP, _dp, _VP]),
+            "vmc_ensemble_sweep_traced": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int64, _dp, _ip, _ip,
+                                                    _dp, _VP]),
             "vmc_ensemble_measure_noalloc": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_get_inverse": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_reset_stats": (C.c_int, [_VP, _VP]),
@@ -314,6 +316,24 @@ class Ensemble:
         tr = np.zeros((self.W, self.N, MV_COUNT))
         _check(lib().vmc_ensemble_sweep_given(self._h, _ptr(p), _ptr(u), _ptr(tr), self.stream))
         return tr
+
+    def sweep_traced(self, walker, measure=True, adapt=False, block_size=1, positions=True):
+        """One sweep (vmc_ensemble_sweep_traced, generic kernel, FP64) with
+        walker `walker`'s local energy after each of its moves: a dict of
+        e_local [N], moved [N], accepted [N] and, with positions, pos [N][N][3]
+        (the walker after each move), in sweep order."""
+        flags = (SWEEP_MEASURE if measure else 0) | (SWEEP_ADAPT if adapt else 0)
+        el = np.zeros(self.N)
+        mv = np.zeros(self.N, dtype=np.int32)
+        acc = np.zeros(self.N, dtype=np.int32)
+        pos = np.zeros((self.N, self.N, 3)) if positions else None
+        _check(lib().vmc_ensemble_sweep_traced(self._h, flags, block_size, int(walker), _ptr(el),
+                                               mv.ctypes.data_as(_ip), acc.ctypes.data_as(_ip),
+                                               _ptr(pos) if positions else None, self.stream))
+        out = dict(e_local=el, moved=mv, accepted=acc)
+        if positions:
+            out["pos"] = pos
+        return out
 
     def measure(self):
         """Per-walker local energy parts (vmc_ensemble_measure) as a dict of
