@@ -434,12 +434,24 @@ def configs_run(V, torch, sweeps=10):
         torch.cuda.synchronize()
         kt = {k: v[0] / v[1] * 1e3 for k, v in e.timing().items() if v[1]}
         key = f"N{n}_{'mixed_K' + str(K) if prec == V.PREC_MIXED else 'fp64'}"
-        out[key] = dict(walkers=W, ms_per_sweep=dt * 1e3, moves_per_s=W * n / dt,
-                        kernel="register" if e.kernel() == V.KERNEL_REGISTER else "generic",
+        kname = {V.KERNEL_REGISTER: "register", V.KERNEL_DELAYED: "delayed"}.get(e.kernel(), "generic")
+        out[key] = dict(walkers=W, ms_per_sweep=dt * 1e3, moves_per_s=W * n / dt, kernel=kname,
                         us_per_launch={k: round(v, 1) for k, v in kt.items()})
         if prec == V.PREC_MIXED:
             d = e.drift()
             out[key]["drift"] = {"max": float(d.max()), "median": float(np.median(d))}
+            # A measured mixed sweep refreshes every inverse in FP64 for its
+            # local energy, so K acts on the unmeasured (equilibration)
+            # sweeps: their time and re-inversions per walker and sweep.
+            st0 = e.stats()[:, V.STAT["REINV"]].sum()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            e.sweep(sweeps)
+            torch.cuda.synchronize()
+            dtu = (time.perf_counter() - t0) / sweeps
+            out[key]["unmeasured"] = dict(
+                ms_per_sweep=dtu * 1e3, moves_per_s=W * n / dtu,
+                reinversions_per_walker_sweep=float(e.stats()[:, V.STAT["REINV"]].sum() - st0) / (W * sweeps))
         e.close()
     return out
 

@@ -46,7 +46,7 @@ enum { VMC_MOVE_UNIFORM = 0, VMC_MOVE_GAUSSIAN = 1 };
 enum { VMC_ERFC_AS7126 = 0, VMC_ERFC_EXACT = 1 };
 
 /* Sweep kernel selection (vmc_ensemble_set_kernel). */
-enum { VMC_KERNEL_AUTO = 0, VMC_KERNEL_GENERIC = 1, VMC_KERNEL_REGISTER = 2 };
+enum { VMC_KERNEL_AUTO = 0, VMC_KERNEL_GENERIC = 1, VMC_KERNEL_REGISTER = 2, VMC_KERNEL_DELAYED = 3 };
 
 /* Sweep flags. */
 enum { VMC_SWEEP_MEASURE = 1, VMC_SWEEP_ADAPT = 2 };
@@ -69,8 +69,10 @@ typedef struct vmc_system {
     int32_t erfc_kind;         /* VMC_ERFC_*; AS7126 is energy_tracking.cpp:101-139 */
     int32_t move_kind;         /* VMC_MOVE_*; UNIFORM is simulation.cpp:101-103 */
     int32_t precision;         /* VMC_PREC_* */
-    int32_t reinvert_interval; /* accepted moves between FP64 re-inversions; 0 = never
-                                  (the reference never re-inverts) */
+    int32_t reinvert_interval; /* K: accepted moves per determinant between FP64
+                                  re-inversions (a walker is due once its accepted moves
+                                  over all determinants reach K x determinants); 0 = never
+                                  (the reference never re-inverts); at most 2^24 */
 } vmc_system;
 
 /* Per-walker FP64 statistics, vmc_ensemble_read_stats() layout (doubles). */
@@ -230,13 +232,19 @@ int vmc_ensemble_get_inverse(vmc_ensemble *e, double *inv, void *stream);
 
 /* Choose the sweep kernel: VMC_KERNEL_GENERIC keeps the inverses in LDS (or
  * HBM) for any N; VMC_KERNEL_REGISTER keeps each inverse row in a lane's
- * registers (at most 32 electrons per spin, equal spins or polarised).
- * VMC_KERNEL_AUTO (default) takes the register kernel whenever it applies.
- * No reference counterpart: the reference has a single CPU code path; this
- * is the autotuner's knob.  VMC_ERR_INVALID if the system does not fit the
- * requested kernel. */
+ * registers (7, 19, 27, 33, 57 or 81 electrons per spin, FP32 also 123 and
+ * 147; equal spins or polarised); VMC_KERNEL_DELAYED (FP64, at most 160
+ * electrons per spin) keeps the inverse in HBM and applies accepted moves
+ * as delayed rank-8 Woodbury updates on the matrix cores.
+ * VMC_KERNEL_AUTO (default) takes the register kernel whenever it applies,
+ * else, in FP64, the delayed-update kernel when the inverses do not fit the
+ * generic kernel's LDS.  No reference counterpart: the reference has a
+ * single CPU code path (its rank-1 update, slater_plane_wave.cpp:367-423);
+ * this is the autotuner's knob.  VMC_ERR_INVALID if the system does not fit
+ * the requested kernel. */
 int vmc_ensemble_set_kernel(vmc_ensemble *e, int32_t kernel);
-/* The kernel the next sweep launch will use (VMC_KERNEL_GENERIC or _REGISTER). */
+/* The kernel the next sweep launch will use (VMC_KERNEL_GENERIC, _REGISTER
+ * or _DELAYED). */
 int vmc_ensemble_get_kernel(const vmc_ensemble *e, int32_t *kernel);
 /* Register-kernel build variant: 0 = unbounded registers, 1 = at most 128
  * VGPRs (4 waves per SIMD, some spills).  No reference counterpart; an
@@ -369,7 +377,7 @@ int vmc_ensemble_footprint(const vmc_system *sys, int64_t n_walkers, int64_t *by
  * timing `sweeps` measured sweeps of each candidate with HIP events.  No
  * reference counterpart (the reference has no device layout to tune). */
 typedef struct vmc_tuning {
-    int32_t kernel;            /* VMC_KERNEL_GENERIC or VMC_KERNEL_REGISTER */
+    int32_t kernel;            /* VMC_KERNEL_GENERIC, _REGISTER or _DELAYED */
     int32_t walker_threads;    /* generic kernel: 64, 128 or 256 threads per walker */
     int32_t rr_variant;        /* register kernel build variant */
     int32_t inv_placement;     /* generic kernel: 1 LDS, 2 HBM */

@@ -43,8 +43,13 @@ def assert_same(a, b):
 
 
 def alone(V, s, W, seed, kernel):
+    """The system alone with the kernel its batch group runs: a batch takes
+    the register kernels or the generic one, so an FP64 system that alone
+    would take the delayed-update kernel runs the generic kernel there."""
     e = V.Ensemble(s, W, seed)
     e.set_kernel(kernel)
+    if e.kernel() == V.KERNEL_DELAYED:
+        e.set_kernel(V.KERNEL_GENERIC)
     e.init_positions()
     e.sweep(2, adapt=True, block_size=2)
     e.sweep(3, measure=True, block_size=2)
@@ -145,12 +150,13 @@ def test_batch_regroup_drops_the_graph(vmc):
 
 
 def test_reinversion_list_with_more_items_than_workgroups(vmc):
-    """4000 walkers of N=14 at K=16: a sweep accepts about 7 moves per walker,
-    so well over a third of the walkers are due after each sweep and the lone
+    """4000 walkers of N=14 at K=8 per determinant: a sweep accepts about 7
+    moves per walker (3.5 per determinant), so well over a third of the
+    walkers are due after each sweep and the lone
     ensemble's re-inversion list holds more (walker, spin) items than its 2048
     workgroups, which then take several items each; the batch path
     re-inverts with one workgroup per item.  Both give the same chains."""
-    s = vmc.System.heg(14, 1.0, 1.0, precision=1, reinvert_interval=16)
+    s = vmc.System.heg(14, 1.0, 1.0, precision=1, reinvert_interval=8)
     W, seed = 4000, 77
     b = vmc.Batch([s], [W], [seed])
     b.init_positions()

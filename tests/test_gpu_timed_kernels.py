@@ -40,6 +40,17 @@ CASES = [
     ("heg162", lambda V, p: V.System.heg(162, 2.0, 1.0, precision=p), 32, "register"),
     ("heg54_generic", lambda V, p: V.System.heg(54, 2.0, 1.0, precision=p), 64, "generic"),
 ]
+# FP64 with the delayed-update kernel (inverse in HBM, rank-8 Woodbury
+# windows): forced at sizes the register kernels also run, and chosen by
+# itself at 123 and 147 per spin.
+FP64_DELAYED = [
+    ("heg14_delayed", lambda V, p: V.System.heg(14, 1.0, 1.0, precision=p), 64, "delayed"),
+    ("heg54_delayed", lambda V, p: V.System.heg(54, 2.0, 1.0, precision=p), 64, "delayed"),
+    ("ref27_delayed", lambda V, p: V.System.reference(27, 12.0, precision=p), 64, "delayed"),
+    ("heg114_delayed", lambda V, p: V.System.heg(114, 5.0, 0.8, precision=p), 32, "delayed"),
+    ("heg246", lambda V, p: V.System.heg(246, 2.0, 1.0, precision=p), 16, "auto"),
+    ("heg294", lambda V, p: V.System.heg(294, 2.0, 1.0, precision=p), 16, "auto"),
+]
 MIXED_ONLY = [
     ("heg246", lambda V, p: V.System.heg(246, 2.0, 1.0, precision=p), 16, "register"),
     ("heg294", lambda V, p: V.System.heg(294, 2.0, 1.0, precision=p), 16, "register"),
@@ -93,7 +104,10 @@ def run(V, O, factory, precision, n_configs, kernel, seed):
     ens = V.Ensemble(s, n_configs, seed)
     if kernel == "generic":
         ens.set_kernel(V.KERNEL_GENERIC)
-    assert ens.kernel() == (V.KERNEL_GENERIC if kernel == "generic" else V.KERNEL_REGISTER)
+    elif kernel == "delayed":
+        ens.set_kernel(V.KERNEL_DELAYED)
+    expect = {"generic": V.KERNEL_GENERIC, "delayed": V.KERNEL_DELAYED, "auto": V.KERNEL_DELAYED}
+    assert ens.kernel() == expect.get(kernel, V.KERNEL_REGISTER)
     ens.set_positions(pos)
     before = ens.measure()
     if precision == V.PREC_FP64:
@@ -219,3 +233,16 @@ def test_timed_mixed_large_match_oracle(vmc, oracle, name, factory, n_configs, k
     b = budget_for(vmc, s, vmc.PREC_MIXED)
     bad = {k: v for k, v in worst.items() if not v <= b(k)}
     assert not bad, f"{name} mixed: beyond budget: {bad}"
+
+
+@pytest.mark.parametrize("name,factory,n_configs,kernel", FP64_DELAYED, ids=[c[0] for c in FP64_DELAYED])
+def test_timed_delayed_update_match_oracle(vmc, oracle, name, factory, n_configs, kernel):
+    """The FP64 delayed-update sweep: every move's ratio and Jastrow change,
+    the decisions, the inverse after the sweep's Woodbury flushes and the
+    local energy, each within 1e-10 of the oracle's rank-1 updates."""
+    s, pos, ref, before, trace, after, inv, got_pos = run(vmc, oracle, factory, vmc.PREC_FP64, n_configs,
+                                                          kernel, seed=seed_of(name))
+    worst, _ = check(vmc, oracle, s, vmc.PREC_FP64, pos, ref, before, trace, after, inv, got_pos)
+    print(name, "fp64", " ".join(f"{k}={v:.1e}" for k, v in worst.items()), flush=True)
+    bad = {k: v for k, v in worst.items() if not v <= 1e-10}
+    assert not bad, f"{name} fp64: beyond budget: {bad}"

@@ -34,7 +34,8 @@ struct DevSys {
     int erfc_kind;   // 0 = Abramowitz-Stegun 7.1.26, 1 = erfc
     int move_kind;   // 0 = uniform cube, 1 = Gaussian
     int use_cut;
-    int reinv_k;     // accepted moves between FP64 re-inversions (0 = never)
+    int reinv_k;     // due threshold of the walker's accepted-move counter, summed over its
+                     // determinants: K x ndet (0 = never re-invert)
     double L, inv_L, half_L;
     double a_same, a_anti, b;
     double alpha, rcut;

@@ -667,7 +667,8 @@ int validate(const vmc_system *s) {
         return fail(VMC_ERR_INVALID, "unknown move kind %d", s->move_kind);
     if (s->erfc_kind != VMC_ERFC_AS7126 && s->erfc_kind != VMC_ERFC_EXACT)
         return fail(VMC_ERR_INVALID, "unknown erfc kind %d", s->erfc_kind);
-    if (s->reinvert_interval < 0) return fail(VMC_ERR_INVALID, "reinvert_interval must be >= 0");
+    if (s->reinvert_interval < 0 || s->reinvert_interval > (1 << 24))
+        return fail(VMC_ERR_INVALID, "reinvert_interval must be in [0, 2^24]");
     if (s->recip_nmax2 < 0) return fail(VMC_ERR_INVALID, "recip_nmax2 must be >= 0");
     if (s->n_up + s->n_dn > 4096) return fail(VMC_ERR_INVALID, "at most 4096 electrons");
     return VMC_OK;
@@ -693,7 +694,10 @@ int build_tables(const vmc_system *sys, uint64_t seed, SystemTables &t) {
     d.erfc_kind = sys->erfc_kind;
     d.move_kind = sys->move_kind;
     d.use_cut = sys->real_cutoff > 0.0 ? 1 : 0;
-    d.reinv_k = sys->reinvert_interval;
+    // K counts accepted moves per determinant; a walker's counter adds up the
+    // accepted moves of all its determinants, so it is due at K per
+    // determinant on average.
+    d.reinv_k = sys->reinvert_interval * d.ndet;
     d.L = sys->box_length;
     d.inv_L = 1.0 / sys->box_length;
     d.half_L = 0.5 * sys->box_length;
@@ -828,10 +832,67 @@ bool rrw_supported(const DevSys &d, int real_bytes) {
     return rrw_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng, real_bytes);
 }
 
+// Sweep kernel family of an ensemble: the register-resident kernels where an
+// instantiation exists, in FP64 the delayed-update kernel where the inverses
+// do not fit the generic kernel's LDS, else the generic kernel; or the
+// family the caller forced (vmc_ensemble_set_kernel validated it).
+enum { kFamGeneric = 0, kFamRegister = 1, kFamMultiwave = 2, kFamDelayed = 3 };
+int sweep_family(const vmc_ensemble *e) {
+    const DevSys &d = e->tables.dev;
+    const int rb = e->tables.real_bytes;
+    if (e->kernel == VMC_KERNEL_GENERIC) return kFamGeneric;
+    if (e->kernel == VMC_KERNEL_DELAYED) return kFamDelayed;
+    if (rr_supported(d)) return kFamRegister;
+    if (rrw_supported(d, rb)) return kFamMultiwave;
+    if (e->kernel == VMC_KERNEL_AUTO && rb == 8 && du_supported(d) && lds_for(d, 8, true) > kMaxLdsBytes)
+        return kFamDelayed;
+    return kFamGeneric;
+}
+
+// Delayed-update sweeps (FP64), one launch per sweep when each is measured
+// or followed by the periodic re-inversion.  The local energy is the FP64
+// multi-wavefront measurement where that has an instantiation (123 and 147
+// per spin among them: it reads the inverse rows from memory), else the
+// generic one; both take the inverse the sweep left current in HBM.
+int launch_sweep_du(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipStream_t stream) {
+    const DevSys &d = e->tables.dev;
+    const unsigned grid = static_cast<unsigned>(e->W);
+    const bool measure = flags & VMC_SWEEP_MEASURE;
+    const bool reinv = d.reinv_k > 0;
+    const int per_launch = (measure || reinv) ? 1 : n_sweeps;
+    for (int i = 0; i < n_sweeps; i += per_launch) {
+        const int n = n_sweeps - i < per_launch ? n_sweeps - i : per_launch;
+        const unsigned long long s0 = (unsigned long long)e->sweeps_done + (unsigned long long)i;
+        hipEvent_t ev = nullptr;
+        bool timed = e->timer.begin(stream, ev);
+        if (!du_launch(d, e->st, grid, n, s0, flags & ~VMC_SWEEP_MEASURE, SweepGiven{}, stream))
+            return fail(VMC_ERR_INVALID, "no delayed-update kernel for this system");
+        if (timed) e->timer.end(stream, ev, kTimeSweep);
+        if (reinv) {
+            timed = e->timer.begin(stream, ev);
+            reinv_generic_launch(d, e->st, grid, stream, 8);
+            if (timed) e->timer.end(stream, ev, kTimeReinv);
+        }
+        if (measure) {
+            timed = e->timer.begin(stream, ev);
+            if (!rrw_measure64(d, e->st, grid, block_size, nullptr, stream))
+                hipLaunchKernelGGL((k_measure_parts<double, false>), dim3(grid),
+                                   dim3(static_cast<unsigned>(e->walker_threads)), lds_for(d, 8, false), stream, d,
+                                   e->st, static_cast<double *>(nullptr), block_size);
+            if (timed) e->timer.end(stream, ev, kTimeMeasure);
+        }
+    }
+    VMC_HIP_TRY(hipGetLastError());
+    return VMC_OK;
+}
+
 template <typename Real>
 int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipStream_t stream) {
     const DevSys &d = e->tables.dev;
-    if (e->kernel != VMC_KERNEL_GENERIC && rr_supported(d)) {
+    const int fam = sweep_family(e);
+    if constexpr (sizeof(Real) == 8)
+        if (fam == kFamDelayed) return launch_sweep_du(e, n_sweeps, flags, block_size, stream);
+    if (fam == kFamRegister) {
         const unsigned grid = static_cast<unsigned>(e->W);
         const unsigned long long s0 = (unsigned long long)e->sweeps_done;
         const bool ok = sizeof(Real) == 4
@@ -841,7 +902,7 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
         VMC_HIP_TRY(hipGetLastError());
         return VMC_OK;
     }
-    if (e->kernel != VMC_KERNEL_GENERIC && rrw_supported_sizes(d.ndet, d.n_up, d.n_dn, d.ng, (int)sizeof(Real))) {
+    if (fam == kFamMultiwave) {
         const unsigned grid = static_cast<unsigned>(e->W);
         const unsigned long long s0 = (unsigned long long)e->sweeps_done;
         const ReinvLaunch rl{static_cast<int>(sizeof(Real))};
@@ -1153,11 +1214,15 @@ int vmc_ensemble_sweep_given(vmc_ensemble *e, const double *proposals, const dou
     const unsigned grid = static_cast<unsigned>(e->W);
     const int rb = e->tables.real_bytes;
     bool done = false;
-    if (e->kernel != VMC_KERNEL_GENERIC && rr_supported(d)) {
+    const int fam = sweep_family(e);
+    if (fam == kFamRegister) {
         done = rb == 4 ? rr_given_f32(d, e->st, grid, g, s) : rr_given_f64(d, e->st, grid, g, s);
-    } else if (e->kernel != VMC_KERNEL_GENERIC && rrw_supported(d, rb)) {
+    } else if (fam == kFamMultiwave) {
         const ReinvLaunch rl{rb};
         done = rb == 4 ? rrw_given_f32(d, e->st, grid, g, s, rl) : rrw_given_f64(d, e->st, grid, g, s, rl);
+    } else if (fam == kFamDelayed) {
+        done = du_launch(d, e->st, grid, 1, 0, 0, g, s);
+        if (done && d.reinv_k > 0) reinv_generic_launch(d, e->st, grid, s, rb);
     }
     if (!done) {
         const size_t lds_in = lds_for(d, rb, true);
@@ -1192,9 +1257,12 @@ int launch_parts(vmc_ensemble *e, double *d_parts, hipStream_t s) {
     const unsigned grid = static_cast<unsigned>(e->W);
     const int rb = e->tables.real_bytes;
     bool done = false;
-    if (e->kernel != VMC_KERNEL_GENERIC && rr_supported(d))
+    const int fam = sweep_family(e);
+    if (fam == kFamRegister)
         done = rb == 4 ? rr_parts_f32(d, e->st, grid, d_parts, s) : rr_parts_f64(d, e->st, grid, d_parts, s);
-    else if (e->kernel != VMC_KERNEL_GENERIC && rrw_supported(d, rb))
+    else if (fam == kFamDelayed)
+        done = rrw_measure64(d, e->st, grid, 1, d_parts, s);
+    else if (fam == kFamMultiwave)
         done = rb == 4 ? rrw_parts_exact_f32(d, e->st, grid, d_parts, e->inv64, s, ReinvLaunch{rb})
                        : rrw_parts_f64(d, e->st, grid, d_parts, s);
     if (!done && rb == 4) {
@@ -1311,10 +1379,13 @@ int vmc_ensemble_get_inverse(vmc_ensemble *e, double *inv, void *stream) {
 
 int vmc_ensemble_set_kernel(vmc_ensemble *e, int32_t kernel) {
     if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
-    if (kernel != VMC_KERNEL_AUTO && kernel != VMC_KERNEL_GENERIC && kernel != VMC_KERNEL_REGISTER)
+    if (kernel != VMC_KERNEL_AUTO && kernel != VMC_KERNEL_GENERIC && kernel != VMC_KERNEL_REGISTER &&
+        kernel != VMC_KERNEL_DELAYED)
         return fail(VMC_ERR_INVALID, "unknown kernel %d", kernel);
     if (kernel == VMC_KERNEL_REGISTER && !rr_supported(e->tables.dev) && !rrw_supported(e->tables.dev, e->tables.real_bytes))
         return fail(VMC_ERR_INVALID, "register kernel needs 7, 19, 27, 33, 57 or 81 electrons per spin (FP32 also 123, 147)");
+    if (kernel == VMC_KERNEL_DELAYED && (e->tables.real_bytes != 8 || !du_supported(e->tables.dev)))
+        return fail(VMC_ERR_INVALID, "delayed-update kernel needs an FP64 system of at most 160 electrons per spin");
     e->kernel = kernel;
     return VMC_OK;
 }
@@ -1345,9 +1416,10 @@ int vmc_ensemble_set_walker_threads(vmc_ensemble *e, int32_t threads) {
 
 int vmc_ensemble_get_kernel(const vmc_ensemble *e, int32_t *kernel) {
     if (!e || !kernel) return fail(VMC_ERR_INVALID, "NULL argument");
-    *kernel = (e->kernel != VMC_KERNEL_GENERIC && (rr_supported(e->tables.dev) || rrw_supported(e->tables.dev, e->tables.real_bytes)))
-                  ? VMC_KERNEL_REGISTER
-                  : VMC_KERNEL_GENERIC;
+    const int fam = sweep_family(e);
+    *kernel = fam == kFamRegister || fam == kFamMultiwave ? VMC_KERNEL_REGISTER
+              : fam == kFamDelayed                        ? VMC_KERNEL_DELAYED
+                                                          : VMC_KERNEL_GENERIC;
     return VMC_OK;
 }
 

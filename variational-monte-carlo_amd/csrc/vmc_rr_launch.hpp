@@ -154,6 +154,14 @@ bool rrw_given_f64(const DevSys &d, const DevState &st, unsigned grid, const Swe
 bool rrw_parts_f32(const DevSys &d, const DevState &st, unsigned grid, double *parts, hipStream_t stream);
 bool rrw_parts_f64(const DevSys &d, const DevState &st, unsigned grid, double *parts, hipStream_t stream);
 
+// FP64 sweeps with delayed Woodbury updates of HBM-resident inverses
+// (vmc_sweep_du.hip): at most 160 electrons per spin, equal spins or
+// polarised.  du_launch runs n_sweeps sweeps (no measurement) with the
+// Philox draws, or with `given` set one sweep of the caller's moves.
+bool du_supported(const DevSys &d);
+bool du_launch(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps, unsigned long long s0, int flags,
+               const SweepGiven &given, hipStream_t stream);
+
 // Launches n_sweeps sweeps (and, with VMC_SWEEP_MEASURE, one measurement per
 // sweep).  variant 0: no occupancy bound; 1: at most 128 VGPRs.
 bool rr_launch_f32(const DevSys &d, const DevState &st, unsigned grid, int n_sweeps,

@@ -117,6 +117,7 @@ int vmc_autotune(const vmc_system *sys, int64_t n_walkers, double drift_tol, int
             cands.push_back({VMC_KERNEL_REGISTER, 0, 0, 64});
             cands.push_back({VMC_KERNEL_REGISTER, 1, 0, 64});
         }
+        if (!vmc_ensemble_set_kernel(e, VMC_KERNEL_DELAYED)) cands.push_back({VMC_KERNEL_DELAYED, 0, 0, 64});
         const bool lds_ok = !vmc_ensemble_set_inverse_placement(e, 1);
         for (int th : {64, 128, 256}) {
             if (lds_ok) cands.push_back({VMC_KERNEL_GENERIC, 0, 1, th});

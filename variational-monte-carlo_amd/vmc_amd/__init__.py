@@ -28,7 +28,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 MOVE_UNIFORM, MOVE_GAUSSIAN = 0, 1
 ERFC_AS7126, ERFC_EXACT = 0, 1
 SWEEP_MEASURE, SWEEP_ADAPT = 1, 2
-KERNEL_AUTO, KERNEL_GENERIC, KERNEL_REGISTER = 0, 1, 2
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_REGISTER, KERNEL_DELAYED = 0, 1, 2, 3
 
 STAT = dict(NMEAS=0, SUM_E=1, SUM_E2=2, SUM_KIN=3, SUM_POT=4, LAST_E=5, ACCEPTED=6,
             PROPOSED=7, BLK_SUM=8, BLK_IN=9, BLK_N=10, BLK_MEAN=11, BLK_M2=12, LAST_KIN=13,
