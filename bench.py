@@ -538,7 +538,8 @@ def main():
             r = res[k]
             line[k] = {"value": r["moves_per_s"], "step_ms_p50": r["step_ms_p50"],
                        "step_ms_p99": r["step_ms_p99"],
-                       "kernel": "register" if r["kernel"] == V.KERNEL_REGISTER else "generic"}
+                       "kernel": {V.KERNEL_REGISTER: "register", V.KERNEL_DELAYED: "delayed"}.get(r["kernel"],
+                                                                                               "generic")}
             if "measure_ms_avg" in r:
                 line[k].update(measure_ms_avg=r["measure_ms_avg"], sweep_ms_avg=r["sweep_ms_avg"],
                                measure_frac_fp64_peak=r["measure_tflops"] / PEAK_FP64_TFLOPS)
@@ -548,19 +549,16 @@ def main():
         line["mixed_over_fp64_generic"] = (res["mixed"]["moves_per_s"] /
                                            res["fp64_generic"]["moves_per_s"])
     if not a.no_serving:
-        # The full-size stream in mixed precision; the FP64 path serves the
-        # same requests with a twentieth of their sweeps (its generic kernel
-        # at N = 246 and 294 needs about 13 minutes of device time for the
-        # full-size stream), beside the mixed path on that shorter stream.
-        cap = {"mixed": serving_capacity(V, V.PREC_MIXED)}
-        sv = {"mixed": serving_run(V, V.PREC_MIXED, rate=cap["mixed"])}
-        sv["mixed_short"] = serving_run(V, V.PREC_MIXED, sweep_scale=0.05, rate=cap["mixed"])
+        # The full-size stream in mixed precision and in FP64 (the delayed-
+        # update kernel at 123 and 147 per spin), each at 80% of its own
+        # path's capacity.
+        sv = {"mixed": serving_run(V, V.PREC_MIXED)}
         if not a.mixed_only:
-            sv["fp64_short"] = serving_run(V, V.PREC_FP64, sweep_scale=0.05)
+            sv["fp64"] = serving_run(V, V.PREC_FP64)
         sv["what"] = ("16 requests of N from {14,38,54,66,114,162,246,294}, r_s log-uniform in [1,20], "
                       "b uniform in [0.5,2], walkers 128-4096, sweeps 100-2000 (a quarter warmup, E_L every "
-                      "measured sweep), Poisson arrivals at 80% offered load by the alone-throughput model; "
-                      "the _short streams take a twentieth of each request's sweeps")
+                      "measured sweep), Poisson arrivals at 80% offered load by the path's alone-throughput "
+                      "model")
         line["serving"] = sv
         line["latency_ms"]["p50_solve"] = sv["mixed"]["p50_ms"]
         line["latency_ms"]["p99_solve"] = sv["mixed"]["p99_ms"]

@@ -430,33 +430,41 @@ __device__ inline float erfc_fit(float x) {
     return fmaf(e, ye * kLn2, e) * p;
 }
 
-// FP64 erfc for 0 <= x <= kErfcFitMax: erfc(x) = exp(-x^2) Q(s),
-// s = (x - 3) / (x + 3), Q the degree-14 interpolant of erfcx at the
-// Chebyshev nodes of the s interval (fit_erfc.py: Q within 6e-16 relative of
-// erfcx, exp(-x^2) Q(s) within 6e-16 of erfc with a correctly rounded exp).
-// One branch-free path for every lane, where the library erfc splits the
-// range into intervals that the lanes of a wavefront take in turn.  The
-// rounding of x^2 enters as a first-order correction of the exponential.
+// FP64 erfc for 0 <= x <= kErfcFitMax: Q(t), t = 0.8 x - 1, the degree-24
+// Chebyshev interpolant of erfc on [0, 2.5] in the power basis of t
+// (fit_erfc.py: within 4e-16 of erfc in absolute terms, which is what the
+// Ewald sum of erfc(alpha r) / r over pairs needs).  One branch-free path for
+// every lane with no exponential and no division, where the library erfc
+// splits the range into intervals that the lanes of a wavefront take in
+// turn.
 __device__ inline double erfc_fit64(double x) {
-    const double s = (x - 3.0) * rcp_nr(x + 3.0);
-    double q = -5.626491863172647e-07;
-    q = fma(q, s, -6.1853504882410555e-06);
-    q = fma(q, s, -2.5954587494259244e-05);
-    q = fma(q, s, -3.812811629936757e-05);
-    q = fma(q, s, 3.882381513054981e-05);
-    q = fma(q, s, 2.6665540044490982e-05);
-    q = fma(q, s, -0.0006073342239491165);
-    q = fma(q, s, 0.0007034695201026614);
-    q = fma(q, s, 0.004267800284827522);
-    q = fma(q, s, -0.024392808541876164);
-    q = fma(q, s, 0.07166578534896151);
-    q = fma(q, s, -0.15011594257305944);
-    q = fma(q, s, 0.24560380124505782);
-    q = fma(q, s, -0.32623356006410514);
-    q = fma(q, s, 0.17900115118097112);
-    const double h = x * x, lo = fma(x, x, -h);  // x^2 = h + lo
-    const double e = exp(-h);
-    return fma(e, -lo, e) * q;
+    const double t = fma(x, 0.8, -1.0);
+    double q = -1.5789738524349255e-08;
+    q = fma(q, t, -1.0547604630012188e-08);
+    q = fma(q, t, 2.3109297654588739e-07);
+    q = fma(q, t, -4.204118927684107e-09);
+    q = fma(q, t, -1.945561418732383e-06);
+    q = fma(q, t, 1.681147304115916e-06);
+    q = fma(q, t, 1.1729907997669007e-05);
+    q = fma(q, t, -2.2665916030465085e-05);
+    q = fma(q, t, -4.804828007235314e-05);
+    q = fma(q, t, 0.00018224272805793717);
+    q = fma(q, t, 6.853680201713117e-05);
+    q = fma(q, t, -0.0010155399183195398);
+    q = fma(q, t, 0.0007674433199906389);
+    q = fma(q, t, 0.003771362281381614);
+    q = fma(q, t, -0.007390152521692959);
+    q = fma(q, t, -0.006538901006467077);
+    q = fma(q, t, 0.03396080776560146);
+    q = fma(q, t, -0.017290032190568103);
+    q = fma(q, t, -0.08125790773493699);
+    q = fma(q, t, 0.14398509643356755);
+    q = fma(q, t, 0.015037607984594071);
+    q = fma(q, t, -0.3272183497393615);
+    q = fma(q, t, 0.46195531727986267);
+    q = fma(q, t, -0.2956514030591187);
+    q = fma(q, t, 0.07709987174354177);
+    return q;
 }
 
 template <typename Real>
