@@ -149,6 +149,11 @@ def roofline_block(head, head_name):
     return line
 
 
+def log(msg):
+    """Progress on stderr (the JSON line alone goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def pct(xs, q):
     s = sorted(xs)
     return s[min(len(s) - 1, int(q * len(s)))]
@@ -496,6 +501,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     res = {}
+    log("timed runs")
     if not a.fp64_only:
         res["mixed"] = gpu_run(V, torch, V.PREC_MIXED, a.steps, a.warmup, a.reinvert,
                                V.KERNEL_AUTO)
@@ -549,12 +555,15 @@ def main():
         line["mixed_over_fp64_generic"] = (res["mixed"]["moves_per_s"] /
                                            res["fp64_generic"]["moves_per_s"])
     if not a.no_serving:
+        log("served streams")
         # The full-size stream in mixed precision and in FP64 (the delayed-
         # update kernel at 123 and 147 per spin), each at 80% of its own
         # path's capacity.
         sv = {"mixed": serving_run(V, V.PREC_MIXED)}
+        log("served stream, mixed: done")
         if not a.mixed_only:
             sv["fp64"] = serving_run(V, V.PREC_FP64)
+            log("served stream, fp64: done")
         sv["what"] = ("16 requests of N from {14,38,54,66,114,162,246,294}, r_s log-uniform in [1,20], "
                       "b uniform in [0.5,2], walkers 128-4096, sweeps 100-2000 (a quarter warmup, E_L every "
                       "measured sweep), Poisson arrivals at 80% offered load by the path's alone-throughput "
@@ -564,9 +573,12 @@ def main():
         line["latency_ms"]["p99_solve"] = sv["mixed"]["p99_ms"]
         line["latency_ms"]["solve_what"] = ("request latency (submit to result) of the full-size served "
                                             "stream (16 requests, 80% offered load), mixed")
+        log("uneven batch")
         line["batch"] = batch_run(V, torch)
+        log("other configurations")
         line["configs"] = configs_run(V, torch)
     # Physics at the configuration's stated length, independent seeds.
+    log("physics runs")
     phys = {}
     if "mixed" in res:
         phys["mixed"] = physics_run(V, torch, V.PREC_MIXED, SEED + 1)
@@ -581,6 +593,7 @@ def main():
                               "reinversions_per_walker_step": v["reinversions_per_walker_step"],
                               "drift_max": v["drift_max"]} for k, v in res.items()}
     if not a.no_cpu_baseline:
+        log("CPU baseline")
         line["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
         ref = reference_cpu(min(a.cpu_seconds, 5.0))
         if ref is not None:

@@ -219,7 +219,7 @@ struct RRShared {
     double stat[kStatCount];
     double misc[4];                // sigma, log|det| up, log|det| down
     Real phi[2][32];               // orbital rows of the pending moves (by spin)
-    Real row[2][32];               // inverse rows of the accepted movers (by spin)
+    Real row[32];                  // inverse row of the moved electron
     double gjcol[32];
     int perm[32];
     // The run prop | pph | tab as one staging area (inverse blocks, the
@@ -878,6 +878,44 @@ struct RRKernel {
         return a * rn * wrcp<Real>((Real)1 + c.b * rn);
     }
 
+    // Sherman-Morrison for an accepted move of electron (hm, i) with ratio R
+    // (row s_k of every lane of spin hm), then the Jastrow row sums: every
+    // other lane adds its new pair term un minus its old one, recomputed from
+    // the mover's old position.  Lane lm takes the proposal (xn, yn, zn) and
+    // the row sum jn.
+    __device__ static void apply_move(const RRConst<Real> &c, Sh &sh, RRLane<Real, NMAX> &L, int lane,
+                                      bool has_e, int h, int r, int hm, int i, double R, Real sk, Real un,
+                                      double jn, double xn, double yn, double zn) {
+        constexpr int NP = NU;
+        const int lm = hm * 32 + i;
+        if (lane == lm) {
+#pragma unroll
+            for (int m = 0; m < NP; ++m) sh.row[m] = L.inv[m];
+        }
+        __syncthreads();
+        {
+            const Real inv_ratio = wrcp<Real>((Real)R);
+            Real f = (r == i) ? (Real)1 - inv_ratio : sk * inv_ratio;
+            f = (h == hm && r < NP) ? f : (Real)0;
+#pragma unroll
+            for (int m = 0; m < NP; ++m) L.inv[m] -= sh.row[m] * f;
+        }
+        const double xo = rr::readlane(L.x, lm), yo = rr::readlane(L.y, lm), zo = rr::readlane(L.z, lm);
+        if (has_e && lane != lm) {
+            const Real ao = (h == hm) ? c.a_same : c.a_anti;
+            L.U += (double)un - (double)pair_with(c, L, xo, yo, zo, ao);
+        }
+        if (lane == lm) {
+            L.U = jn;
+            L.x = xn;
+            L.y = yn;
+            L.z = zn;
+            sh.pos[0][lane] = xn;
+            sh.pos[1][lane] = yn;
+            sh.pos[2][lane] = zn;
+        }
+    }
+
     // The moves of up electron i and then down electron i (the sweep order
     // of equal spins), evaluated together: the two determinants are
     // independent, so the wavefront's two halves form the orbital rows and
@@ -932,22 +970,23 @@ struct RRKernel {
         const double jn_u = (double)rr::wave_total(un_u);
         const double jn_d0 = (double)rr::wave_total(un_d);    // up electron at its old position
         const Real ud_old = rr::readlane(un_d, lu);
-        const Real unu_ld = rr::readlane(un_u, ld);            // down electron with the up proposal
         // Up move.
         const double dj_u = jn_u - rr::readlane(L.U, lu);
         const int acc_u = accept_test(Ru, dj_u, sh.prop[3][lu]);
-        // On acceptance every row sum trades its pair term with the up
-        // electron's old position for the one with the proposal; the down
-        // decision needs the down electron's row sum after that.
-        const double xuo = rr::readlane(L.x, lu), yuo = rr::readlane(L.y, lu), zuo = rr::readlane(L.z, lu);
-        Real po_u = (Real)0;
-        if (acc_u && has_e && lane != lu) po_u = pair_with(c, L, xuo, yuo, zuo, h == 0 ? c.a_same : c.a_anti);
-        const double U_ld = rr::readlane(L.U, ld);
-        const double U_ld_after = acc_u ? U_ld + ((double)unu_ld - (double)rr::readlane(po_u, ld)) : U_ld;
+        if (acc_u) {
+            apply_move(c, sh, L, lane, has_e, h, r, 0, i, Ru, sk, un_u, jn_u, xu, yu, zu);
+            if (lane == lu) un_d = ud_new;
+            ldt.add(0, Ru);
+        }
         // Down move, after the up move's outcome.
         const double jn_d = acc_u ? jn_d0 + ((double)ud_new - (double)ud_old) : jn_d0;
-        const double dj_d = jn_d - U_ld_after;
+        const double dj_d = jn_d - rr::readlane(L.U, ld);
         const int acc_d = accept_test(Rd, dj_d, sh.prop[3][ld]);
+        if (acc_d) {
+            __syncthreads();  // the up move's reads of the broadcast row are done
+            apply_move(c, sh, L, lane, has_e, h, r, 1, i, Rd, sk, un_d, jn_d, xd, yd, zd);
+            ldt.add(1, Rd);
+        }
         if (rec && lane == 0) {
             double *ru = rec + kTraceCount * i, *rd = rec + kTraceCount * (NU + i);
             ru[0] = Ru;
@@ -957,61 +996,6 @@ struct RRKernel {
             rd[1] = dj_d;
             rd[2] = (double)acc_d;
         }
-        if (!(acc_u | acc_d)) return 0;
-        // Sherman-Morrison (simulation.cpp:133-145, slater_plane_wave.cpp:
-        // 367-423) of both determinants in one pass: each lane updates its row
-        // with its own spin's mover row and ratio (the down ratio and its s_k
-        // come from the down inverse, which the up move leaves alone), one FMA
-        // per element: rows k != p lose (s_k / R) row_p, row p becomes
-        // row_p / R, and the lanes of a rejected spin subtract 0 x (a finite
-        // row).
-        if ((lane == lu && acc_u) || (lane == ld && acc_d)) {
-#pragma unroll
-            for (int m = 0; m < NP; ++m) sh.row[h][m] = L.inv[m];
-        }
-        const double xdo = rr::readlane(L.x, ld), ydo = rr::readlane(L.y, ld), zdo = rr::readlane(L.z, ld);
-        __syncthreads();
-        {
-            const int acc_h = h ? acc_d : acc_u;
-            const Real inv_ratio = wrcp<Real>((Real)(h ? Rd : Ru));
-            Real f = (r == i) ? (Real)1 - inv_ratio : sk * inv_ratio;
-            f = (acc_h && r < NP) ? f : (Real)0;
-#pragma unroll
-            for (int m = 0; m < NP; ++m) L.inv[m] -= sh.row[h][m] * f;
-        }
-        // Jastrow row sums, the up move's change then the down move's (the
-        // up electron, when it moved, pairs with the down electron's old
-        // position from its proposal: unu_ld).
-        Real po_d = (Real)0;
-        if (acc_d && has_e && lane != ld)
-            po_d = (lane == lu && acc_u) ? unu_ld : pair_with(c, L, xdo, ydo, zdo, h == 1 ? c.a_same : c.a_anti);
-        if (has_e) {
-            if (lane == ld && acc_d) {
-                L.U = jn_d;
-            } else {
-                if (lane == lu && acc_u) L.U = jn_u;
-                else if (acc_u) L.U += (double)un_u - (double)po_u;
-                if (acc_d) L.U += (double)((lane == lu && acc_u) ? ud_new : un_d) - (double)po_d;
-            }
-        }
-        if (lane == lu && acc_u) {
-            L.x = xu;
-            L.y = yu;
-            L.z = zu;
-            sh.pos[0][lane] = xu;
-            sh.pos[1][lane] = yu;
-            sh.pos[2][lane] = zu;
-        }
-        if (lane == ld && acc_d) {
-            L.x = xd;
-            L.y = yd;
-            L.z = zd;
-            sh.pos[0][lane] = xd;
-            sh.pos[1][lane] = yd;
-            sh.pos[2][lane] = zd;
-        }
-        if (acc_u) ldt.add(0, Ru);
-        if (acc_d) ldt.add(1, Rd);
         return acc_u + acc_d;
     }
 
@@ -1092,7 +1076,7 @@ struct RRKernel {
         if (!accepted) return 0;
         if (lane == lp) {
 #pragma unroll
-            for (int m = 0; m < NP; ++m) sh.row[0][m] = L.inv[m];
+            for (int m = 0; m < NP; ++m) sh.row[m] = L.inv[m];
         }
         __syncthreads();
         {
@@ -1100,7 +1084,7 @@ struct RRKernel {
             Real f = (r == rp) ? (Real)1 - inv_ratio : sk * inv_ratio;
             f = (accepted && h == HP && r < NP) ? f : (Real)0;
 #pragma unroll
-            for (int m = 0; m < NP; ++m) L.inv[m] -= sh.row[0][m] * f;
+            for (int m = 0; m < NP; ++m) L.inv[m] -= sh.row[m] * f;
         }
         if (!accepted) return 0;
         ldt.add(HP, R);
@@ -1200,7 +1184,6 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
     sh.pos[1][lane] = L.y;
     sh.pos[2][lane] = L.z;
     if (lane < kStatCount) sh.stat[lane] = st.stats[(size_t)w * kStatCount + lane];
-    (&sh.row[0][0])[lane] = (Real)0;  // finite rows for the lanes of a rejected spin
     if (lane == 0) {
         sh.misc[0] = st.sigma[w];
         sh.misc[1] = st.logdet[2 * w + 0];
