@@ -476,6 +476,8 @@ def main():
                     help="skip the served-stream, batch and other-configuration runs")
     ap.add_argument("--configs-only", action="store_true",
                     help="run only the other-configuration timings (for profiling them)")
+    ap.add_argument("--no-physics", action="store_true",
+                    help="skip the 500+5000-sweep physics runs (for profiling the timed kernels)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run plain device copies of 1 GiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -577,13 +579,18 @@ def main():
         line["batch"] = batch_run(V, torch)
         log("other configurations")
         line["configs"] = configs_run(V, torch)
-    # Physics at the configuration's stated length, independent seeds.
+    # Physics at the configuration's stated length: four runs per path, every
+    # run with its own seed, pooled (mean of the four, standard error
+    # sqrt(sum se^2) / 4).
     log("physics runs")
     phys = {}
-    if "mixed" in res:
-        phys["mixed"] = physics_run(V, torch, V.PREC_MIXED, SEED + 1)
-    if "fp64" in res:
-        phys["fp64"] = physics_run(V, torch, V.PREC_FP64, SEED + 2)
+    for k, prec, off in (("mixed", V.PREC_MIXED, 1), ("fp64", V.PREC_FP64, 2)):
+        if k not in res or a.no_physics:
+            continue
+        runs = [physics_run(V, torch, prec, SEED + off + 2 * i) for i in range(4)]
+        ses = [r["se_per_electron"] for r in runs]
+        phys[k] = {"runs": runs, "e_per_electron": sum(r["e_per_electron"] for r in runs) / len(runs),
+                   "se_per_electron": (sum(x * x for x in ses) ** 0.5 / len(runs)) if all(ses) else None}
     if len(phys) == 2 and phys["mixed"]["se_per_electron"] and phys["fp64"]["se_per_electron"]:
         d = phys["mixed"]["e_per_electron"] - phys["fp64"]["e_per_electron"]
         phys["difference_in_sigma"] = abs(d) / (phys["mixed"]["se_per_electron"] ** 2 +

@@ -202,3 +202,28 @@ def test_batch_members_refuse_direct_sweeps(vmc):
     with pytest.raises(vmc.VmcError):
         m.sweep(1)
     b.close()
+
+
+def test_batch_mixes_one_and_two_determinant_members(vmc):
+    """Generic-kernel batch in mixed precision with K > 0 holding a polarised
+    system (one determinant of 19) and an unpolarised one (two of 19): their
+    re-inversions form separate sets by order and determinant count, so each
+    member equals its alone run and both determinants' drift stays small."""
+    import dataclasses
+
+    pol = dataclasses.replace(vmc.System.reference(19, 6.0, precision=vmc.PREC_MIXED), reinvert_interval=8)
+    unpol = vmc.System.heg(38, 2.0, 1.0, precision=vmc.PREC_MIXED, reinvert_interval=8)
+    sy, W, seeds = [pol, unpol], [24, 24], [31, 32]
+    b = vmc.Batch(sy, W, seeds)
+    b.set_kernel(vmc.KERNEL_GENERIC)
+    b.init_positions()
+    b.sweep(2, adapt=True, block_size=2)
+    b.sweep(3, measure=True, block_size=2)
+    got = [(b.member(i).positions(), b.member(i).stats(), b.member(i).steps()) for i in range(2)]
+    drift = [b.member(i).drift() for i in range(2)]
+    b.close()
+    for i, s in enumerate(sy):
+        ref = alone(vmc, s, W[i], seeds[i], vmc.KERNEL_GENERIC)
+        assert_same([got[i]], [ref])
+        assert np.all(ref[1][:, vmc.STAT["REINV"]] >= 1), i
+        assert np.all(drift[i] < 1e-3), (i, drift[i].max())
