@@ -38,7 +38,8 @@ enum {
     VMC_ERR_SINGULAR = 3, /* no non-singular start found (reference: std::runtime_error,
                              simulation.cpp:68-70) */
     VMC_ERR_MEMORY = 4,   /* device allocation failed */
-    VMC_ERR_STATE = 5     /* call out of order */
+    VMC_ERR_STATE = 5,    /* call out of order */
+    VMC_ERR_CANCELLED = 6 /* a served request ended by vmc_server_cancel */
 };
 
 enum { VMC_PREC_FP64 = 0, VMC_PREC_MIXED = 1 };
@@ -434,6 +435,13 @@ int vmc_server_submit(vmc_server *s, const vmc_request *req, int64_t *id);
 /* Up to `capacity` finished results; with wait != 0 blocks until one is
  * ready or nothing is in flight. */
 int vmc_server_poll(vmc_server *s, vmc_result *out, int32_t capacity, int32_t *n_out, int32_t wait);
+/* Cancels request `id`: a request still waiting for admission ends at once
+ * (no sweeps), a running one after the turn in flight; either reports a
+ * result with status VMC_ERR_CANCELLED and the moves its sweeps made.
+ * VMC_ERR_STATE when the request already finished or the id is unknown.
+ * No reference counterpart: the reference's chains always run to the end
+ * (src/main.cpp:50-68, futures joined at :75-76). */
+int vmc_server_cancel(vmc_server *s, int64_t id);
 /* Finishes every submitted request, then stops the worker. */
 int vmc_server_destroy(vmc_server *s);
 

@@ -114,3 +114,37 @@ def test_server_reports_a_failed_sweep(vmc):
     assert res[bad]["moves"] == 32 * 14 * 3  # the warmup sweeps only
     assert res[good]["status"] == 0
     assert res[good]["moves"] == 32 * 14 * 8
+
+
+def test_server_cancel(vmc):
+    """A running request ends after its turn in flight and a waiting one at
+    once; both report ERR_CANCELLED with the moves their sweeps made, and the
+    request admitted after them completes as it does alone."""
+    s = vmc.System.heg(54, 2.0, 1.0, precision=vmc.PREC_MIXED, reinvert_interval=64)
+    small = vmc.System.heg(14, 1.0, 1.0)
+    srv = vmc.Server(1 << 34, max_active=1)
+    long_id = srv.submit(s, 256, 2, 200000, 31)   # minutes of sweeps if left alone
+    waiting = srv.submit(s, 256, 2, 10, 32)
+    after = srv.submit(small, 32, 3, 5, 33, block_size=1)
+    srv.cancel(waiting)
+    time.sleep(0.3)
+    srv.cancel(long_id)
+    res = collect(srv, 3)
+    with pytest.raises(vmc.VmcError):
+        srv.cancel(long_id)       # already reported
+    with pytest.raises(vmc.VmcError):
+        srv.cancel(after + 100)   # never submitted
+    srv.close()
+    print({k: (v["status"], v["moves"], round(v["latency_s"], 3)) for k, v in res.items()}, flush=True)
+    assert res[waiting]["status"] == vmc.ERR_CANCELLED and res[waiting]["moves"] == 0
+    r = res[long_id]
+    assert r["status"] == vmc.ERR_CANCELLED
+    assert 0 < r["moves"] < 256 * 54 * 200002 and r["moves"] % (256 * 54) == 0
+    assert r["latency_s"] < 30.0
+    assert res[after]["status"] == 0 and res[after]["moves"] == 32 * 14 * 8
+    alone = vmc.Server(1 << 34, max_active=1)
+    i = alone.submit(small, 32, 3, 5, 33, block_size=1)
+    ref = collect(alone, 1)[i]
+    alone.close()
+    for k in ("mean_energy", "standard_error", "acceptance_rate"):
+        assert ref[k] == res[after][k], k

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <set>
 #include <thread>
 #include <vector>
 
@@ -192,6 +193,8 @@ struct vmc_server {
     int64_t next_id = 1;
     int64_t used = 0;
     int64_t in_flight = 0;  // submitted, result not yet in `done`
+    std::set<int64_t> live;     // ids submitted whose result is not yet in `done`
+    std::set<int64_t> cancels;  // running requests to end after their turn in flight
     bool stopping = false;
     std::vector<hipStream_t> free_streams;
     std::vector<hipEvent_t> free_events;
@@ -209,6 +212,7 @@ void vmc_server::finish(Active &a, std::vector<vmc_result> &out) {
     std::vector<double> st(static_cast<size_t>(a.p.req.n_walkers) * VMC_STAT_COUNT);
     if (hipStreamSynchronize(a.stream) != hipSuccess && r.status == VMC_OK) r.status = VMC_ERR_DEVICE;
     if (r.status == VMC_OK) r.status = vmc_ensemble_read_stats(a.e, st.data(), a.stream);
+    // (a cancelled request reports its moves and times, no energy)
     if (r.status == VMC_OK) aggregate(st, a.p.req.n_walkers, &r.mean_energy, &r.standard_error, &r.acceptance_rate);
     const int N = a.p.req.system.n_up + a.p.req.system.n_dn;
     r.moves = static_cast<double>(a.p.req.n_walkers) * N * static_cast<double>(a.sweeps_done);
@@ -275,6 +279,16 @@ void vmc_server::run() {
             a.meas_left = p.req.measure_sweeps;
             active.push_back(std::move(a));
         }
+        // Cancelled requests issue no more turns; they report once their
+        // turns in flight complete.
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (Active &a : active)
+                if (cancels.erase(a.p.id)) {
+                    if (a.status == VMC_OK) a.status = VMC_ERR_CANCELLED;
+                    a.warm_left = a.meas_left = 0;
+                }
+        }
         // One turn per active request: a few sweeps on its own stream, at
         // most two turns in flight so every stream's queue stays short.
         bool issued = false;
@@ -333,7 +347,11 @@ void vmc_server::run() {
         }
         if (!results.empty()) {
             std::lock_guard<std::mutex> lk(mu);
-            for (auto &r : results) done.push_back(r);
+            for (auto &r : results) {
+                done.push_back(r);
+                live.erase(r.id);
+                cancels.erase(r.id);
+            }
             in_flight -= static_cast<int64_t>(results.size());
             cv.notify_all();
         }
@@ -370,6 +388,7 @@ int vmc_server_submit(vmc_server *s, const vmc_request *req, int64_t *id) {
     if (bytes > s->budget) return VMC_ERR_MEMORY;
     *id = s->next_id++;
     ++s->in_flight;
+    s->live.insert(*id);
     s->pending.push_back({*id, *req, now_s(), bytes});
     s->cv.notify_all();
     return VMC_OK;
@@ -384,6 +403,29 @@ int vmc_server_poll(vmc_server *s, vmc_result *out, int32_t capacity, int32_t *n
     for (int32_t i = 0; i < n; ++i) out[i] = s->done[static_cast<size_t>(i)];
     s->done.erase(s->done.begin(), s->done.begin() + n);
     *n_out = n;
+    return VMC_OK;
+}
+
+int vmc_server_cancel(vmc_server *s, int64_t id) {
+    if (!s) return VMC_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->live.count(id)) return VMC_ERR_STATE;
+    for (auto it = s->pending.begin(); it != s->pending.end(); ++it) {
+        if (it->id != id) continue;
+        vmc_result r{};
+        r.id = id;
+        r.status = VMC_ERR_CANCELLED;
+        r.walkers = it->req.n_walkers;
+        r.standard_error = -1.0;
+        r.queued_s = r.latency_s = now_s() - it->t_submit;
+        s->pending.erase(it);
+        s->done.push_back(r);
+        s->live.erase(id);
+        --s->in_flight;
+        s->cv.notify_all();
+        return VMC_OK;
+    }
+    s->cancels.insert(id);  // admitted: the worker ends it after its turn in flight
     return VMC_OK;
 }
 

@@ -21,8 +21,8 @@ HEADER_PATH = os.path.join(REPO_DIR, "include", "vmc_hip.h")
 
 VMC_OK = 0
 ERRORS = {1: "invalid argument", 2: "device error", 3: "singular", 4: "out of memory",
-          5: "bad state"}
-ERR_INVALID, ERR_DEVICE, ERR_SINGULAR, ERR_MEMORY, ERR_STATE = 1, 2, 3, 4, 5
+          5: "bad state", 6: "cancelled"}
+ERR_INVALID, ERR_DEVICE, ERR_SINGULAR, ERR_MEMORY, ERR_STATE, ERR_CANCELLED = 1, 2, 3, 4, 5, 6
 
 PREC_FP64, PREC_MIXED = 0, 1
 MOVE_UNIFORM, MOVE_GAUSSIAN = 0, 1
@@ -204,6 +204,7 @@ def lib():
             "vmc_server_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(_VP)]),
             "vmc_server_submit": (C.c_int, [_VP, C.POINTER(Request), C.POINTER(C.c_int64)]),
             "vmc_server_poll": (C.c_int, [_VP, C.POINTER(Result), C.c_int32, _ip, C.c_int32]),
+            "vmc_server_cancel": (C.c_int, [_VP, C.c_int64]),
             "vmc_server_destroy": (C.c_int, [_VP]),
             "vmc_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
                                            _dp, _ip, C.c_int32, C.c_int32, C.POINTER(_VP)]),
@@ -588,6 +589,10 @@ class Server:
         n = C.c_int32()
         _check(lib().vmc_server_poll(self._h, buf, capacity, C.byref(n), 1 if wait else 0))
         return [buf[i].as_dict() for i in range(n.value)]
+
+    def cancel(self, request_id):
+        """Ends a request: its result reports status ERR_CANCELLED."""
+        _check(lib().vmc_server_cancel(self._h, int(request_id)))
 
     def close(self):
         if self._h:
