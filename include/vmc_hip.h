@@ -445,6 +445,111 @@ int vmc_server_cancel(vmc_server *s, int64_t id);
 /* Finishes every submitted request, then stops the worker. */
 int vmc_server_destroy(vmc_server *s);
 
+/* ------------------------------------------------------------------ per-move objects
+ * One configuration's determinant, Jastrow and Ewald pieces with the
+ * reference's per-move calls.  The state the reference keeps in class members
+ * is resident on the current device.  Each call is one small launch on the
+ * object's own stream and returns when its result is on the host.  FP64, the
+ * reference's physics: one plane-wave determinant over all n particles, one
+ * Pade a, alpha = 6/L, the A&S 7.1.26 erfc.  They back the host facade's
+ * SlaterPlaneWave, JastrowPade, EnergyTracker and WaveFunction classes for
+ * one walker; the ensemble API above is the throughput path.  Positions are
+ * caller-owned SoA arrays x[n], y[n], z[n] as in Particles
+ * (particles.hpp:10-75), n in [1, VMC_SINGLE_MAX_N].  Objects are not
+ * thread-safe (as the reference's). */
+#define VMC_SINGLE_MAX_N 1024
+
+typedef struct vmc_slater vmc_slater;
+/* SlaterPlaneWave(particles, L) (slater_plane_wave.cpp:30-152): closed-shell
+ * orbitals, k = 2 pi n / L, zeroed trig cache and matrices. */
+int vmc_slater_create(int32_t n, double box_length, vmc_slater **out);
+int vmc_slater_destroy(vmc_slater *s);
+/* Unique k count and the row strides of the n x n matrices and of the trig
+ * cache: n and n_unique_k padded to 8 doubles, as the reference pads
+ * (aligned_soa.hpp:63-65, slater_plane_wave.cpp:31-33,138). */
+int vmc_slater_shape(const vmc_slater *s, int32_t *n_unique_k, int32_t *matrix_stride, int32_t *trig_stride);
+/* n vectors and k vectors [n_unique_k][3], orbital -> k index [n], orbital
+ * type [n] (0 cos, 1 sin) (slater_plane_wave.hpp:83-95,108-138). */
+int vmc_slater_tables(const vmc_slater *s, int32_t *n_vec, double *k_vec, int32_t *orbital_k,
+                      uint8_t *orbital_type);
+/* log_abs_det (slater_plane_wave.cpp:202-318): trig cache of every particle,
+ * D, partial-pivot LU with pivot tolerance 1e-12 (matrix.hpp:13-59),
+ * log|det| = sum log|U_ii|, and the transposed inverse by n LU solves
+ * (matrix.hpp:61-96).  -inf for a singular D, the inverse then unchanged. */
+int vmc_slater_log_abs_det(vmc_slater *s, const double *x, const double *y, const double *z,
+                           double *log_abs_det);
+/* update_trig_cache, save_trig_row, restore_trig_row
+ * (slater_plane_wave.cpp:154-200); p in [0, n). */
+int vmc_slater_update_trig(vmc_slater *s, int32_t p, double x, double y, double z);
+int vmc_slater_save_trig(vmc_slater *s, int32_t p);
+int vmc_slater_restore_trig(vmc_slater *s, int32_t p);
+/* build_row (slater_plane_wave.cpp:320-349): the n orbitals at particle p's
+ * trig cache row, copied to row. */
+int vmc_slater_build_row(vmc_slater *s, int32_t p, double *row);
+/* determinant_ratio (slater_plane_wave.cpp:351-365). */
+int vmc_slater_ratio(vmc_slater *s, int32_t p, const double *row, double *ratio);
+/* accept_move (slater_plane_wave.cpp:367-423): Sherman-Morrison update of the
+ * transposed inverse and row p of D replaced by row. */
+int vmc_slater_accept(vmc_slater *s, int32_t p, const double *row, double ratio);
+/* add_derivatives (slater_plane_wave.cpp:425-499): adds grad log|D| and
+ * lap log|D| of every particle into the arrays (in/out, n each). */
+int vmc_slater_add_derivatives(vmc_slater *s, double *grad_x, double *grad_y, double *grad_z, double *lap);
+/* Copies of the device state: D, the transposed inverse and the LU factors
+ * [n][matrix_stride]; the sin and cos caches [n][trig_stride]; pivots [n]. */
+enum { VMC_SLATER_D = 0, VMC_SLATER_INV = 1, VMC_SLATER_LU = 2, VMC_SLATER_SIN = 3, VMC_SLATER_COS = 4 };
+int vmc_slater_read(const vmc_slater *s, int32_t which, double *out);
+int vmc_slater_pivots(const vmc_slater *s, int32_t *pivots);
+
+typedef struct vmc_pade vmc_pade;
+/* JastrowPade(L, a, b) (jastrow_pade.hpp:5-44): u(r) = a r / (1 + b r) over
+ * minimum-image pairs.  Positions are passed with every call. */
+int vmc_pade_create(double box_length, double a, double b, vmc_pade **out);
+int vmc_pade_destroy(vmc_pade *j);
+/* value (jastrow_pade.cpp:7-55): J = sum over pairs of u(r_ij). */
+int vmc_pade_value(vmc_pade *j, int32_t n, const double *x, const double *y, const double *z, double *value);
+/* add_derivatives (jastrow_pade.cpp:57-138): adds grad_i J and lap_i J
+ * (in/out arrays). */
+int vmc_pade_add_derivatives(vmc_pade *j, int32_t n, const double *x, const double *y, const double *z,
+                             double *grad_x, double *grad_y, double *grad_z, double *lap);
+/* delta_value (jastrow_pade.cpp:140-200): the positions hold particle `moved`
+ * at its new place, old_x/y/z is where it was. */
+int vmc_pade_delta_value(vmc_pade *j, int32_t n, const double *x, const double *y, const double *z, int32_t moved,
+                         double old_x, double old_y, double old_z, double *delta);
+/* update_derivatives_for_move (jastrow_pade.cpp:203-317): updates Jastrow
+ * grad/lap arrays (in/out) for one move, positions as for delta_value. */
+int vmc_pade_update_for_move(vmc_pade *j, int32_t n, const double *x, const double *y, const double *z,
+                             int32_t moved, double old_x, double old_y, double old_z, double *grad_x,
+                             double *grad_y, double *grad_z, double *lap);
+
+typedef struct vmc_ewald vmc_ewald;
+/* EnergyTracker(L, N) (energy_tracking.cpp:6-70): alpha = 6/L, half-space G
+ * with G^2 <= 4 alpha^2 ln 1e6 and their weights, self and background terms;
+ * S_G, V_real and V_recip are 0 until initialised. */
+int vmc_ewald_create(double box_length, double num_particles, vmc_ewald **out);
+int vmc_ewald_destroy(vmc_ewald *t);
+/* initialize_structure_factors (energy_tracking.cpp:146-191). */
+int vmc_ewald_init_structure_factors(vmc_ewald *t, int32_t n, const double *x, const double *y, const double *z);
+/* initialize_reciprocal_energy from the cached S_G (energy_tracking.cpp:72-91). */
+int vmc_ewald_init_reciprocal(vmc_ewald *t);
+/* initialize_real_energy (energy_tracking.cpp:93-144). */
+int vmc_ewald_init_real(vmc_ewald *t, int32_t n, const double *x, const double *y, const double *z);
+/* update_structure_factors (energy_tracking.cpp:193-251): S_G and V_recip
+ * for one particle moved from old to new. */
+int vmc_ewald_update_structure_factors(vmc_ewald *t, double old_x, double old_y, double old_z, double new_x,
+                                       double new_y, double new_z);
+/* update_real_energy (energy_tracking.cpp:280-365): positions with particle
+ * `moved` at its new place. */
+int vmc_ewald_update_real(vmc_ewald *t, int32_t n, const double *x, const double *y, const double *z, int32_t moved,
+                          double old_x, double old_y, double old_z);
+/* eval_total_energy (energy_tracking.cpp:253-278,367-382): -1/2 sum(lap +
+ * |grad|^2) of the arrays plus the cached V_real + V_recip + self +
+ * background. */
+int vmc_ewald_total_energy(vmc_ewald *t, int32_t n, const double *grad_x, const double *grad_y,
+                           const double *grad_z, const double *lap, double *energy);
+/* Cached state; s_re and s_im ([n_g] each) may be NULL. */
+int vmc_ewald_state(const vmc_ewald *t, double *v_real, double *v_recip, double *self_term, double *background,
+                    int32_t *n_g, double *s_re, double *s_im);
+
 #ifdef __cplusplus
 }
 #endif

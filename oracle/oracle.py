@@ -369,6 +369,8 @@ def ref():
         R.ref_simulation_run.argtypes = [C.c_size_t, C.c_double, C.c_size_t, C.c_size_t,
                                          C.c_double, C.c_uint64, C.c_size_t, C.c_double,
                                          C.c_double, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _dp]
+        R.ref_move_chain.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, C.c_int,
+                                     _ip, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _dp]
         R.ref_blocking.argtypes = [C.c_size_t, _dp, C.c_size_t, _dp, _dp]
         R.ref_write_fixed.argtypes = [C.c_int, C.c_char_p, _ip]
         R.ref_config.argtypes = [C.c_char_p, _dp]
@@ -447,6 +449,23 @@ def ref_local_energy(pos, L, a=0.25, b=1.0):
     g, lp = np.zeros((n, 3)), np.zeros(n)
     ref().ref_local_energy(n, L, a, b, _ptr(pos), C.byref(lp_), _ptr(g), _ptr(lp), C.byref(e))
     return lp_.value, g, lp, e.value
+
+
+def ref_move_chain(pos, L, a, b, moved, newpos, accept):
+    """The reference's per-move sequence (Simulation::metropolis_step and the
+    measured step's derivative and energy calls) over given moves: per move
+    (ratio, delta J, E_L after it), then the final transposed inverse and
+    grad / lap of ln psi."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n, M = pos.shape[0], len(moved)
+    mv = np.ascontiguousarray(moved, dtype=np.int32)
+    q = np.ascontiguousarray(newpos, dtype=np.float64).reshape(M, 3)
+    acc = np.ascontiguousarray(accept, dtype=np.int32)
+    r, dj, e = np.zeros(M), np.zeros(M), np.zeros(M)
+    inv, g, lp = np.zeros((n, n)), np.zeros((n, 3)), np.zeros(n)
+    ref().ref_move_chain(n, L, a, b, _ptr(pos), M, _ptr(mv, _ip), _ptr(q), _ptr(acc, _ip), _ptr(r),
+                         _ptr(dj), _ptr(e), _ptr(inv), _ptr(g), _ptr(lp))
+    return r, dj, e, inv, g, lp
 
 
 def ref_simulation_run(n, L, warmup_steps, measure_steps, step, seed, block, a=0.25, b=1.0,

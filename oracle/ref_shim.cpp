@@ -238,6 +238,63 @@ int ref_local_energy(int n, double L, double a, double b, const double* pos, dou
     return 0;
 }
 
+// The per-move call sequence of Simulation::metropolis_step and measure
+// (simulation.cpp:80-156,212-222) over given moves: move m takes particle
+// moved[m] to newpos[m] (already in the box) and is accepted iff accept[m].
+// Records per move the determinant ratio, delta J and the local energy after
+// the move; at the end the transposed inverse and grad/lap of ln psi.
+int ref_move_chain(int n, double L, double a, double b, const double* pos, int n_moves,
+                   const int* moved, const double* newpos, const int* accept, double* ratio,
+                   double* delta_j, double* e_local, double* inv_final, double* grad_final,
+                   double* lap_final) {
+    Particles p{static_cast<std::size_t>(n)};
+    load_positions(p, pos);
+    WaveFunction wf{p, L, a, b};
+    (void)wf.evaluate_log_psi(p);
+    wf.evaluate_derivatives(p);
+    EnergyTracker t{L, static_cast<double>(n)};
+    t.initialize_structure_factors(p);
+    t.initialize_reciprocal_energy();
+    t.initialize_real_energy(p);
+    auto& s{wf.slater_plane_wave_get()};
+    double* x{p.pos_x_get()};
+    double* y{p.pos_y_get()};
+    double* z{p.pos_z_get()};
+    for (int m = 0; m < n_moves; ++m) {
+        const std::size_t k{static_cast<std::size_t>(moved[m])};
+        const double ox{x[k]}, oy{y[k]}, oz{z[k]};
+        x[k] = newpos[3 * m + 0];
+        y[k] = newpos[3 * m + 1];
+        z[k] = newpos[3 * m + 2];
+        s.save_trig_row(k);
+        s.update_trig_cache(k, p);
+        const double* row{s.build_row(k)};
+        ratio[m] = s.determinant_ratio(k, row);
+        delta_j[m] = wf.jastrow_pade_get().delta_value(p, k, ox, oy, oz);
+        if (accept[m]) {
+            s.accept_move(k, row, ratio[m]);
+            t.update_structure_factors(ox, oy, oz, x[k], y[k], z[k]);
+            t.update_real_energy(k, ox, oy, oz, p);
+        } else {
+            s.restore_trig_row(k);
+            x[k] = ox;
+            y[k] = oy;
+            z[k] = oz;
+        }
+        wf.evaluate_derivatives(p, accept[m] != 0, k, ox, oy, oz);
+        e_local[m] = t.eval_total_energy(p);
+    }
+    copy_matrix(s.inv_determinant_get(), static_cast<std::size_t>(n), s.matrix_row_stride_get(),
+                inv_final);
+    for (int i = 0; i < n; ++i) {
+        grad_final[3 * i + 0] = p.grad_log_psi_x_get()[i];
+        grad_final[3 * i + 1] = p.grad_log_psi_y_get()[i];
+        grad_final[3 * i + 2] = p.grad_log_psi_z_get()[i];
+        lap_final[i] = p.lap_log_psi_get()[i];
+    }
+    return 0;
+}
+
 // Simulation::run (simulation.cpp:265-290) with a recording writer.
 int ref_simulation_run(std::size_t n, double L, std::size_t warmup_steps,
                        std::size_t measure_steps, double step_size, std::uint64_t seed,

@@ -962,6 +962,12 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
 
 }  // namespace
 
+namespace vmc {
+// Sets the message vmc_last_error() returns on this thread (for the
+// library's other translation units); returns code.
+int set_last_error(int code, const char *msg) { return fail(code, "%s", msg); }
+}  // namespace vmc
+
 extern "C" {
 
 const char *vmc_last_error(void) { return g_last_error.c_str(); }

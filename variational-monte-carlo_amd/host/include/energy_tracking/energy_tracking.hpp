@@ -1,35 +1,45 @@
-// energy_tracking/energy_tracking.hpp -- Ewald potential and local energy of
-// one configuration, evaluated on the device.
+// energy_tracking/energy_tracking.hpp -- Ewald potential of one configuration
+// with the reference's cached, per-move updated terms, on the device.
 //
 // Public interface of the reference's EnergyTracker
-// (src/energy_tracking/energy_tracking.hpp:36-53) with its physics: alpha =
-// 6/L, the G^2 <= 4 alpha^2 ln 1e6 reciprocal set, the A&S 7.1.26 erfc over
-// every minimum-image pair, self and background terms
-// (src/energy_tracking/energy_tracking.cpp:6-70).  eval_total_energy
-// evaluates V_real and V_recip of the current positions in FP64 on the device
-// (vmc_frozen_eval) and adds the kinetic energy -1/2 sum(lap + |grad|^2) of
-// the Particles arrays, as the reference does
-// (energy_tracking.cpp:253-278,367-382).  The class holds no per-move state:
-// the reference's initialise/update calls (energy_tracking.hpp:39-47), which
-// keep its structure factors and pair sums current between evaluations, have
-// no counterpart here; in the device sweep S(G) and the pair sums are
-// rebuilt at each measurement.
+// (src/energy_tracking/energy_tracking.hpp:10-53) over the vmc_ewald object of
+// the C ABI: alpha = 6/L, the half-space G set with G^2 <= 4 alpha^2 ln 1e6,
+// the structure factors S_G and the cached V_real and V_recip live in device
+// memory.  initialize_* build them from a configuration, update_* follow one
+// accepted move, and eval_total_energy adds the kinetic energy
+// -1/2 sum(lap + |grad|^2) of the Particles arrays to the cached potentials
+// and the self and background constants
+// (src/energy_tracking/energy_tracking.cpp:6-382).  As in the reference, the
+// potentials are 0 until initialised.  A device failure throws
+// std::runtime_error.  Movable, not copyable.
 #pragma once
 
 #include "particles/particles.hpp"
 
 #include <cstddef>
 
+struct vmc_ewald;
+
 class EnergyTracker {
 public:
     explicit EnergyTracker(double box_length, double num_particles);
+    ~EnergyTracker();
+    EnergyTracker(EnergyTracker &&other) noexcept;
+    EnergyTracker &operator=(EnergyTracker &&other) noexcept;
+    EnergyTracker(const EnergyTracker &) = delete;
+    EnergyTracker &operator=(const EnergyTracker &) = delete;
+
+    void initialize_reciprocal_energy();
+    void initialize_real_energy(const Particles &particles);
+    void initialize_structure_factors(const Particles &particles);
+
+    void update_structure_factors(double old_x, double old_y, double old_z, double new_x, double new_y,
+                                  double new_z);
+    void update_real_energy(std::size_t moved_idx, double old_x, double old_y, double old_z,
+                            const Particles &particles);
 
     double eval_total_energy(const Particles &particles) const;
 
-    // The potential part alone, V_real + V_recip + self + background.
-    double potential_energy(const Particles &particles) const;
-
 private:
-    double box_length_;
-    std::size_t n_;
+    vmc_ewald *h_ = nullptr;
 };

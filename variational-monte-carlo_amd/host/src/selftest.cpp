@@ -7,11 +7,14 @@
 //   vmc_host_selftest select <b_min> <b e>... the optimiser's choice
 //   vmc_host_selftest wavefunction <n> <L> <a> <b> <file of 3n positions>
 //                                            ln psi, E_L, grad/lap (needs a GPU)
+//   vmc_host_selftest chain <file>           the reference's per-move call
+//                                            sequence over given moves (needs a GPU)
 #include "blocking_analysis/blocking_analysis.hpp"
 #include "config/config.hpp"
 #include "energy_tracking/energy_tracking.hpp"
 #include "optimizer/jastrow_optimizer.hpp"
 #include "particles/particles.hpp"
+#include "slater_plane_wave/slater_plane_wave.hpp"
 #include "wavefunction/wavefunction.hpp"
 #include "output_writer/output_writer.hpp"
 
@@ -109,11 +112,77 @@ int wavefunction_cmd(char **argv) {
     const double lp = wf.evaluate_log_psi(p);
     wf.evaluate_derivatives(p);
     EnergyTracker et{L, static_cast<double>(n)};
+    et.initialize_structure_factors(p);
+    et.initialize_reciprocal_energy();
+    et.initialize_real_energy(p);
     const double e = et.eval_total_energy(p);
     std::cout << std::setprecision(std::numeric_limits<double>::max_digits10) << "log_psi=" << lp
               << "\ne_local=" << e << "\n";
     for (std::size_t i = 0; i < n; ++i)
         std::cout << "gl=" << p.grad_log_psi_x_get()[i] << " " << p.grad_log_psi_y_get()[i] << " "
+                  << p.grad_log_psi_z_get()[i] << " " << p.lap_log_psi_get()[i] << "\n";
+    return 0;
+}
+
+// Input: "n L a b M", n lines of positions, M lines "particle x y z accepted".
+// Runs Simulation::metropolis_step's calls on the facade classes with the
+// given proposals and decisions, then the measured step's derivative and
+// energy calls (reference simulation.cpp:80-156,212-222).  Prints per move
+// "move ratio delta_j e_local", then the final transposed inverse rows
+// ("inv ...") and grad/lap ("gl ...").
+int chain_cmd(const char *path) {
+    std::ifstream in{path};
+    std::size_t n = 0, M = 0;
+    double L = 0.0, a = 0.0, b = 0.0;
+    in >> n >> L >> a >> b >> M;
+    Particles p{n};
+    double *x = p.pos_x_get(), *y = p.pos_y_get(), *z = p.pos_z_get();
+    for (std::size_t i = 0; i < n; ++i) in >> x[i] >> y[i] >> z[i];
+    WaveFunction wf{p, L, a, b};
+    (void)wf.evaluate_log_psi(p);
+    wf.evaluate_derivatives(p);
+    EnergyTracker et{L, static_cast<double>(n)};
+    et.initialize_structure_factors(p);
+    et.initialize_reciprocal_energy();
+    et.initialize_real_energy(p);
+    SlaterPlaneWave &s = wf.slater_plane_wave_get();
+    std::cout << std::setprecision(std::numeric_limits<double>::max_digits10);
+    for (std::size_t m = 0; m < M; ++m) {
+        std::size_t k = 0;
+        int accepted = 0;
+        double nx = 0.0, ny = 0.0, nz = 0.0;
+        in >> k >> nx >> ny >> nz >> accepted;
+        if (!in || k >= n) throw std::invalid_argument("chain: bad move line");
+        const double ox = x[k], oy = y[k], oz = z[k];
+        x[k] = nx;
+        y[k] = ny;
+        z[k] = nz;
+        s.save_trig_row(k);
+        s.update_trig_cache(k, p);
+        const double *row = s.build_row(k);
+        const double ratio = s.determinant_ratio(k, row);
+        const double dj = wf.jastrow_pade_get().delta_value(p, k, ox, oy, oz);
+        if (accepted) {
+            s.accept_move(k, row, ratio);
+            et.update_structure_factors(ox, oy, oz, nx, ny, nz);
+            et.update_real_energy(k, ox, oy, oz, p);
+        } else {
+            s.restore_trig_row(k);
+            x[k] = ox;
+            y[k] = oy;
+            z[k] = oz;
+        }
+        wf.evaluate_derivatives(p, accepted != 0, k, ox, oy, oz);
+        std::cout << "move " << m << " " << ratio << " " << dj << " " << et.eval_total_energy(p) << "\n";
+    }
+    const std::size_t S = s.matrix_row_stride_get();
+    for (std::size_t r = 0; r < n; ++r) {
+        std::cout << "inv";
+        for (std::size_t c = 0; c < n; ++c) std::cout << " " << s.inv_determinant_get()[r * S + c];
+        std::cout << "\n";
+    }
+    for (std::size_t i = 0; i < n; ++i)
+        std::cout << "gl " << p.grad_log_psi_x_get()[i] << " " << p.grad_log_psi_y_get()[i] << " "
                   << p.grad_log_psi_z_get()[i] << " " << p.lap_log_psi_get()[i] << "\n";
     return 0;
 }
@@ -128,6 +197,7 @@ int main(int argc, char **argv) {
         if (argc >= 3 && !std::strcmp(argv[1], "scan")) return scan_cmd(argv[2]);
         if (argc >= 5 && !std::strcmp(argv[1], "select")) return select_cmd(argc, argv);
         if (argc >= 7 && !std::strcmp(argv[1], "wavefunction")) return wavefunction_cmd(argv);
+        if (argc >= 3 && !std::strcmp(argv[1], "chain")) return chain_cmd(argv[2]);
         std::cerr << "usage: vmc_host_selftest config <path> | write <fmt> <out> | blocking <bs> <x...>\n";
         return 2;
     } catch (const std::exception &e) {
