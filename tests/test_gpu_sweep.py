@@ -443,3 +443,26 @@ def test_kernel_switches_keep_the_oracle_twin(vmc, oracle, name, factory):
                                        measure=True)
             np.testing.assert_allclose(pos[w], walkers[w].positions(), rtol=0, atol=1e-9)
             assert abs(st[w, vmc.STAT["LAST_E"]] - e) <= 1e-9 * max(abs(e), 1e-3 * s.N), (sweep, w)
+
+
+@pytest.mark.parametrize("prec,W,groups", [(1, 2048, (1, 4, 3)), (0, 1030, (1, 2, 5)), (1, 300, (1, 8))])
+def test_walker_groups_give_the_same_chains(vmc, prec, W, groups):
+    """Measured register sweeps split over walker groups on their own streams
+    (ragged splits included) leave every walker exactly where one group
+    leaves it: positions, inverses and statistics bit for bit."""
+    out = []
+    for g in groups:
+        s = vmc.System.heg(54, 2.0, 1.0, precision=prec, reinvert_interval=64 if prec else 0)
+        e = vmc.Ensemble(s, W, 77, step0=0.6)
+        e.set_walker_groups(g)
+        e.init_positions()
+        e.sweep(2, adapt=True)
+        e.sweep(4, measure=True, block_size=2)
+        e.sweep(1, measure=True, block_size=2)
+        out.append((e.positions(), e.inverse(), e.stats()))
+        print(f"prec={prec} W={W} groups={g} done", flush=True)
+        e.close()
+    for g, (p, i, st) in zip(groups[1:], out[1:]):
+        np.testing.assert_array_equal(p, out[0][0], err_msg=f"groups={g}")
+        np.testing.assert_array_equal(i, out[0][1], err_msg=f"groups={g}")
+        np.testing.assert_array_equal(st, out[0][2], err_msg=f"groups={g}")

@@ -78,6 +78,11 @@ struct DevState {
     // point, which leaves the walkers' state alone), 0 when the re-inversion
     // also replaces the stored inverse and its bookkeeping (a measured sweep).
     int inv64_only = 0;
+    // Ensemble index of this state's walker 0: the register sweep keys its
+    // Philox draws by w0 + w, so a view of a walker range (a walker group
+    // swept on a stream of its own) draws exactly what the whole ensemble
+    // would.
+    int w0 = 0;
 };
 
 // Caller-supplied moves of one sweep (vmc_ensemble_sweep_given): electron e

@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -778,6 +779,8 @@ size_t lds_for(const DevSys &d, int real_bytes, bool inv_lds) {
 
 }  // namespace
 
+constexpr int kMaxWalkerGroups = 8;
+
 struct vmc_ensemble {
     SystemTables tables;
     DevState st{};
@@ -796,8 +799,19 @@ struct vmc_ensemble {
     // multi-wavefront or generic kernels re-inverts into for its FP64 local
     // energy (DevState::inv64 of that sweep's re-inversion).
     double *inv64 = nullptr;
+    // Walker groups of the register kernels' measured sweeps: the walkers are
+    // split into `groups` contiguous ranges, each swept and measured on a
+    // stream of its own (forked from and joined back into the caller's
+    // stream), so one group's sweep overlaps another's local energy.
+    // Events are made on first use.
+    int groups = 1;
+    hipEvent_t gjoin[kMaxWalkerGroups] = {};
+    hipEvent_t gfork = nullptr;
 
     ~vmc_ensemble() {
+        for (int g = 0; g < kMaxWalkerGroups; ++g)
+            if (gjoin[g]) (void)hipEventDestroy(gjoin[g]);
+        if (gfork) (void)hipEventDestroy(gfork);
         if (inv64) (void)hipFree(inv64);
         if (st.pos) (void)hipFree(st.pos);
         if (st.inv) (void)hipFree(st.inv);
@@ -886,6 +900,71 @@ int launch_sweep_du(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hi
     return VMC_OK;
 }
 
+// The state of walkers [b, b + n) of an ensemble: every per-walker array
+// offset to walker b, w0 = b for the Philox counters.  The due-walker count
+// stays with the group holding walker 0 (the measurement clears it there).
+DevState walker_view(const vmc_ensemble *e, int64_t b, int64_t n) {
+    const DevSys &d = e->tables.dev;
+    const DevState &a = e->st;
+    const size_t u = static_cast<size_t>(b);
+    DevState v = a;
+    v.W = static_cast<int>(n);
+    v.w0 = a.w0 + static_cast<int>(b);
+    v.pos = a.pos + u * 3 * d.N;
+    v.inv = static_cast<unsigned char *>(a.inv) + u * d.ninv * e->tables.real_bytes;
+    v.sigma = a.sigma + u;
+    v.stats = a.stats + u * kStatCount;
+    v.logdet = a.logdet + 2 * u;
+    v.since = a.since + u;
+    v.flags = a.flags + u;
+    v.scratch = a.scratch ? a.scratch + u * d.nmax * d.nmax : nullptr;
+    v.drift = a.drift + u;
+    if (a.resid) v.resid = a.resid + u * static_cast<size_t>(ns_resid_floats(d.nmax, e->tables.real_bytes));
+    v.jrow = a.jrow + u * d.N;
+    v.due = b == 0 ? a.due : nullptr;
+    v.due_list = a.due_list + u;
+    return v;
+}
+
+// The walker-group streams of the current device, kept for the life of the
+// process and shared by every ensemble on that device (ensembles used from
+// several host threads at once serialise on them, in stream order).  Each
+// is made with a CU mask of every CU of the device, which gives it a
+// hardware queue of its own: plain streams share the process's few hardware
+// queues, and two groups on one queue run one after the other (measured:
+// the groups' overlap came and went with the order streams were made in).
+int group_streams(int G, hipStream_t out[kMaxWalkerGroups]) {
+    static std::mutex mu;
+    static std::vector<std::array<hipStream_t, kMaxWalkerGroups>> per_device;
+    int dev = 0;
+    VMC_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    if (per_device.size() <= static_cast<size_t>(dev)) per_device.resize(static_cast<size_t>(dev) + 1);
+    auto &ss = per_device[static_cast<size_t>(dev)];
+    for (int g = 0; g < G; ++g) {
+        hipStream_t &st = ss[static_cast<size_t>(g)];
+        if (!st) {
+            int cus = 0;
+            VMC_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            std::vector<uint32_t> mask(static_cast<size_t>((cus + 31) / 32), 0u);
+            for (int c = 0; c < cus; ++c) mask[static_cast<size_t>(c / 32)] |= 1u << (c % 32);
+            VMC_HIP_TRY(hipExtStreamCreateWithCUMask(&st, static_cast<uint32_t>(mask.size()), mask.data()));
+        }
+        out[g] = st;
+    }
+    return VMC_OK;
+}
+
+// Walker groups for a register-kernel launch: the caller's count (one by
+// default), and one for sweeps that need a listed re-inversion (FP64 with
+// K > 0) or are timed per kernel.  Only measured sweeps are split.
+int walker_groups(const vmc_ensemble *e, int flags) {
+    const DevSys &d = e->tables.dev;
+    if (!(flags & VMC_SWEEP_MEASURE) || e->timer.on) return 1;
+    if (e->tables.real_bytes == 8 && d.reinv_k > 0) return 1;
+    return static_cast<int>(std::min<int64_t>(e->groups < 1 ? 1 : e->groups, e->W));
+}
+
 template <typename Real>
 int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipStream_t stream) {
     const DevSys &d = e->tables.dev;
@@ -893,12 +972,39 @@ int launch_sweep(vmc_ensemble *e, int n_sweeps, int flags, int block_size, hipSt
     if constexpr (sizeof(Real) == 8)
         if (fam == kFamDelayed) return launch_sweep_du(e, n_sweeps, flags, block_size, stream);
     if (fam == kFamRegister) {
-        const unsigned grid = static_cast<unsigned>(e->W);
         const unsigned long long s0 = (unsigned long long)e->sweeps_done;
-        const bool ok = sizeof(Real) == 4
-                            ? rr_launch_f32(d, e->st, grid, n_sweeps, s0, flags, block_size, e->rr_variant, stream, e->timer)
-                            : rr_launch_f64(d, e->st, grid, n_sweeps, s0, flags, block_size, e->rr_variant, stream, e->timer);
-        if (!ok) return fail(VMC_ERR_INVALID, "no register-resident instantiation for this system");
+        const auto run = [&](const DevState &st, unsigned grid, hipStream_t s) {
+            return sizeof(Real) == 4
+                       ? rr_launch_f32(d, st, grid, n_sweeps, s0, flags, block_size, e->rr_variant, s, e->timer)
+                       : rr_launch_f64(d, st, grid, n_sweeps, s0, flags, block_size, e->rr_variant, s, e->timer);
+        };
+        const int G = walker_groups(e, flags);
+        if (G <= 1) {
+            if (!run(e->st, static_cast<unsigned>(e->W), stream))
+                return fail(VMC_ERR_INVALID, "no register-resident instantiation for this system");
+            VMC_HIP_TRY(hipGetLastError());
+            return VMC_OK;
+        }
+        // Every group on a library stream of its own, made together so that
+        // they sit on distinct hardware queues (the caller's stream, wherever
+        // it sits, only forks and joins).  Fork: the group streams wait for
+        // the caller's stream; join: the caller's stream waits for them.
+        hipStream_t gs[kMaxWalkerGroups];
+        if (const int rc = group_streams(G, gs)) return rc;
+        if (!e->gfork) VMC_HIP_TRY(hipEventCreateWithFlags(&e->gfork, hipEventDisableTiming));
+        for (int g = 0; g < G; ++g)
+            if (!e->gjoin[g]) VMC_HIP_TRY(hipEventCreateWithFlags(&e->gjoin[g], hipEventDisableTiming));
+        VMC_HIP_TRY(hipEventRecord(e->gfork, stream));
+        for (int g = 0; g < G; ++g) VMC_HIP_TRY(hipStreamWaitEvent(gs[g], e->gfork, 0));
+        for (int g = 0; g < G; ++g) {
+            const int64_t b = e->W * g / G, n = e->W * (g + 1) / G - b;
+            if (!run(walker_view(e, b, n), static_cast<unsigned>(n), gs[g]))
+                return fail(VMC_ERR_INVALID, "no register-resident instantiation for this system");
+        }
+        for (int g = 0; g < G; ++g) {
+            VMC_HIP_TRY(hipEventRecord(e->gjoin[g], gs[g]));
+            VMC_HIP_TRY(hipStreamWaitEvent(stream, e->gjoin[g], 0));
+        }
         VMC_HIP_TRY(hipGetLastError());
         return VMC_OK;
     }
@@ -1393,6 +1499,14 @@ int vmc_ensemble_set_kernel(vmc_ensemble *e, int32_t kernel) {
     if (kernel == VMC_KERNEL_DELAYED && (e->tables.real_bytes != 8 || !du_supported(e->tables.dev)))
         return fail(VMC_ERR_INVALID, "delayed-update kernel needs an FP64 system of at most 160 electrons per spin");
     e->kernel = kernel;
+    return VMC_OK;
+}
+
+int vmc_ensemble_set_walker_groups(vmc_ensemble *e, int32_t groups) {
+    if (!e) return fail(VMC_ERR_INVALID, "ensemble is NULL");
+    if (groups < 1 || groups > kMaxWalkerGroups)
+        return fail(VMC_ERR_INVALID, "walker groups must be in [1, %d]", kMaxWalkerGroups);
+    e->groups = groups;
     return VMC_OK;
 }
 

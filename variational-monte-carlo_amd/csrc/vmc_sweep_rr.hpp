@@ -1218,7 +1218,8 @@ __device__ __forceinline__ void sweep_rr_body(const DevSys &s, const DevState &s
         }
         rec = given.trace + (size_t)w * N * kTraceCount;
     } else {
-        if (has_e) rr_proposal<Real>(s, sh.misc[0], L.x, L.y, L.z, e, w, sweep, lane, &sh.prop[0][0], &sh.pph[0][0]);
+        if (has_e) rr_proposal<Real>(s, sh.misc[0], L.x, L.y, L.z, e, st.w0 + w, sweep, lane, &sh.prop[0][0],
+                                      &sh.pph[0][0]);
     }
     __syncthreads();
     // Sweep order (sweep_visit): with two determinants (ND == NU) the spins

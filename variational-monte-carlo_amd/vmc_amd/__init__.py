@@ -195,6 +195,7 @@ def lib():
             "vmc_ensemble_set_kernel": (C.c_int, [_VP, C.c_int32]),
             "vmc_ensemble_get_kernel": (C.c_int, [_VP, _ip]),
             "vmc_ensemble_set_rr_variant": (C.c_int, [_VP, C.c_int32]),
+            "vmc_ensemble_set_walker_groups": (C.c_int, [_VP, C.c_int32]),
             "vmc_ensemble_read_stats": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_read_steps": (C.c_int, [_VP, _dp, _VP]),
             "vmc_ensemble_read_drift": (C.c_int, [_VP, _dp, _VP]),
@@ -376,6 +377,11 @@ class Ensemble:
 
     def set_rr_variant(self, variant):
         _check(lib().vmc_ensemble_set_rr_variant(self._h, variant))
+
+    def set_walker_groups(self, groups):
+        """Walker groups (1 to 8) of the register kernels' measured sweeps,
+        each on a stream of its own; the chains do not depend on it."""
+        _check(lib().vmc_ensemble_set_walker_groups(self._h, int(groups)))
 
     def reset_stats(self):
         _check(lib().vmc_ensemble_reset_stats(self._h, self.stream))
