@@ -17,6 +17,11 @@
 //   the rounding floor of the FP32 inverse.  A drift of 0.5 or more, a non-finite
 //   one, or no convergence within kNSSteps falls back to Gauss-Jordan from
 //   scratch, which also refreshes log|det D|.
+// With an FP64 output (a mixed measured sweep's FP64 local energy) and a
+// first-pass drift d with n d^3 <= 1e-8, one pass gives it instead:
+// Y (I + R + R^2), residual R^3, with R^2 and the product by Y on the FP32
+// matrix cores (both are corrections of order d; their FP32 rounding stays
+// far below the residual).
 // The identity padding keeps the padded block exact: Z, R and Y stay the
 // identity there and zero across, so the n x n block evolves as if unpadded.
 // The reference never re-inverts: it keeps its FP64 inverse by
@@ -42,6 +47,20 @@ constexpr double kNSDone = 5e-4;
 // n d^2 <= kRefineTarget (the register kernels' in-kernel refinement rule):
 // its FP64 result Y + Y R has a residual of at most n d^2.
 __device__ __forceinline__ bool ns_exact_from(int n, double dm) { return (double)n * dm * dm <= kRefineTarget; }
+// One pass Y (I + R + R^2) has the residual R^3, at most n d^3: the FP64
+// output in one pass when that is within the same target.
+__device__ __forceinline__ bool ns_series_from(int n, double dm) {
+    return (double)n * dm * dm * dm <= kRefineTarget;
+}
+
+typedef float ns_v4f __attribute__((ext_vector_type(4)));
+
+// Output layouts of the 16 x 16 x 4 matrix instructions (lane l, register q):
+// v_mfma_f64_16x16x4f64 holds C[4q + (l >> 4)][l & 15], v_mfma_f32_16x16x4f32
+// holds C[4 (l >> 4) + q][l & 15]; both take A[l & 15][l >> 4] and
+// B[l >> 4][l & 15].
+__device__ __forceinline__ int ns_row64(int q, int lane) { return 4 * q + (lane >> 4); }
+__device__ __forceinline__ int ns_row32(int q, int lane) { return 4 * (lane >> 4) + q; }
 
 // R = I - Z takes E's storage once Z = E Y is in registers (E is rebuilt
 // for a further step, rarely needed), so a workgroup holds E and Y only:
@@ -113,6 +132,29 @@ __device__ __forceinline__ void ns_mm(const TA *A, const TB *B, ns_v4d acc[NSTil
     }
 }
 
+// acc[i] = (A B) on tile i of this wave, FP32 operands and accumulation on
+// v_mfma_f32_16x16x4f32 (A, B row-major LDS matrices, pitch PF).
+template <int NT, int PF>
+__device__ __forceinline__ void ns_mm32(const float *A, const float *B, ns_v4f acc[NSTiles<NT>::kCount], int wv,
+                                        int lane) {
+    using T = NSTiles<NT>;
+    const int lo = lane & 15, hi = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < T::kCount; ++i) acc[i] = ns_v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+    for (int kk = 0; kk < 4 * NT; ++kk) {
+        const int k = 4 * kk + hi;
+#pragma unroll
+        for (int i = 0; i < T::kCount; ++i) {
+            if (T::valid(i, wv)) {
+                const float a = A[(16 * T::row(i, wv) + lo) * PF + k];
+                const float b = B[k * PF + 16 * T::col(i, wv) + lo];
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i], 0, 0, 0);
+            }
+        }
+    }
+}
+
 // E = D^T of determinant d in FP64 (row pitch `pitch`): four threads per
 // electron i, each writing every fourth orbital of column i from the axis
 // phasors of the electron's FP64 position.  With `pad`, rows and columns n..P
@@ -170,7 +212,7 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
         }
         __syncthreads();
         double d0 = 0.0;
-        bool ok = false;
+        bool ok = false, series = false;
 #pragma unroll 1
         for (int it = 0; it < kNSSteps; ++it) {
             ns_v4d z[T::kCount];
@@ -203,6 +245,10 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
                 ok = true;
                 break;
             }
+            if (Y64 && ns_series_from(n, dm)) {
+                series = true;
+                break;
+            }
             ns_v4d y[T::kCount];
             ns_mm<NT, float, PF, float, PF>(sh.Y, R, y, wv, lane);
             float yn[T::kCount][4];
@@ -233,6 +279,44 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
             }
             ns_build<NT>(s, gp, d, sh.E, PE, true);  // R held E's place
             __syncthreads();
+        }
+        if (series) {
+            // R <- R + R R, then Y (I + R + R^2) = Y + Y R', in FP32 on the
+            // matrix cores, each lane updating its own output entries.
+            const int ll = lane & 15;
+            ns_v4f acc[T::kCount];
+            ns_mm32<NT, PF>(R, R, acc, wv, lane);
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < T::kCount; ++i) {
+                if (!T::valid(i, wv)) continue;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    R[(16 * T::row(i, wv) + ns_row32(q, lane)) * PF + 16 * T::col(i, wv) + ll] += acc[i][q];
+            }
+            __syncthreads();
+            ns_mm32<NT, PF>(sh.Y, R, acc, wv, lane);
+#pragma unroll
+            for (int i = 0; i < T::kCount; ++i) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = 16 * T::row(i, wv) + ns_row32(q, lane);
+                    const int c = 16 * T::col(i, wv) + ll;
+                    const double v = T::valid(i, wv) ? (double)sh.Y[r * PF + c] + (double)acc[i][q] : 0.0;
+                    acc[i][q] = (float)v;
+                    if (T::valid(i, wv) && r < n && c < n) Y64[r * n + c] = v;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < T::kCount; ++i) {
+                if (!T::valid(i, wv)) continue;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    sh.Y[(16 * T::row(i, wv) + ns_row32(q, lane)) * PF + 16 * T::col(i, wv) + ll] = acc[i][q];
+            }
+            __syncthreads();
+            ok = true;
         }
         drift = (d0 > drift || d0 != d0) ? d0 : drift;
         if (ok) {
@@ -322,6 +406,42 @@ __device__ __forceinline__ void nsg_mm(LA load_a, LB load_b, ns_v4d acc[(NT / 2)
     }
 }
 
+// nsg_mm with FP32 operands and accumulation on v_mfma_f32_16x16x4f32.
+template <int NT, typename LA, typename LB>
+__device__ __forceinline__ void nsg_mm32(LA load_a, LB load_b, ns_v4f acc[(NT / 2) * (NT / 2)], int wv,
+                                         int lane) {
+    constexpr int H = NT / 2;
+    const int lo = lane & 15, hi = lane >> 4;
+    const int r0 = (wv >> 1) * H, c0 = (wv & 1) * H;
+#pragma unroll
+    for (int i = 0; i < H * H; ++i) acc[i] = ns_v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    float a[H], b[H], an[H], bn[H];
+#pragma unroll
+    for (int u = 0; u < H; ++u) {
+        a[u] = load_a(16 * (r0 + u) + lo, hi);
+        b[u] = load_b(hi, 16 * (c0 + u) + lo);
+    }
+#pragma unroll 1
+    for (int kk = 0; kk < 4 * NT; ++kk) {
+        const int kn = 4 * (kk + 1 < 4 * NT ? kk + 1 : kk) + hi;
+#pragma unroll
+        for (int u = 0; u < H; ++u) {
+            an[u] = load_a(16 * (r0 + u) + lo, kn);
+            bn[u] = load_b(kn, 16 * (c0 + u) + lo);
+        }
+#pragma unroll
+        for (int u = 0; u < H; ++u)
+#pragma unroll
+            for (int v = 0; v < H; ++v)
+                acc[u * H + v] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[v], acc[u * H + v], 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < H; ++u) {
+            a[u] = an[u];
+            b[u] = bn[u];
+        }
+    }
+}
+
 template <int NT>
 __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &st, int w) {
     using S = NSGShared<NT>;
@@ -357,7 +477,7 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
         const auto load_y = [&](int r, int k) { return (double)sh.Y[r * PF + k]; };
         const auto load_r = [&](int r, int k) { return (double)R[r * P + k]; };
         double d0 = 0.0;
-        bool ok = false;
+        bool ok = false, series = false;
 #pragma unroll 1
         for (int it = 0; it < kNSSteps; ++it) {
             ns_v4d z[H * H];
@@ -388,6 +508,10 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
                 ok = true;
                 break;
             }
+            if (Y64 && ns_series_from(n, dm)) {
+                series = true;
+                break;
+            }
             ns_v4d y[H * H];
             nsg_mm<NT>(load_y, load_r, y, wv, lane);
             float yn[H * H][4];
@@ -413,6 +537,41 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
                 ok = true;
                 break;
             }
+        }
+        if (series) {
+            // R <- R + R R, then Y (I + R + R^2) = Y + Y R', in FP32 on the
+            // matrix cores, each lane updating its own output entries.
+            const auto load_y32 = [&](int r, int k) { return sh.Y[r * PF + k]; };
+            const auto load_r32 = [&](int r, int k) { return R[r * P + k]; };
+            int rl = 16 * r0, cl = 16 * c0 + (lane & 15);
+            asm volatile("" : "+v"(rl), "+v"(cl));
+            ns_v4f acc[H * H];
+            nsg_mm32<NT>(load_r32, load_r32, acc, wv, lane);
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < H * H; ++i)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    R[(rl + 16 * (i / H) + ns_row32(q, lane)) * P + cl + 16 * (i % H)] += acc[i][q];
+            __syncthreads();
+            nsg_mm32<NT>(load_y32, load_r32, acc, wv, lane);
+#pragma unroll
+            for (int i = 0; i < H * H; ++i)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = rl + 16 * (i / H) + ns_row32(q, lane), c = cl + 16 * (i % H);
+                    const double v = (double)sh.Y[r * PF + c] + (double)acc[i][q];
+                    acc[i][q] = (float)v;
+                    if (r < n && c < n) Y64[r * n + c] = v;
+                }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < H * H; ++i)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    sh.Y[(rl + 16 * (i / H) + ns_row32(q, lane)) * PF + cl + 16 * (i % H)] = acc[i][q];
+            __syncthreads();
+            ok = true;
         }
         drift = (d0 > drift || d0 != d0) ? d0 : drift;
         if (ok) {
@@ -452,13 +611,15 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
     }
 }
 
+// Two workgroups per CU up to 96 per spin (37 KB of LDS each at NT = 6).
 template <int NT>
-__global__ __launch_bounds__(kNSThreads) void k_reinv_nsg(DevSys s, DevState st) {
+__global__ __launch_bounds__(kNSThreads, NT <= 6 ? 2 : 1) void k_reinv_nsg(DevSys s, DevState st) {
     reinv_nsg_body<NT>(s, st, blockIdx.x);
 }
 
 template <int NT>
-__global__ __launch_bounds__(kNSThreads) void k_reinv_nsg_b(const BatchItem *__restrict__ items, int n_items) {
+__global__ __launch_bounds__(kNSThreads, NT <= 6 ? 2 : 1) void k_reinv_nsg_b(const BatchItem *__restrict__ items,
+                                                                           int n_items) {
     const BatchItem &it = items[batch_find(items, n_items, blockIdx.x)];
     reinv_nsg_body<NT>(it.s, it.st, blockIdx.x - it.walker_begin);
 }
