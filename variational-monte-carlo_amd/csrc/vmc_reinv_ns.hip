@@ -249,18 +249,19 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
                 series = true;
                 break;
             }
-            ns_v4d y[T::kCount];
-            ns_mm<NT, float, PF, float, PF>(sh.Y, R, y, wv, lane);
-            float yn[T::kCount][4];
+            // Y R on the FP32 matrix cores: a correction of order d, whose FP32
+            // rounding (about 1e-7 of d) stays below what the step leaves.
+            ns_v4f y[T::kCount];
+            ns_mm32<NT, PF>(sh.Y, R, y, wv, lane);
             const bool exact = Y64 && ns_exact_from(n, dm);  // this step's FP64 result is the output
 #pragma unroll
             for (int i = 0; i < T::kCount; ++i) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int r = 16 * T::row(i, wv) + 4 * q + hl;
+                    const int r = 16 * T::row(i, wv) + ns_row32(q, lane);
                     const int c = 16 * T::col(i, wv) + ll;
-                    const double v = (double)sh.Y[r * PF + c] + y[i][q];
-                    yn[i][q] = T::valid(i, wv) ? (float)v : 0.0f;
+                    const double v = T::valid(i, wv) ? (double)sh.Y[r * PF + c] + (double)y[i][q] : 0.0;
+                    y[i][q] = (float)v;
                     if (exact && T::valid(i, wv) && r < n && c < n) Y64[r * n + c] = v;
                 }
             }
@@ -270,7 +271,7 @@ __device__ __forceinline__ void reinv_ns_body(const DevSys &s, const DevState &s
                 if (!T::valid(i, wv)) continue;
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    sh.Y[(16 * T::row(i, wv) + 4 * q + hl) * PF + 16 * T::col(i, wv) + ll] = yn[i][q];
+                    sh.Y[(16 * T::row(i, wv) + ns_row32(q, lane)) * PF + 16 * T::col(i, wv) + ll] = y[i][q];
             }
             if (Y64 ? exact : dm < kNSDone) {
                 ok = true;
@@ -512,26 +513,30 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
                 series = true;
                 break;
             }
-            ns_v4d y[H * H];
-            nsg_mm<NT>(load_y, load_r, y, wv, lane);
-            float yn[H * H][4];
-            int yl = rl * PF + cl;
-            asm volatile("" : "+v"(yl));
+            // Y R on the FP32 matrix cores: a correction of order d, whose FP32
+            // rounding (about 1e-7 of d) stays below what the step leaves.
+            const auto load_y32 = [&](int r, int k) { return sh.Y[r * PF + k]; };
+            const auto load_r32 = [&](int r, int k) { return R[r * P + k]; };
+            ns_v4f y[H * H];
+            nsg_mm32<NT>(load_y32, load_r32, y, wv, lane);
+            // first row of this lane's FP32-layout entries, and its column
+            int rl32 = 16 * r0 + 4 * (lane >> 4), cl32 = cl;
+            asm volatile("" : "+v"(rl32), "+v"(cl32));
             const bool exact = Y64 && ns_exact_from(n, dm);  // this step's FP64 result is the output
 #pragma unroll
             for (int i = 0; i < H * H; ++i)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const double v = (double)sh.Y[yl + (16 * (i / H) + 4 * q) * PF + 16 * (i % H)] + y[i][q];
-                    yn[i][q] = (float)v;
-                    const int r = rl + 16 * (i / H) + 4 * q, c = cl + 16 * (i % H);
+                    const int r = rl32 + 16 * (i / H) + q, c = cl32 + 16 * (i % H);
+                    const double v = (double)sh.Y[r * PF + c] + (double)y[i][q];
+                    y[i][q] = (float)v;
                     if (exact && r < n && c < n) Y64[r * n + c] = v;
                 }
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < H * H; ++i)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) sh.Y[yl + (16 * (i / H) + 4 * q) * PF + 16 * (i % H)] = yn[i][q];
+                for (int q = 0; q < 4; ++q) sh.Y[(rl32 + 16 * (i / H) + q) * PF + cl32 + 16 * (i % H)] = y[i][q];
             __syncthreads();
             if (Y64 ? exact : dm < kNSDone) {
                 ok = true;
@@ -633,15 +638,23 @@ void launch_nsg(const DevSys &d, const DevState &st, const BatchItem *items, int
         hipLaunchKernelGGL((k_reinv_nsg<NT>), dim3(grid), dim3(kNSThreads), 0, stream, d, st);
 }
 
+// Workgroups per CU the registers must allow (the LDS allows as many):
+// four up to 32 per spin, three up to 64, two up to 80.
+template <int NT>
+constexpr int ns_blocks() {
+    return NT <= 2 ? 4 : NT <= 4 ? 3 : NT == 5 ? 2 : 1;
+}
+
 // Workgroup g re-inverts determinant g % ndet of walker g / ndet.
 template <int NT>
-__global__ __launch_bounds__(kNSThreads) void k_reinv_ns(DevSys s, DevState st) {
+__global__ __launch_bounds__(kNSThreads, ns_blocks<NT>()) void k_reinv_ns(DevSys s, DevState st) {
     reinv_ns_body<NT>(s, st, blockIdx.x / s.ndet, blockIdx.x % s.ndet);
 }
 
 // Batch form over a group of systems of one size (and determinant count).
 template <int NT>
-__global__ __launch_bounds__(kNSThreads) void k_reinv_ns_b(const BatchItem *__restrict__ items, int n_items) {
+__global__ __launch_bounds__(kNSThreads, ns_blocks<NT>()) void k_reinv_ns_b(const BatchItem *__restrict__ items,
+                                                                          int n_items) {
     const int ndet = items[0].s.ndet;
     const int g = blockIdx.x / ndet;
     const BatchItem &it = items[batch_find(items, n_items, g)];
