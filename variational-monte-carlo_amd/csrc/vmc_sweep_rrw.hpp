@@ -757,9 +757,17 @@ __device__ __forceinline__ void measure_rrw_body(const DevSys &s, const DevState
                       offsetof(Sh, tab) == offsetof(Sh, pph) + sizeof(sh.pph) &&
                       sizeof(sh.prop) + sizeof(sh.pph) + sizeof(sh.tab) >= sizeof(Real) * sf::table_reals<Real>(),
                   "structure-factor tables exceed the run prop | pph | tab");
+    // Column segments (a third of the LDS reads per G) keep only as many
+    // threads busy as there are segments (about 80 for |n|^2 <= 25): from
+    // 256 threads one G vector per thread (at most 768 G) keeps more of
+    // them busy, which measured faster there.
     const auto pos_of = [](int ax, int e) { return sh.pos[ax][e]; };
-    const double acc = sf::energy_cols<Real, N, NT>(s, reinterpret_cast<sf::Pair<Real> *>(&sh.prop[0][0]),
-                                                    (int)threadIdx.x, pos_of);
+    auto *sf_tab = reinterpret_cast<sf::Pair<Real> *>(&sh.prop[0][0]);
+    double acc;
+    if constexpr (NT >= 256)
+        acc = sf::energy<Real, N, (12 * kWave + NT - 1) / NT, NT>(s, sf_tab, (int)threadIdx.x, pos_of);
+    else
+        acc = sf::energy_cols<Real, N, NT>(s, sf_tab, (int)threadIdx.x, pos_of);
     const double vrecip = __dmul_rn(s.pref, K::block_total(acc, sh.red));  // (uncontracted, as in every kernel)
     if (parts) {
         const double J = 0.5 * K::block_total(K::jastrow_row(c, sh, T, x, y, z), sh.red);
