@@ -374,9 +374,10 @@ __device__ double energy_g(const DevSys &s, Pair<Real> *tab, int tid, PosOf pos_
 // segments t, t + NT, ...): per staged electron two LDS reads give
 // X^mx Y^my, a third Z^a0, and Z^1 is a broadcast; the steps a0, a0 + 1, ...
 // multiply Z^a on, each adding X^mx Y^my Z^a to S(mx, my, a) and its
-// conjugate-in-z partner X^mx Y^my Z^-a to S(mx, my, -a) with four fused
-// accumulations each.  Against one G vector per slot this reads a third of
-// the LDS bytes per electron.  The thread's share is reduced by the caller.
+// conjugate-in-z partner X^mx Y^my Z^-a to S(mx, my, -a): the two share their
+// four real products, so four fused accumulations per step serve both.
+// Against one G vector per slot this reads a third of the LDS bytes per
+// electron.  The thread's share is reduced by the caller.
 template <typename Real, int N, int NT, typename PosOf>
 __device__ double energy_cols(const DevSys &s, Pair<Real> *tab, int tid, PosOf pos_of) {
     constexpr int P = kPitch;
@@ -391,9 +392,13 @@ __device__ double energy_cols(const DevSys &s, Pair<Real> *tab, int tid, PosOf p
         const bool act = sg < s.nseg;
         const int4 G = act ? s.sfseg[sg] : make_int4(0, 0, 0, 0);
         const int ox = G.x * P, oy = (kY0 + G.y) * P, oz = (kZ0 + G.z) * P;
-        Real pr[S], pi[S], mr[S], mi[S];
+        // Per step the four product sums of X^mx Y^my = (xr, xi) with
+        // Z^a = (zr, zi): sum xr zr, sum xi zi, sum xr zi, sum xi zr.
+        // S(mx, my, +a) = (rr - ii, ri + ir) and S(mx, my, -a) =
+        // (rr + ii, ir - ri) follow once at the end.
+        Real rr[S], ii[S], ri[S], ir[S];
 #pragma unroll
-        for (int k = 0; k < S; ++k) pr[k] = pi[k] = mr[k] = mi[k] = (Real)0;
+        for (int k = 0; k < S; ++k) rr[k] = ii[k] = ri[k] = ir[k] = (Real)0;
 #pragma unroll 1
         for (int c = 0; c < kChunks; ++c) {
             __syncthreads();
@@ -409,14 +414,10 @@ __device__ double energy_cols(const DevSys &s, Pair<Real> *tab, int tid, PosOf p
                 const Real xr = X.re * Y.re - X.im * Y.im, xi = X.re * Y.im + X.im * Y.re;
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
-                    pr[k] = fma(xr, Zp.re, pr[k]);
-                    pr[k] = fma(-xi, Zp.im, pr[k]);
-                    pi[k] = fma(xr, Zp.im, pi[k]);
-                    pi[k] = fma(xi, Zp.re, pi[k]);
-                    mr[k] = fma(xr, Zp.re, mr[k]);
-                    mr[k] = fma(xi, Zp.im, mr[k]);
-                    mi[k] = fma(xi, Zp.re, mi[k]);
-                    mi[k] = fma(-xr, Zp.im, mi[k]);
+                    rr[k] = fma(xr, Zp.re, rr[k]);
+                    ii[k] = fma(xi, Zp.im, ii[k]);
+                    ri[k] = fma(xr, Zp.im, ri[k]);
+                    ir[k] = fma(xi, Zp.re, ir[k]);
                     if (k + 1 < S) {
                         const Real t = Zp.re * Z.re - Zp.im * Z.im;
                         Zp.im = Zp.re * Z.im + Zp.im * Z.re;
@@ -429,9 +430,11 @@ __device__ double energy_cols(const DevSys &s, Pair<Real> *tab, int tid, PosOf p
         if (act) {
             const double *w = s.sfw + (size_t)sg * 2 * S;
 #pragma unroll
-            for (int k = 0; k < S; ++k)
-                acc += w[2 * k] * ((double)pr[k] * (double)pr[k] + (double)pi[k] * (double)pi[k]) +
-                       w[2 * k + 1] * ((double)mr[k] * (double)mr[k] + (double)mi[k] * (double)mi[k]);
+            for (int k = 0; k < S; ++k) {
+                const double pr = (double)rr[k] - (double)ii[k], pi = (double)ri[k] + (double)ir[k];
+                const double mr = (double)rr[k] + (double)ii[k], mi = (double)ir[k] - (double)ri[k];
+                acc += w[2 * k] * (pr * pr + pi * pi) + w[2 * k + 1] * (mr * mr + mi * mi);
+            }
         }
     }
     __syncthreads();
