@@ -372,18 +372,16 @@ __device__ double energy_g(const DevSys &s, Pair<Real> *tab, int tid, PosOf pos_
 
 // sum_G w_G |S_G|^2 by column segments (DevSys::sfseg; thread t takes
 // segments t, t + NT, ...): per staged electron two LDS reads give
-// X^mx Y^my, a third Z^a0, and Z^1 is a broadcast; the steps a0, a0 + 1, ...
-// multiply Z^a on, each adding X^mx Y^my Z^a to S(mx, my, a) and its
-// conjugate-in-z partner X^mx Y^my Z^-a to S(mx, my, -a): the two share their
-// four real products, so four fused accumulations per step serve both.
-// Against one G vector per slot this reads a third of the LDS bytes per
-// electron.  The thread's share is reduced by the caller.
+// X^mx Y^my, and each step a = a0, a0 + 1, ... reads Z^a, adding
+// X^mx Y^my Z^a to S(mx, my, a) and its conjugate-in-z partner
+// X^mx Y^my Z^-a to S(mx, my, -a): the two share their four real products,
+// so four fused accumulations per step serve both.  The thread's share is
+// reduced by the caller.
 template <typename Real, int N, int NT, typename PosOf>
 __device__ double energy_cols(const DevSys &s, Pair<Real> *tab, int tid, PosOf pos_of) {
     constexpr int P = kPitch;
     constexpr int S = kSfSegSteps;
     constexpr int kChunks = (N + kRRChunk - 1) / kRRChunk;
-    constexpr int oz1 = (kZ0 + 1) * P;
     double acc = 0.0;
 #pragma unroll 1
     for (int p0 = 0; p0 < s.nseg; p0 += NT) {
@@ -391,7 +389,12 @@ __device__ double energy_cols(const DevSys &s, Pair<Real> *tab, int tid, PosOf p
         asm volatile("" : "+v"(sg));  // the segment reads stay in this pass
         const bool act = sg < s.nseg;
         const int4 G = act ? s.sfseg[sg] : make_int4(0, 0, 0, 0);
-        const int ox = G.x * P, oy = (kY0 + G.y) * P, oz = (kZ0 + G.z) * P;
+        const int ox = G.x * P, oy = (kY0 + G.y) * P;
+        // Entry of Z^(a0 + k) per step; steps past the segment's end (zero
+        // weight) read the highest staged power instead.
+        int oz[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) oz[k] = (kZ0 + (G.z + k < kRRMaxPow ? G.z + k : kRRMaxPow)) * P;
         // Per step the four product sums of X^mx Y^my = (xr, xi) with
         // Z^a = (zr, zi): sum xr zr, sum xi zi, sum xr zi, sum xi zr.
         // S(mx, my, +a) = (rr - ii, ri + ir) and S(mx, my, -a) =
@@ -409,20 +412,15 @@ __device__ double energy_cols(const DevSys &s, Pair<Real> *tab, int tid, PosOf p
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < kRRChunk; ++i) {
-                const Pair<Real> X = tab[ox + i], Y = tab[oy + i], Z = tab[oz1 + i];
-                Pair<Real> Zp = tab[oz + i];
+                const Pair<Real> X = tab[ox + i], Y = tab[oy + i];
                 const Real xr = X.re * Y.re - X.im * Y.im, xi = X.re * Y.im + X.im * Y.re;
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
-                    rr[k] = fma(xr, Zp.re, rr[k]);
-                    ii[k] = fma(xi, Zp.im, ii[k]);
-                    ri[k] = fma(xr, Zp.im, ri[k]);
-                    ir[k] = fma(xi, Zp.re, ir[k]);
-                    if (k + 1 < S) {
-                        const Real t = Zp.re * Z.re - Zp.im * Z.im;
-                        Zp.im = Zp.re * Z.im + Zp.im * Z.re;
-                        Zp.re = t;
-                    }
+                    const Pair<Real> Z = tab[oz[k] + i];
+                    rr[k] = fma(xr, Z.re, rr[k]);
+                    ii[k] = fma(xi, Z.im, ii[k]);
+                    ri[k] = fma(xr, Z.im, ri[k]);
+                    ir[k] = fma(xi, Z.re, ir[k]);
                 }
                 if (i % 2 == 1) __builtin_amdgcn_sched_barrier(0);
             }
