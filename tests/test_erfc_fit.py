@@ -64,9 +64,9 @@ def fit_module():
 def source_coefficients64():
     text = (CSRC / "vmc_device.hpp").read_text()
     body = text[text.index("__device__ inline double erfc_fit64(double x)"):]
-    body = body[body.index("double q ="): body.index("return q;")]
-    vals = [float(v) for v in re.findall(r"([-+]?\d\.\d+(?:e[-+]\d+)?)\)?;", body)]
-    return vals[::-1]
+    body = body[body.index("constexpr double c[25] = {"): body.index("};")]
+    body = body[body.index("{") + 1:]
+    return [float(v) for v in body.replace("\n", " ").split(",")]
 
 
 def test_fp64_coefficients_are_the_fit():
