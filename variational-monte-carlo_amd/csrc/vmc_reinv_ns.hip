@@ -372,37 +372,62 @@ struct NSGShared {
     double red[2 * kNSWaves];
 };
 
-template <int NT, typename LA, typename LB>
-__device__ __forceinline__ void nsg_mm(LA load_a, LB load_b, ns_v4d acc[(NT / 2) * (NT / 2)], int wv,
-                                       int lane) {
+// The wave's output tiles r0 .. r0 + H - 1 by columns c0 + CB .. c0 + CB +
+// NC - 1 (acc[u * NC + v]).
+template <int NT, int CB, int NC, typename LA, typename LB>
+__device__ __forceinline__ void nsg_mm(LA load_a, LB load_b, ns_v4d acc[(NT / 2) * NC], int wv, int lane) {
     constexpr int H = NT / 2;
     const int lo = lane & 15, hi = lane >> 4;
-    const int r0 = (wv >> 1) * H, c0 = (wv & 1) * H;
+    const int r0 = (wv >> 1) * H, c0 = (wv & 1) * H + CB;
 #pragma unroll
-    for (int i = 0; i < H * H; ++i) acc[i] = ns_v4d{0.0, 0.0, 0.0, 0.0};
-    double a[H], b[H], an[H], bn[H];
+    for (int i = 0; i < H * NC; ++i) acc[i] = ns_v4d{0.0, 0.0, 0.0, 0.0};
+    double a[H], b[NC], an[H], bn[NC];
 #pragma unroll
-    for (int u = 0; u < H; ++u) {
-        a[u] = load_a(16 * (r0 + u) + lo, hi);
-        b[u] = load_b(hi, 16 * (c0 + u) + lo);
-    }
+    for (int u = 0; u < H; ++u) a[u] = load_a(16 * (r0 + u) + lo, hi);
+#pragma unroll
+    for (int v = 0; v < NC; ++v) b[v] = load_b(hi, 16 * (c0 + v) + lo);
 #pragma unroll 1
     for (int kk = 0; kk < 4 * NT; ++kk) {
         const int kn = 4 * (kk + 1 < 4 * NT ? kk + 1 : kk) + hi;
 #pragma unroll
-        for (int u = 0; u < H; ++u) {
-            an[u] = load_a(16 * (r0 + u) + lo, kn);
-            bn[u] = load_b(kn, 16 * (c0 + u) + lo);
-        }
+        for (int u = 0; u < H; ++u) an[u] = load_a(16 * (r0 + u) + lo, kn);
+#pragma unroll
+        for (int v = 0; v < NC; ++v) bn[v] = load_b(kn, 16 * (c0 + v) + lo);
 #pragma unroll
         for (int u = 0; u < H; ++u)
 #pragma unroll
-            for (int v = 0; v < H; ++v)
-                acc[u * H + v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[v], acc[u * H + v], 0, 0, 0);
+            for (int v = 0; v < NC; ++v)
+                acc[u * NC + v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[v], acc[u * NC + v], 0, 0, 0);
 #pragma unroll
-        for (int u = 0; u < H; ++u) {
-            a[u] = an[u];
-            b[u] = bn[u];
+        for (int u = 0; u < H; ++u) a[u] = an[u];
+#pragma unroll
+        for (int v = 0; v < NC; ++v) b[v] = bn[v];
+    }
+}
+
+// Z = E Y over the wave's column tiles CB .. CB + NC - 1 (nsg_mm), then
+// their part of R = I - Z (FP32, row pitch P) and of max|R| over the n x n
+// block into dm.
+template <int NT, int CB, int NC, typename LA, typename LB>
+__device__ __forceinline__ void nsg_resid(LA load_e, LB load_y, float *R, int n, int wv, int lane, double &dm) {
+    constexpr int H = NT / 2, P = 16 * NT;
+    ns_v4d z[H * NC];
+    nsg_mm<NT, CB, NC>(load_e, load_y, z, wv, lane);
+    // This lane's first row and column, tied to the product's result:
+    // otherwise every (tile, register) address and bound test below is
+    // formed before the product's k loop and held in registers through it.
+    int rl = 16 * ((wv >> 1) * H) + (lane >> 4), cl = 16 * ((wv & 1) * H + CB) + (lane & 15);
+    const double z0 = z[0][0];
+    asm volatile("" : "+v"(rl), "+v"(cl) : "v"(z0));
+#pragma unroll
+    for (int i = 0; i < H * NC; ++i) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = rl + 16 * (i / NC) + 4 * q;
+            const int c = cl + 16 * (i % NC);
+            const double dv = (r == c ? 1.0 : 0.0) - z[i][q];
+            if (r < n && c < n) dm = (fabs(dv) > dm || dv != dv) ? fabs(dv) : dm;
+            R[r * P + c] = (float)dv;
         }
     }
 }
@@ -481,24 +506,16 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
         bool ok = false, series = false;
 #pragma unroll 1
         for (int it = 0; it < kNSSteps; ++it) {
-            ns_v4d z[H * H];
-            nsg_mm<NT>(load_e, load_y, z, wv, lane);
-            // This lane's first row and column, opaque to loop-invariant code
-            // motion: otherwise every (tile, register) address and bound test
-            // below is hoisted out of the step loop and held in registers.
-            int rl = 16 * r0 + (lane >> 4), cl = 16 * c0 + (lane & 15);
-            asm volatile("" : "+v"(rl), "+v"(cl));
+            // Up to 4 x 4 tiles per wave in one pass; 5 x 5 (200 accumulator
+            // registers in FP64) in two, which keeps them out of the
+            // accumulation registers' copies around every MFMA.
             double dm = 0.0;
-#pragma unroll
-            for (int i = 0; i < H * H; ++i) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int r = rl + 16 * (i / H) + 4 * q;
-                    const int c = cl + 16 * (i % H);
-                    const double dv = (r == c ? 1.0 : 0.0) - z[i][q];
-                    if (r < n && c < n) dm = (fabs(dv) > dm || dv != dv) ? fabs(dv) : dm;
-                    R[r * P + c] = (float)dv;
-                }
+            if constexpr (H <= 4) {
+                nsg_resid<NT, 0, H>(load_e, load_y, R, n, wv, lane, dm);
+            } else {
+                constexpr int HA = (H + 1) / 2;
+                nsg_resid<NT, 0, HA>(load_e, load_y, R, n, wv, lane, dm);
+                nsg_resid<NT, HA, H - HA>(load_e, load_y, R, n, wv, lane, dm);
             }
             dm = block_max(dm, sh.red);  // its barriers also publish R
             if (it == 0) d0 = dm;
@@ -520,7 +537,7 @@ __device__ __forceinline__ void reinv_nsg_body(const DevSys &s, const DevState &
             ns_v4f y[H * H];
             nsg_mm32<NT>(load_y32, load_r32, y, wv, lane);
             // first row of this lane's FP32-layout entries, and its column
-            int rl32 = 16 * r0 + 4 * (lane >> 4), cl32 = cl;
+            int rl32 = 16 * r0 + 4 * (lane >> 4), cl32 = 16 * c0 + (lane & 15);
             asm volatile("" : "+v"(rl32), "+v"(cl32));
             const bool exact = Y64 && ns_exact_from(n, dm);  // this step's FP64 result is the output
 #pragma unroll
